@@ -1,0 +1,873 @@
+// tl_api.cpp -- host orchestration behind the C ABI: device contexts, one-time
+// constant upload, the plain per-model launch loop, the persistent batch
+// (captured in a hipGraph), the launch-shape autotuner, moments and the
+// stage entry points.
+//
+// Replaces the reference's driver loop (tau_leaping/tau_leaping.cpp:127-150:
+// Clone n-1 times, InitSeeds, PerformCalc per realization) and the statistics
+// stage (tau_leaping/statistics.cpp:15-111).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "tl_internal.hpp"
+#include "tl_model.hpp"
+
+namespace tl {
+
+// --------------------------------------------------------------- errors --
+static thread_local std::string g_last_error;
+
+tl_status set_error(tl_status st, const std::string &msg) {
+    g_last_error = msg;
+    return st;
+}
+
+// ------------------------------------------------------------- contexts --
+namespace {
+
+struct Ctx {
+    DeviceContext dc;
+    std::vector<std::pair<void *, size_t>> scratch;
+};
+
+std::mutex g_ctx_mu;
+std::map<int, std::unique_ptr<Ctx>> g_ctx;
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+tl_status device_context(int device, DeviceContext *&out) {
+    std::lock_guard<std::mutex> g(g_ctx_mu);
+    auto it = g_ctx.find(device);
+    if (it != g_ctx.end()) {
+        TL_HIP(hipSetDevice(device));
+        out = &it->second->dc;
+        return TL_OK;
+    }
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return set_error(TL_ERR_NO_DEVICE, "no HIP device available");
+    if (device < 0 || device >= n)
+        return set_error(TL_ERR_INVALID_ARG, "device " + std::to_string(device) + " out of range");
+    TL_HIP(hipSetDevice(device));
+    auto c = std::make_unique<Ctx>();
+    c->dc.device = device;
+    TL_HIP(hipStreamCreateWithFlags(&c->dc.stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    TL_HIP(hipGetDeviceProperties(&prop, device));
+    c->dc.n_cu = prop.multiProcessorCount;
+    c->dc.lds_per_block = prop.sharedMemPerBlock;
+    out = &c->dc;
+    g_ctx[device] = std::move(c);
+    return TL_OK;
+}
+
+tl_status scratch(DeviceContext &dc, int slot, size_t bytes, void *&ptr) {
+    std::lock_guard<std::mutex> g(g_ctx_mu);
+    Ctx &c = *g_ctx.at(dc.device);
+    if ((int)c.scratch.size() <= slot) c.scratch.resize(slot + 1, {nullptr, 0});
+    auto &s = c.scratch[slot];
+    if (s.second < bytes) {
+        if (s.first) TL_HIP(hipFree(s.first));
+        s.first = nullptr;
+        s.second = 0;
+        TL_HIP(hipMalloc(&s.first, bytes));
+        s.second = bytes;
+    }
+    ptr = s.first;
+    return TL_OK;
+}
+
+// -------------------------------------------------------- model upload --
+void release_device_copies(tl_model &m) {
+    for (auto &kv : m.dev) {
+        if (kv.second->buf) {
+            hipSetDevice(kv.first);
+            hipFree(kv.second->buf);
+        }
+        delete kv.second;
+    }
+    m.dev.clear();
+}
+
+tl_status device_model(const tl_model &m, int device, DevModel &out, double *upload_seconds) {
+    std::lock_guard<std::mutex> g(m.dev_mu);
+    auto it = m.dev.find(device);
+    if (it != m.dev.end() && it->second->version == m.version) {
+        out = it->second->dm;
+        if (upload_seconds) *upload_seconds = 0.0;
+        return TL_OK;
+    }
+    const double t0 = now_s();
+    // one allocation: c64 | c32 | s0 | s1 | row_ptr | csr | col_ptr | csc | x0 | order | g_order
+    const size_t M = (size_t)m.M, N = (size_t)m.N, Z = (size_t)std::max(m.nnz, 1);
+    std::vector<size_t> off;
+    size_t total = 0;
+    auto add = [&](size_t bytes) {
+        off.push_back(total);
+        total += (bytes + 255) & ~(size_t)255;
+    };
+    add(M * 8);
+    add(M * 4);
+    add(M * 4);
+    add(M * 4);
+    add((N + 1) * 4);
+    add(Z * 4);
+    add((M + 1) * 4);
+    add(Z * 4);
+    add(N * 4);
+    add(M);
+    add(N);
+    std::vector<unsigned char> host(total, 0);
+    std::vector<float> c32(M);
+    for (size_t j = 0; j < M; ++j) c32[j] = (float)m.rates[j];
+    memcpy(&host[off[0]], m.rates.data(), M * 8);
+    memcpy(&host[off[1]], c32.data(), M * 4);
+    memcpy(&host[off[2]], m.s0.data(), M * 4);
+    memcpy(&host[off[3]], m.s1.data(), M * 4);
+    memcpy(&host[off[4]], m.row_ptr.data(), (N + 1) * 4);
+    if (m.nnz) memcpy(&host[off[5]], m.csr.data(), (size_t)m.nnz * 4);
+    memcpy(&host[off[6]], m.col_ptr.data(), (M + 1) * 4);
+    if (m.nnz) memcpy(&host[off[7]], m.csc.data(), (size_t)m.nnz * 4);
+    memcpy(&host[off[8]], m.x0.data(), N * 4);
+    memcpy(&host[off[9]], m.order.data(), M);
+    memcpy(&host[off[10]], m.g_order.data(), N);
+
+    DeviceModelCache *c;
+    if (it != m.dev.end()) {
+        c = it->second;
+    } else {
+        c = new DeviceModelCache();
+        c->device = device;
+        m.dev[device] = c;
+    }
+    if (c->bytes < total) {
+        if (c->buf) TL_HIP(hipFree(c->buf));
+        c->buf = nullptr;
+        c->bytes = 0;
+        TL_HIP(hipMalloc(&c->buf, total));
+        c->bytes = total;
+    }
+    TL_HIP(hipMemcpy(c->buf, host.data(), total, hipMemcpyHostToDevice));
+    unsigned char *b = (unsigned char *)c->buf;
+    DevModel &d = c->dm;
+    d.N = m.N;
+    d.M = m.M;
+    d.nnz = m.nnz;
+    d.c64 = (const double *)(b + off[0]);
+    d.c32 = (const float *)(b + off[1]);
+    d.s0 = (const int32_t *)(b + off[2]);
+    d.s1 = (const int32_t *)(b + off[3]);
+    d.row_ptr = (const int32_t *)(b + off[4]);
+    d.csr = (const int32_t *)(b + off[5]);
+    d.col_ptr = (const int32_t *)(b + off[6]);
+    d.csc = (const int32_t *)(b + off[7]);
+    d.x0 = (const int32_t *)(b + off[8]);
+    d.order = (const int8_t *)(b + off[9]);
+    d.g_order = (const int8_t *)(b + off[10]);
+    c->version = m.version;
+    out = d;
+    if (upload_seconds) *upload_seconds = now_s() - t0;
+    return TL_OK;
+}
+
+namespace {
+
+// RAII device buffers for one call.
+struct DevBufs {
+    std::vector<void *> ptrs;
+    ~DevBufs() {
+        for (void *p : ptrs) hipFree(p);
+    }
+    template <class T>
+    tl_status alloc(T *&p, size_t n) {
+        void *v = nullptr;
+        hipError_t e = hipMalloc(&v, std::max<size_t>(n, 1) * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return set_error(TL_ERR_ALLOC, std::string("hipMalloc(") + std::to_string(n * sizeof(T)) +
+                                               " bytes): " + hipGetErrorString(e));
+        }
+        ptrs.push_back(v);
+        p = (T *)v;
+        return TL_OK;
+    }
+};
+
+#define TL_TRY(expr)                      \
+    do {                                  \
+        tl_status st_ = (expr);           \
+        if (st_ != TL_OK) return st_;     \
+    } while (0)
+
+tl_status check_params(const tl_model *m, const tl_sim_params *p) {
+    if (!m || !p) return set_error(TL_ERR_INVALID_ARG, "null model or params");
+    if (!(p->t_final > 0.0) || !std::isfinite(p->t_final)) return set_error(TL_ERR_INVALID_ARG, "t_final must be > 0");
+    if (!(p->eps > 0.0)) return set_error(TL_ERR_INVALID_ARG, "eps must be > 0");
+    if (p->n_realizations < 1) return set_error(TL_ERR_INVALID_ARG, "n_realizations must be >= 1");
+    if (p->precision != TL_FP64 && p->precision != TL_FP32) return set_error(TL_ERR_INVALID_ARG, "bad precision");
+    if (p->n_track < 0 || p->n_track > m->N) return set_error(TL_ERR_INVALID_ARG, "n_track out of range");
+    if (p->n_track > 0 && !p->track) return set_error(TL_ERR_INVALID_ARG, "track is NULL");
+    for (int32_t q = 0; q < p->n_track; ++q)
+        if (p->track[q] < 0 || p->track[q] >= m->N) return set_error(TL_ERR_INVALID_ARG, "tracked species out of range");
+    if (p->n_intervals < 0) return set_error(TL_ERR_INVALID_ARG, "n_intervals must be >= 0");
+    if (p->model_id >= (1u << 30)) return set_error(TL_ERR_INVALID_ARG, "model_id must be < 2^30");
+    return TL_OK;
+}
+
+std::vector<int32_t> track_list(const tl_model *m, const tl_sim_params *p) {
+    std::vector<int32_t> t;
+    if (p->n_track > 0)
+        t.assign(p->track, p->track + p->n_track);
+    else
+        for (int32_t i = 0; i < m->N; ++i) t.push_back(i);
+    return t;
+}
+
+// mean and sample standard deviation (n-1) of [R][T] samples, in FP64
+// (StepStat2StdDev, statistics.cpp:67-87).
+template <class T>
+void moments(const T *x, int64_t R, int32_t nT, double *mean, double *sd) {
+    std::vector<double> s(nT, 0.0), s2(nT, 0.0);
+    for (int64_t r = 0; r < R; ++r)
+        for (int32_t q = 0; q < nT; ++q) s[q] += (double)x[r * nT + q];
+    for (int32_t q = 0; q < nT; ++q) s[q] /= (double)R;
+    for (int64_t r = 0; r < R; ++r)
+        for (int32_t q = 0; q < nT; ++q) {
+            const double d = (double)x[r * nT + q] - s[q];
+            s2[q] += d * d;
+        }
+    for (int32_t q = 0; q < nT; ++q) {
+        if (mean) mean[q] = s[q];
+        if (sd) sd[q] = R > 1 ? std::sqrt(s2[q] / (double)(R - 1)) : 0.0;
+    }
+}
+
+void fill_summary(tl_result *res, const std::vector<int64_t> &steps, const std::vector<int64_t> &ssa,
+                  const std::vector<int64_t> &rej, const std::vector<int32_t> &status) {
+    res->total_steps = 0;
+    res->total_ssa_events = 0;
+    res->total_rejections = 0;
+    res->n_failed = 0;
+    res->first_failure = TL_OK;
+    for (size_t r = 0; r < steps.size(); ++r) {
+        res->total_steps += steps[r];
+        res->total_ssa_events += ssa[r];
+        res->total_rejections += rej[r];
+        if (status[r] != TL_OK) {
+            if (res->n_failed == 0) res->first_failure = status[r];
+            res->n_failed++;
+        }
+    }
+    if (res->steps) std::copy(steps.begin(), steps.end(), res->steps);
+}
+
+// ------------------------------------------------------------ plain loop --
+tl_status run_plain(const tl_model *m, const tl_sim_params *p, tl_result *res) {
+    DeviceContext *dc;
+    TL_TRY(device_context(p->device, dc));
+    hipStream_t s = dc->stream;
+    DevModel dm;
+    double up = 0.0;
+    TL_TRY(device_model(*m, p->device, dm, &up));
+    if (p->sensitivities) return set_error(TL_ERR_INVALID_ARG, "sensitivities need the persistent form");
+    if (p->n_intervals > 0) return set_error(TL_ERR_INVALID_ARG, "the time grid needs the persistent form");
+    const bool is64 = p->precision == TL_FP64;
+    const int64_t R = p->n_realizations;
+    const size_t rs = is64 ? 8 : 4;
+    const std::vector<int32_t> trk = track_list(m, p);
+    const int32_t T = (int32_t)trk.size();
+    // channel chunks so that the propensity pass has >= ~256k threads
+    int32_t nchunk = (int32_t)std::min<int64_t>(std::max<int64_t>(1, (262144 + R - 1) / R), std::max(1, m->M / 64));
+    DevBufs B;
+    int32_t *x, *xn, *k, *d_track, *d_outx;
+    void *a, *tau_i;
+    uint8_t *crit, *which_i;
+    double *a0, *a0c, *tau1, *part = nullptr, *partc = nullptr;
+    TL_TRY(B.alloc(x, (size_t)m->N * R));
+    TL_TRY(B.alloc(xn, (size_t)m->N * R));
+    TL_TRY(B.alloc(k, (size_t)m->M * R));
+    unsigned char *araw, *tauraw;
+    TL_TRY(B.alloc(araw, (size_t)m->M * R * rs));
+    TL_TRY(B.alloc(tauraw, (size_t)m->N * R * rs));
+    a = araw;
+    tau_i = tauraw;
+    TL_TRY(B.alloc(crit, (size_t)m->M * R));
+    TL_TRY(B.alloc(which_i, (size_t)m->N * R));
+    TL_TRY(B.alloc(a0, R));
+    TL_TRY(B.alloc(a0c, R));
+    TL_TRY(B.alloc(tau1, R));
+    if (nchunk > 1) {
+        TL_TRY(B.alloc(part, (size_t)nchunk * R));
+        TL_TRY(B.alloc(partc, (size_t)nchunk * R));
+    }
+    TL_TRY(B.alloc(d_track, T));
+    TL_TRY(B.alloc(d_outx, (size_t)T * R));
+    PlainState st;
+    TL_TRY(B.alloc(st.t, R));
+    TL_TRY(B.alloc(st.tau1_used, R));
+    TL_TRY(B.alloc(st.override_tau1, R));
+    TL_TRY(B.alloc(st.tau, R));
+    TL_TRY(B.alloc(st.epoch, R));
+    TL_TRY(B.alloc(st.steps, R));
+    TL_TRY(B.alloc(st.ssa, R));
+    TL_TRY(B.alloc(st.rej, R));
+    TL_TRY(B.alloc(st.status, R));
+    TL_TRY(B.alloc(st.retries, R));
+    TL_TRY(B.alloc(st.jc, R));
+    TL_TRY(B.alloc(st.active, R));
+    TL_TRY(B.alloc(st.mode, R));
+    TL_TRY(B.alloc(st.clamped, R));
+    TL_TRY(B.alloc(st.neg, R));
+    TL_TRY(B.alloc(st.n_done, 1));
+    TL_HIP(hipMemcpyAsync(d_track, trk.data(), T * 4, hipMemcpyHostToDevice, s));
+
+    hipEvent_t e0, e1;
+    TL_HIP(hipEventCreate(&e0));
+    TL_HIP(hipEventCreate(&e1));
+    const double t0 = now_s();
+    TL_HIP(hipEventRecord(e0, s));
+    launch_fill_u64(st.n_done, 0ull, 1, s);
+    launch_plain_init(dm, R, x, st, s);
+    unsigned long long *h_done = nullptr;
+    TL_HIP(hipHostMalloc((void **)&h_done, sizeof(unsigned long long)));
+    *h_done = 0;
+    const uint64_t seed = p->seed;
+    for (int64_t it = 0;; ++it) {
+        launch_prop(is64, dm, R, nchunk, x, a, crit, a0, a0c, part, partc, st.active, s);
+        launch_tau(is64, dm, R, p->eps, x, a, crit, tau_i, which_i, tau1, nullptr, nullptr, st.active, s);
+        launch_plain_select(is64, dm, R, seed, p->model_id, p->realization_offset, p->t_final, p->max_iter, x, a, crit,
+                            a0, a0c, tau1, st, s);
+        launch_fire(is64, dm, R, seed, p->model_id, p->realization_offset, st.epoch, a, crit, st.tau, st.jc, k, st.mode,
+                    s);
+        launch_update(dm, R, x, k, xn, st.neg, st.mode, s);
+        launch_plain_commit(dm, R, x, xn, st, p->t_final, s);
+        if ((it & 15) == 15) {
+            TL_HIP(hipMemcpyAsync(h_done, st.n_done, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+            TL_HIP(hipStreamSynchronize(s));
+            if ((int64_t)*h_done >= R) break;
+        }
+    }
+    TL_HIP(hipEventRecord(e1, s));
+    launch_gather_track(R, T, d_track, x, d_outx, s);
+    std::vector<int32_t> hx((size_t)T * R), hst(R);
+    std::vector<int64_t> hsteps(R), hssa(R), hrej(R);
+    TL_HIP(hipMemcpyAsync(hx.data(), d_outx, hx.size() * 4, hipMemcpyDeviceToHost, s));
+    TL_HIP(hipMemcpyAsync(hsteps.data(), st.steps, R * 8, hipMemcpyDeviceToHost, s));
+    TL_HIP(hipMemcpyAsync(hssa.data(), st.ssa, R * 8, hipMemcpyDeviceToHost, s));
+    TL_HIP(hipMemcpyAsync(hrej.data(), st.rej, R * 8, hipMemcpyDeviceToHost, s));
+    TL_HIP(hipMemcpyAsync(hst.data(), st.status, R * 4, hipMemcpyDeviceToHost, s));
+    TL_HIP(hipStreamSynchronize(s));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipHostFree(h_done);
+    if (res) {
+        res->kernel_ms = ms;
+        res->upload_seconds = up;
+        res->block_size = 256;
+        if (res->final_x) std::copy(hx.begin(), hx.end(), res->final_x);
+        moments(hx.data(), R, T, res->mean, res->stddev);
+        if (res->grid_times) res->grid_times[0] = p->t_final;
+        fill_summary(res, hsteps, hssa, hrej, hst);
+        res->sim_seconds = now_s() - t0;
+    }
+    return TL_OK;
+}
+
+// ------------------------------------------------------ persistent batch --
+struct Entry {
+    const tl_model *m;
+    const tl_sim_params *p;
+    tl_result *res;
+    DevModel dm;
+    std::vector<int32_t> track;
+    double cost;
+    // device outputs
+    int32_t *d_track, *d_outx, *d_status;
+    int64_t *d_steps, *d_ssa, *d_rej;
+    double *d_score, *d_dtime, *d_grid, *d_dmean, *d_dse, *d_mean;
+};
+
+struct TuneKey {
+    bool is64, sens;
+    int32_t n_max, m_max;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(is64, sens, n_max, m_max) < std::tie(o.is64, o.sens, o.n_max, o.m_max);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, PersistentConfig> g_tuned;
+
+std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, bool sens, int32_t n_max, int32_t m_max) {
+    std::vector<PersistentConfig> out;
+    const int shapes[][2] = {{64, 64}, {256, 64}, {256, 256}, {512, 512}, {1024, 1024}};
+    for (auto &sh : shapes)
+        for (int al = 1; al >= 0; --al) {
+            PersistentConfig c{sh[0], sh[1], al != 0};
+            const size_t lds = persistent_lds_bytes(c, is64, n_max, m_max);
+            if (lds > dc.lds_per_block) continue;
+            if (persistent_occupancy(c, is64, sens, lds) <= 0) continue;
+            out.push_back(c);
+        }
+    return out;
+}
+
+// Default shape without timing: one wavefront per realization unless the
+// model has many channels; propensities in LDS when they fit.
+PersistentConfig heuristic(const std::vector<PersistentConfig> &cands, int32_t m_max) {
+    auto pick = [&](int b, int g) -> const PersistentConfig * {
+        for (auto &c : cands)
+            if (c.block == b && c.group == g) return &c;  // a_lds variants come first
+        return nullptr;
+    };
+    const PersistentConfig *c = nullptr;
+    if (m_max <= 4096) c = pick(256, 64);
+    if (!c && m_max <= 16384) c = pick(256, 256);
+    if (!c) c = pick(1024, 1024);
+    if (!c) c = pick(512, 512);
+    if (!c && !cands.empty()) c = &cands.front();
+    return c ? *c : PersistentConfig{0, 0, false};
+}
+
+tl_status launch_batch(DeviceContext &dc, const PersistentConfig &cfg, bool is64, bool sens, ModelItem *d_items,
+                       int32_t n_models, int64_t n_items, int32_t n_max, int32_t m_max, unsigned long long *d_counter,
+                       int64_t grid_cap, bool use_graph, float *ms_out) {
+    hipStream_t s = dc.stream;
+    const size_t lds = persistent_lds_bytes(cfg, is64, n_max, m_max);
+    const int occ = std::max(1, persistent_occupancy(cfg, is64, sens, lds));
+    const int ng = cfg.block / cfg.group;
+    int64_t grid = (int64_t)dc.n_cu * occ;
+    grid = std::min<int64_t>(grid, (n_items + ng - 1) / ng);
+    if (grid_cap > 0) grid = std::min(grid, grid_cap);
+    grid = std::max<int64_t>(grid, 1);
+    PersistentLaunch L;
+    L.items = d_items;
+    L.n_models = n_models;
+    L.n_items = n_items;
+    L.n_max = n_max;
+    L.m_max = m_max;
+    L.counter = d_counter;
+    L.slots = grid * ng;
+    L.grid = (int)grid;
+    L.lds_bytes = lds;
+    L.a_glob = nullptr;
+    if (!cfg.a_lds) TL_TRY(scratch(dc, 0, (size_t)L.slots * m_max * (is64 ? 8 : 4), L.a_glob));
+    hipEvent_t e0, e1;
+    TL_HIP(hipEventCreate(&e0));
+    TL_HIP(hipEventCreate(&e1));
+    if (use_graph) {
+        hipGraph_t gph;
+        hipGraphExec_t gx;
+        TL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        launch_fill_u64(d_counter, 0ull, 1, s);
+        const hipError_t le = launch_persistent(cfg, is64, sens, L, s);
+        TL_HIP(hipStreamEndCapture(s, &gph));
+        TL_HIP(le);
+        TL_HIP(hipGraphInstantiate(&gx, gph, nullptr, nullptr, 0));
+        TL_HIP(hipEventRecord(e0, s));
+        TL_HIP(hipGraphLaunch(gx, s));
+        TL_HIP(hipEventRecord(e1, s));
+        TL_HIP(hipStreamSynchronize(s));
+        hipGraphExecDestroy(gx);
+        hipGraphDestroy(gph);
+    } else {
+        launch_fill_u64(d_counter, 0ull, 1, s);
+        TL_HIP(hipEventRecord(e0, s));
+        TL_HIP(launch_persistent(cfg, is64, sens, L, s));
+        TL_HIP(hipEventRecord(e1, s));
+        TL_HIP(hipStreamSynchronize(s));
+    }
+    TL_HIP(hipGetLastError());
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (ms_out) *ms_out = ms;
+    return TL_OK;
+}
+
+// Time each launch shape on a bounded sample (first few steps of a subset
+// of the work items) and keep the fastest in realization-steps per second.
+tl_status autotune(DeviceContext &dc, bool is64, bool sens, const std::vector<ModelItem> &items_in, int64_t n_items,
+                   int32_t n_max, int32_t m_max, unsigned long long *d_counter, int64_t *d_steps_all,
+                   PersistentConfig &best) {
+    TuneKey key{is64, sens, n_max, m_max};
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            best = it->second;
+            return TL_OK;
+        }
+    }
+    auto cands = candidates(dc, is64, sens, n_max, m_max);
+    if (cands.empty()) return set_error(TL_ERR_TOO_LARGE, "model too large for any persistent launch shape");
+    best = heuristic(cands, m_max);
+    const int64_t sample = std::min<int64_t>(n_items, (int64_t)dc.n_cu * 16);
+    if (cands.size() > 1 && n_items >= 4 * sample) {
+        std::vector<ModelItem> trial = items_in;
+        for (auto &mi : trial) mi.max_iter = (mi.max_iter > 0) ? std::min<int64_t>(mi.max_iter, 24) : 24;
+        ModelItem *d_trial = nullptr;
+        TL_HIP(hipMalloc(&d_trial, trial.size() * sizeof(ModelItem)));
+        hipMemcpy(d_trial, trial.data(), trial.size() * sizeof(ModelItem), hipMemcpyHostToDevice);
+        double best_rate = -1.0;
+        for (auto &c : cands) {
+            float ms = 0.f;
+            tl_status st = launch_batch(dc, c, is64, sens, d_trial, (int32_t)trial.size(), sample, n_max, m_max,
+                                        d_counter, 0, false, &ms);
+            if (st != TL_OK) continue;
+            // steps of the sampled items: items 0..sample-1 belong to the first models
+            int64_t done = 0;
+            int64_t left = sample;
+            for (auto &mi : trial) {
+                if (left <= 0) break;
+                const int64_t take = std::min<int64_t>(left, mi.R);
+                std::vector<int64_t> h(take);
+                hipMemcpy(h.data(), mi.out_steps, take * 8, hipMemcpyDeviceToHost);
+                for (auto v : h) done += v;
+                left -= take;
+            }
+            const double rate = ms > 0 ? (double)done / ms : 0.0;
+            if (rate > best_rate) {
+                best_rate = rate;
+                best = c;
+            }
+        }
+        hipFree(d_trial);
+    }
+    (void)d_steps_all;
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tuned[key] = best;
+    return TL_OK;
+}
+
+tl_status run_persistent(const tl_model *const *models, int32_t n_models, const tl_sim_params *params,
+                         tl_result *results) {
+    const int device = params[0].device;
+    const bool is64 = params[0].precision == TL_FP64;
+    const bool sens = params[0].sensitivities != 0;
+    for (int32_t i = 0; i < n_models; ++i) {
+        TL_TRY(check_params(models[i], &params[i]));
+        if ((params[i].precision == TL_FP64) != is64 || (params[i].sensitivities != 0) != sens ||
+            params[i].device != device)
+            return set_error(TL_ERR_INVALID_ARG, "precision, sensitivities and device must agree across the batch");
+    }
+    DeviceContext *dc;
+    TL_TRY(device_context(device, dc));
+    hipStream_t s = dc->stream;
+    DevBufs B;
+    std::vector<Entry> E(n_models);
+    double up_total = 0.0;
+    int32_t n_max = 0, m_max = 0;
+    for (int32_t i = 0; i < n_models; ++i) {
+        Entry &e = E[i];
+        e.m = models[i];
+        e.p = &params[i];
+        e.res = results ? &results[i] : nullptr;
+        double up = 0.0;
+        TL_TRY(device_model(*e.m, device, e.dm, &up));
+        up_total += up;
+        e.track = track_list(e.m, e.p);
+        n_max = std::max(n_max, e.m->N);
+        m_max = std::max(m_max, e.m->M);
+        e.cost = (double)e.p->n_realizations * (double)(e.m->M + e.m->nnz);
+        const int64_t R = e.p->n_realizations;
+        const int32_t T = (int32_t)e.track.size();
+        TL_TRY(B.alloc(e.d_track, T));
+        TL_TRY(B.alloc(e.d_outx, (size_t)R * T));
+        TL_TRY(B.alloc(e.d_steps, R));
+        TL_TRY(B.alloc(e.d_ssa, R));
+        TL_TRY(B.alloc(e.d_rej, R));
+        TL_TRY(B.alloc(e.d_status, R));
+        e.d_score = e.d_dtime = e.d_grid = e.d_dmean = e.d_dse = e.d_mean = nullptr;
+        if (sens) {
+            TL_TRY(B.alloc(e.d_score, (size_t)R * e.m->M));
+            TL_TRY(B.alloc(e.d_dtime, (size_t)R * e.m->M));
+            TL_TRY(B.alloc(e.d_dmean, (size_t)T * e.m->M));
+            TL_TRY(B.alloc(e.d_dse, (size_t)T * e.m->M));
+            TL_TRY(B.alloc(e.d_mean, T));
+        }
+        if (e.p->n_intervals > 0) {
+            const double bytes = (double)R * (e.p->n_intervals + 1) * T * 8.0;
+            if (bytes > 64e9) return set_error(TL_ERR_TOO_LARGE, "time-grid output exceeds 64 GB");
+            TL_TRY(B.alloc(e.d_grid, (size_t)R * (e.p->n_intervals + 1) * T));
+        }
+        TL_HIP(hipMemcpyAsync(e.d_track, e.track.data(), T * 4, hipMemcpyHostToDevice, s));
+    }
+    // work order: most expensive models first (better tail balance)
+    std::vector<int32_t> order(n_models);
+    for (int32_t i = 0; i < n_models; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return E[a].cost > E[b].cost; });
+    std::vector<ModelItem> items(n_models);
+    int64_t base = 0;
+    for (int32_t k = 0; k < n_models; ++k) {
+        Entry &e = E[order[k]];
+        ModelItem &mi = items[k];
+        mi.dm = e.dm;
+        mi.t_final = e.p->t_final;
+        mi.eps = e.p->eps;
+        mi.seed = e.p->seed;
+        mi.max_iter = e.p->max_iter;
+        mi.item_base = base;
+        mi.R = e.p->n_realizations;
+        mi.rid_offset = e.p->realization_offset;
+        mi.model_id = e.p->model_id;
+        mi.n_track = (int32_t)e.track.size();
+        mi.track = e.d_track;
+        mi.out_x = e.d_outx;
+        mi.out_steps = e.d_steps;
+        mi.out_ssa = e.d_ssa;
+        mi.out_rej = e.d_rej;
+        mi.out_status = e.d_status;
+        mi.score = e.d_score;
+        mi.dtime = e.d_dtime;
+        mi.n_intervals = e.p->n_intervals;
+        mi.grid = e.d_grid;
+        base += mi.R;
+    }
+    const int64_t n_items = base;
+    ModelItem *d_items;
+    unsigned long long *d_counter;
+    TL_TRY(B.alloc(d_items, n_models));
+    TL_TRY(B.alloc(d_counter, 1));
+    TL_HIP(hipMemcpyAsync(d_items, items.data(), n_models * sizeof(ModelItem), hipMemcpyHostToDevice, s));
+    TL_HIP(hipStreamSynchronize(s));
+
+    PersistentConfig cfg;
+    const int bs_req = params[0].block_size;
+    if (bs_req > 0) {
+        auto cands = candidates(*dc, is64, sens, n_max, m_max);
+        const int gs = bs_req <= 64 ? 64 : bs_req;
+        const int bs = bs_req <= 64 ? 256 : bs_req;
+        bool found = false;
+        for (auto &c : cands)
+            if (c.block == bs && c.group == gs) {
+                cfg = c;
+                found = true;
+                break;
+            }
+        if (!found)
+            for (auto &c : cands)
+                if (c.group == gs) {
+                    cfg = c;
+                    found = true;
+                    break;
+                }
+        if (!found) return set_error(TL_ERR_INVALID_ARG, "block_size " + std::to_string(bs_req) + " does not fit this model");
+    } else {
+        TL_TRY(autotune(*dc, is64, sens, items, n_items, n_max, m_max, d_counter, nullptr, cfg));
+    }
+
+    const double t0 = now_s();
+    float ms = 0.f;
+    TL_TRY(launch_batch(*dc, cfg, is64, sens, d_items, n_models, n_items, n_max, m_max, d_counter, 0, true, &ms));
+    for (auto &e : E)
+        if (sens && e.res && (e.res->dmean_dc || e.res->dmean_dc_se))
+            launch_grad_reduce(e.p->n_realizations, (int32_t)e.track.size(), e.m->M, e.d_outx, e.d_score, e.d_mean,
+                               e.d_dmean, e.d_dse, s);
+    TL_HIP(hipStreamSynchronize(s));
+    for (auto &e : E) {
+        if (!e.res) continue;
+        const int64_t R = e.p->n_realizations;
+        const int32_t T = (int32_t)e.track.size();
+        std::vector<int32_t> hx((size_t)R * T), hst(R);
+        std::vector<int64_t> hsteps(R), hssa(R), hrej(R);
+        TL_HIP(hipMemcpyAsync(hx.data(), e.d_outx, hx.size() * 4, hipMemcpyDeviceToHost, s));
+        TL_HIP(hipMemcpyAsync(hsteps.data(), e.d_steps, R * 8, hipMemcpyDeviceToHost, s));
+        TL_HIP(hipMemcpyAsync(hssa.data(), e.d_ssa, R * 8, hipMemcpyDeviceToHost, s));
+        TL_HIP(hipMemcpyAsync(hrej.data(), e.d_rej, R * 8, hipMemcpyDeviceToHost, s));
+        TL_HIP(hipMemcpyAsync(hst.data(), e.d_status, R * 4, hipMemcpyDeviceToHost, s));
+        if (sens && e.res->dmean_dc)
+            TL_HIP(hipMemcpyAsync(e.res->dmean_dc, e.d_dmean, (size_t)T * e.m->M * 8, hipMemcpyDeviceToHost, s));
+        if (sens && e.res->dmean_dc_se)
+            TL_HIP(hipMemcpyAsync(e.res->dmean_dc_se, e.d_dse, (size_t)T * e.m->M * 8, hipMemcpyDeviceToHost, s));
+        std::vector<double> hg;
+        const int32_t n_grid = e.p->n_intervals > 0 ? e.p->n_intervals + 1 : 1;
+        if (e.p->n_intervals > 0) {
+            hg.resize((size_t)R * n_grid * T);
+            TL_HIP(hipMemcpyAsync(hg.data(), e.d_grid, hg.size() * 8, hipMemcpyDeviceToHost, s));
+        }
+        TL_HIP(hipStreamSynchronize(s));
+        tl_result *res = e.res;
+        if (res->final_x) std::copy(hx.begin(), hx.end(), res->final_x);
+        if (e.p->n_intervals > 0) {
+            // per grid point over realizations: hg is [R][n_grid][T]
+            std::vector<double> slice((size_t)R * T);
+            for (int32_t g = 0; g < n_grid; ++g) {
+                for (int64_t r = 0; r < R; ++r)
+                    for (int32_t q = 0; q < T; ++q) slice[r * T + q] = hg[((size_t)r * n_grid + g) * T + q];
+                moments(slice.data(), R, T, res->mean ? res->mean + (size_t)g * T : nullptr,
+                        res->stddev ? res->stddev + (size_t)g * T : nullptr);
+                if (res->grid_times) res->grid_times[g] = e.p->t_final / e.p->n_intervals * g;
+            }
+        } else {
+            moments(hx.data(), R, T, res->mean, res->stddev);
+            if (res->grid_times) res->grid_times[0] = e.p->t_final;
+        }
+        fill_summary(res, hsteps, hssa, hrej, hst);
+        res->kernel_ms = ms;
+        res->upload_seconds = up_total;
+        res->block_size = cfg.group == 64 ? 64 : cfg.block;
+        res->sim_seconds = now_s() - t0;
+    }
+    return TL_OK;
+}
+
+}  // namespace
+}  // namespace tl
+
+using namespace tl;
+
+extern "C" {
+
+const char *tl_last_error(void) { return g_last_error.c_str(); }
+const char *tl_version(void) { return "tau_leaping_amd 0.1.0 (gfx950)"; }
+
+tl_status tl_device_count(int32_t *n) {
+    if (!n) return set_error(TL_ERR_INVALID_ARG, "null");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *n = c;
+    return TL_OK;
+}
+
+void tl_sim_params_default(tl_sim_params *p) {
+    if (!p) return;
+    memset(p, 0, sizeof(*p));
+    p->t_final = 1.0;
+    p->eps = 0.03;
+    p->n_realizations = 1;
+    p->seed = 1234;
+    p->max_iter = 0;
+    p->precision = TL_FP64;
+    p->mode = TL_MODE_AUTO;
+}
+
+tl_status tl_simulate(const tl_model *m, const tl_sim_params *p, tl_result *r) {
+    TL_TRY(check_params(m, p));
+    if (p->mode == TL_MODE_PLAIN) return run_plain(m, p, r);
+    return run_persistent(&m, 1, p, r);
+}
+
+tl_status tl_batch_simulate(const tl_model *const *models, int32_t n_models, const tl_sim_params *params,
+                            tl_result *results, int32_t mode) {
+    if (!models || !params || n_models <= 0) return set_error(TL_ERR_INVALID_ARG, "empty batch");
+    if (mode == TL_MODE_PLAIN) {
+        for (int32_t i = 0; i < n_models; ++i) {
+            TL_TRY(check_params(models[i], &params[i]));
+            TL_TRY(run_plain(models[i], &params[i], results ? &results[i] : nullptr));
+        }
+        return TL_OK;
+    }
+    return run_persistent(models, n_models, params, results);
+}
+
+// ---------------------------------------------------------------- stages --
+static tl_status stage_setup(const tl_model *m, int32_t precision, int64_t R, DeviceContext *&dc, DevModel &dm) {
+    if (!m) return set_error(TL_ERR_INVALID_ARG, "null model");
+    if (R < 1) return set_error(TL_ERR_INVALID_ARG, "R must be >= 1");
+    if (precision != TL_FP64 && precision != TL_FP32) return set_error(TL_ERR_INVALID_ARG, "bad precision");
+    int dev = 0;
+    hipGetDevice(&dev);
+    TL_TRY(device_context(dev, dc));
+    return device_model(*m, dev, dm, nullptr);
+}
+
+static tl_status stage_finish(DeviceContext *dc) {
+    TL_HIP(hipGetLastError());
+    TL_HIP(hipStreamSynchronize(dc->stream));
+    return TL_OK;
+}
+
+tl_status tl_stage_propensities(const tl_model *m, int32_t precision, int64_t R, const int32_t *x, void *a,
+                                uint8_t *crit, double *a0, double *a0c) {
+    DeviceContext *dc;
+    DevModel dm;
+    TL_TRY(stage_setup(m, precision, R, dc, dm));
+    if (!x || !a || !crit || !a0 || !a0c) return set_error(TL_ERR_INVALID_ARG, "null buffer");
+    launch_prop(precision == TL_FP64, dm, R, 1, x, a, crit, a0, a0c, nullptr, nullptr, nullptr, dc->stream);
+    return stage_finish(dc);
+}
+
+tl_status tl_stage_tau(const tl_model *m, int32_t precision, int64_t R, double eps, const int32_t *x, const void *a,
+                       const uint8_t *crit, double *tau1, int32_t *argmin, int32_t *which) {
+    DeviceContext *dc;
+    DevModel dm;
+    TL_TRY(stage_setup(m, precision, R, dc, dm));
+    if (!x || !a || !crit || !tau1) return set_error(TL_ERR_INVALID_ARG, "null buffer");
+    void *tau_i;
+    TL_TRY(scratch(*dc, 1, (size_t)m->N * R * (precision == TL_FP64 ? 8 : 4), tau_i));
+    void *which_i;
+    TL_TRY(scratch(*dc, 2, (size_t)m->N * R, which_i));
+    launch_tau(precision == TL_FP64, dm, R, eps, x, a, crit, tau_i, (uint8_t *)which_i, tau1, argmin, which, nullptr,
+               dc->stream);
+    return stage_finish(dc);
+}
+
+tl_status tl_stage_select(const tl_model *m, int32_t precision, int64_t R, uint64_t seed, uint32_t model_id,
+                          int64_t realization_offset, const uint32_t *epoch, const void *a, const uint8_t *crit,
+                          const double *a0c, const double *tau1, const double *t_rem, double *tau, int32_t *jc,
+                          uint8_t *clamped) {
+    DeviceContext *dc;
+    DevModel dm;
+    TL_TRY(stage_setup(m, precision, R, dc, dm));
+    if (!epoch || !a || !crit || !a0c || !tau1 || !t_rem || !tau || !jc || !clamped)
+        return set_error(TL_ERR_INVALID_ARG, "null buffer");
+    launch_select(precision == TL_FP64, dm, R, seed, model_id, realization_offset, epoch, a, crit, a0c, tau1, t_rem,
+                  tau, jc, clamped, dc->stream);
+    return stage_finish(dc);
+}
+
+tl_status tl_stage_fire(const tl_model *m, int32_t precision, int64_t R, uint64_t seed, uint32_t model_id,
+                        int64_t realization_offset, const uint32_t *epoch, const void *a, const uint8_t *crit,
+                        const double *tau, const int32_t *jc, int32_t *k) {
+    DeviceContext *dc;
+    DevModel dm;
+    TL_TRY(stage_setup(m, precision, R, dc, dm));
+    if (!epoch || !a || !crit || !tau || !jc || !k) return set_error(TL_ERR_INVALID_ARG, "null buffer");
+    launch_fire(precision == TL_FP64, dm, R, seed, model_id, realization_offset, epoch, a, crit, tau, jc, k, nullptr,
+                dc->stream);
+    return stage_finish(dc);
+}
+
+tl_status tl_stage_update(const tl_model *m, int64_t R, const int32_t *x, const int32_t *k, int32_t *x_out,
+                          uint8_t *neg) {
+    DeviceContext *dc;
+    DevModel dm;
+    TL_TRY(stage_setup(m, TL_FP64, R, dc, dm));
+    if (!x || !k || !x_out || !neg) return set_error(TL_ERR_INVALID_ARG, "null buffer");
+    launch_fill_u8(neg, 0, R, dc->stream);
+    launch_update(dm, R, x, k, x_out, neg, nullptr, dc->stream);
+    return stage_finish(dc);
+}
+
+tl_status tl_stage_ssa(const tl_model *m, int32_t precision, int64_t R, uint64_t seed, uint32_t model_id,
+                       int64_t realization_offset, int32_t *x, double *t, uint32_t *epoch, double t_final,
+                       int32_t n_events, int32_t *events) {
+    DeviceContext *dc;
+    DevModel dm;
+    TL_TRY(stage_setup(m, precision, R, dc, dm));
+    if (!x || !t || !epoch || !events) return set_error(TL_ERR_INVALID_ARG, "null buffer");
+    void *a;
+    TL_TRY(scratch(*dc, 3, (size_t)m->M * R * (precision == TL_FP64 ? 8 : 4), a));
+    launch_ssa(precision == TL_FP64, dm, R, seed, model_id, realization_offset, x, t, epoch, t_final, n_events, events,
+               a, dc->stream);
+    return stage_finish(dc);
+}
+
+}  // extern "C"

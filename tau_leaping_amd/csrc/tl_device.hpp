@@ -1,0 +1,193 @@
+// tl_device.hpp -- device-side building blocks shared by the stage kernels
+// and the persistent engine (gfx950).
+//
+// Reference formulas (file:line in the reference tree):
+//   propensity / criticality ... tau_leaping/functions.h:100-123 (nc = 5, model.cu:225)
+//   GetG ........................ tau_leaping/CSRMatrixPacked.cu:294-309
+//   tau_i ....................... tau_leaping/CSRMatrixPacked.cu:329-355
+//   Poisson / Box-Muller ........ tau_leaping/model.cu:477-520, functions.h:37-54
+//   packed entry ................ tau_leaping/functions.h:126-146
+// The reference draws from a per-channel hybrid Tausworthe state
+// (d_rng.cuh:27-35); here every draw comes from a Philox4x32-10 counter
+// stream keyed per realization, so draws never depend on launch shape.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tl {
+
+constexpr int kNc = 5;                 // criticality threshold
+constexpr double kSsaFactor = 10.0;    // direct branch when tau1 < 10/a0
+constexpr int kSsaSteps = 100;         // direct events per direct step
+constexpr int kPoissonSwitch = 80;     // Knuth below, normal approximation at or above
+constexpr int kKnuthCap = 300;         // Knuth returns 300 after 299 draws
+constexpr int kMaxRetries = 32;        // rejected leaps in a row before giving up
+constexpr float kFltMax = 3.402823466e+38f;
+
+constexpr uint32_t kKindPoisson = 0u;  // ctr[3] low bits: per-channel firing stream
+constexpr uint32_t kKindLeap = 1u;     //                  leap scalars (r1, r2)
+constexpr uint32_t kKindSsa = 2u;      //                  direct-event scalars
+
+// Status codes (mirror tl_api.h)
+constexpr int kOk = 0;
+constexpr int kRetryExhausted = 7;
+constexpr int kIterCap = 8;
+constexpr int kStalled = 12;
+
+// Model constants in device memory, shared by every realization.
+struct DevModel {
+    int32_t N, M, nnz;
+    const double *c64;       // [M] rates
+    const float *c32;        // [M] rates rounded once to float
+    const int8_t *order;     // [M] 1, 2, 3
+    const int32_t *s0;       // [M]
+    const int32_t *s1;       // [M]
+    const int32_t *row_ptr;  // [N+1] species-major
+    const int32_t *csr;      // [nnz] (channel << 3) | ((v + 2) & 7)
+    const int32_t *col_ptr;  // [M+1] channel-major
+    const int32_t *csc;      // [nnz] (species << 3) | ((v + 2) & 7)
+    const int8_t *g_order;   // [N] order[i] for i < M else 0
+    const int32_t *x0;       // [N]
+};
+
+// ---- Philox4x32-10 ----
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t lo0 = 0xD2511F53u * c.x;
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c.x);
+        const uint32_t lo1 = 0xCD9E8D57u * c.z;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z);
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+        k.x += 0x9E3779B9u;
+        k.y += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// key = (seed_lo, seed_hi + realization id)
+__device__ __forceinline__ uint2 rng_key(uint64_t seed, int64_t rid) {
+    return make_uint2((uint32_t)seed, (uint32_t)(seed >> 32) + (uint32_t)rid);
+}
+
+__device__ __forceinline__ double unif64(uint32_t b) {
+    return ((double)b + 0.5) * 2.3283064365386962890625e-10;
+}
+__device__ __forceinline__ float unif32(uint32_t b) {
+    return ((float)(b >> 9) + 0.5f) * 1.1920928955078125e-07f;
+}
+
+template <class R> struct Real;
+template <> struct Real<double> {
+    static __device__ __forceinline__ double unif(uint32_t b) { return unif64(b); }
+    static __device__ __forceinline__ double exp_(double v) { return exp(v); }
+    static __device__ __forceinline__ double log_(double v) { return log(v); }
+    static __device__ __forceinline__ double sqrt_(double v) { return sqrt(v); }
+    static __device__ __forceinline__ double fabs_(double v) { return fabs(v); }
+    static __device__ __forceinline__ const double *rates(const DevModel &m) { return m.c64; }
+};
+template <> struct Real<float> {
+    static __device__ __forceinline__ float unif(uint32_t b) { return unif32(b); }
+    static __device__ __forceinline__ float exp_(float v) { return expf(v); }
+    static __device__ __forceinline__ float log_(float v) { return logf(v); }
+    static __device__ __forceinline__ float sqrt_(float v) { return sqrtf(v); }
+    static __device__ __forceinline__ float fabs_(float v) { return fabsf(v); }
+    static __device__ __forceinline__ const float *rates(const DevModel &m) { return m.c32; }
+};
+
+// Propensity of one channel; operation order (c*x0)*x1 and ((c*x0)*(x0-1))/2
+// as in IsCriticalAndPropensity.
+template <class R>
+__device__ __forceinline__ R propensity(int order, R c, int32_t x0, int32_t x1, bool &crit) {
+    if (order == 1) {
+        crit = x0 < kNc;
+        return c * (R)x0;
+    } else if (order == 2) {
+        crit = (x0 < kNc) || (x1 < kNc);
+        return c * (R)x0 * (R)x1;
+    } else {
+        crit = x0 * 2 < kNc;
+        return c * (R)x0 * (R)(x0 - 1) / (R)2;
+    }
+}
+
+template <class R>
+__device__ __forceinline__ R getg(int order, int32_t x) {
+    if (order == 1) return (R)1;
+    if (order == 2) return (R)2;
+    if (order == 3) return (R)2 + (R)1 / ((R)x - (R)1);
+    return (R)0;
+}
+
+// tau_i from the row sums (mu, sigma^2) of species i with population xi.
+// Sets which = 0 when the mean bound is the smaller one, 1 otherwise.
+template <class R>
+__device__ __forceinline__ R tau_species(R eps, int32_t xi, int order_i, R mu, R sig, int &which) {
+    const R g = getg<R>(order_i, xi);
+    const R q = eps * (R)xi / g;
+    const R tmp = q > (R)1 ? q : (R)1;
+    const R t1 = tmp / Real<R>::fabs_(mu);
+    const R t2 = tmp * tmp / Real<R>::fabs_(sig);
+    which = (t1 <= t2) ? 0 : 1;
+    return t1 < t2 ? t1 : t2;
+}
+
+// Firing count for a non-critical channel: Knuth product below 80 (capped at
+// 300), else |mean + sqrt(mean) * z| truncated, z from the polar method.
+// Uniform n of the stream is output (n & 3) of Philox block n >> 2 at
+// counter (block, channel, epoch, ctr3).
+template <class R>
+__device__ __forceinline__ int32_t poisson_draw(R mean, uint2 key, uint32_t channel, uint32_t epoch,
+                                                uint32_t ctr3) {
+    uint32_t blk = 0;
+    if (mean < (R)kPoissonSwitch) {
+        const R lam = (R)1 / Real<R>::exp_(mean);
+        R val = (R)1;
+        int32_t n = 1;
+        for (;;) {
+            const uint4 o = philox4x32_10(make_uint4(blk++, channel, epoch, ctr3), key);
+            val *= Real<R>::unif(o.x);
+            if (val < lam) return n - 1;
+            if (++n == kKnuthCap) return kKnuthCap;
+            val *= Real<R>::unif(o.y);
+            if (val < lam) return n - 1;
+            if (++n == kKnuthCap) return kKnuthCap;
+            val *= Real<R>::unif(o.z);
+            if (val < lam) return n - 1;
+            if (++n == kKnuthCap) return kKnuthCap;
+            val *= Real<R>::unif(o.w);
+            if (val < lam) return n - 1;
+            if (++n == kKnuthCap) return kKnuthCap;
+        }
+    } else {
+        R u, s;
+        for (;;) {
+            const uint4 o = philox4x32_10(make_uint4(blk++, channel, epoch, ctr3), key);
+            u = 2 * Real<R>::unif(o.x) - 1;
+            R v = 2 * Real<R>::unif(o.y) - 1;
+            s = u * u + v * v;
+            if (s < 1) break;
+            u = 2 * Real<R>::unif(o.z) - 1;
+            v = 2 * Real<R>::unif(o.w) - 1;
+            s = u * u + v * v;
+            if (s < 1) break;
+        }
+        const R z = u * Real<R>::sqrt_(-2 * Real<R>::log_(s) / s);
+        const R y = mean + Real<R>::sqrt_(mean) * z;
+        return (int32_t)Real<R>::fabs_(y);
+    }
+}
+
+// Leap / direct-event scalars r1 (pick) and r2 (waiting time), FP64.
+__device__ __forceinline__ void step_uniforms(uint2 key, uint32_t epoch, uint32_t ctr3_kind, double &r1,
+                                              double &r2) {
+    const uint4 o = philox4x32_10(make_uint4(0u, 0u, epoch, ctr3_kind), key);
+    r1 = unif64(o.x);
+    r2 = unif64(o.y);
+}
+
+__device__ __forceinline__ int32_t entry_index(int32_t p) { return p >> 3; }
+__device__ __forceinline__ int32_t entry_value(int32_t p) { return (p & 7) - 2; }
+
+}  // namespace tl

@@ -1,0 +1,142 @@
+// tl_internal.hpp -- declarations shared by the host orchestration (tl_api.cpp)
+// and the HIP sources (tl_stages.hip, tl_persistent.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "tl_api.h"
+#include "tl_device.hpp"
+
+struct tl_model;
+
+namespace tl {
+
+tl_status set_error(tl_status st, const std::string &msg);
+void release_device_copies(tl_model &m);
+
+#define TL_HIP(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return ::tl::set_error(TL_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Device copy of a model's constants (one allocation per device).
+struct DeviceModelCache {
+    int device = -1;
+    uint64_t version = 0;
+    void *buf = nullptr;
+    size_t bytes = 0;
+    DevModel dm{};
+};
+
+// Upload (or reuse) the model's constants on the current device.
+tl_status device_model(const tl_model &m, int device, DevModel &out, double *upload_seconds);
+
+// Per-device context: one stream (default priority) and reusable scratch.
+struct DeviceContext {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int n_cu = 0;
+    size_t lds_per_block = 0;
+};
+tl_status device_context(int device, DeviceContext *&ctx);
+
+// Grow-only device scratch owned by the context.
+tl_status scratch(DeviceContext &ctx, int slot, size_t bytes, void *&ptr);
+
+// ---------------------------------------------------------------- plain --
+// Stage kernels over structure-of-arrays state ([element][R]); launched on
+// ctx.stream.  is64 selects double / float propensities.
+void launch_prop(bool is64, const DevModel &m, int64_t R, int32_t nchunk, const int32_t *x, void *a,
+                 uint8_t *crit, double *a0, double *a0c, double *part, double *partc, const uint8_t *active,
+                 hipStream_t s);
+void launch_tau(bool is64, const DevModel &m, int64_t R, double eps, const int32_t *x, const void *a,
+                const uint8_t *crit, void *tau_i, uint8_t *which_i, double *tau1, int32_t *argmin,
+                int32_t *which, const uint8_t *active, hipStream_t s);
+void launch_select(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase,
+                   const uint32_t *epoch, const void *a, const uint8_t *crit, const double *a0c,
+                   const double *tau1, const double *t_rem, double *tau, int32_t *jc, uint8_t *clamped,
+                   hipStream_t s);
+void launch_fire(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase,
+                 const uint32_t *epoch, const void *a, const uint8_t *crit, const double *tau, const int32_t *jc,
+                 int32_t *k, const uint8_t *mode, hipStream_t s);
+void launch_update(const DevModel &m, int64_t R, const int32_t *x, const int32_t *k, int32_t *x_out, uint8_t *neg,
+                   const uint8_t *mode, hipStream_t s);
+void launch_ssa(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase,
+                int32_t *x, double *t, uint32_t *epoch, double t_final, int32_t n_events, int32_t *events,
+                void *a, hipStream_t s);
+void launch_fill_u8(uint8_t *p, uint8_t v, int64_t n, hipStream_t s);
+void launch_fill_u64(unsigned long long *p, unsigned long long v, int64_t n, hipStream_t s);
+
+// Per-realization bookkeeping of the plain loop ([R] arrays).
+struct PlainState {
+    double *t, *tau1_used, *override_tau1, *tau;
+    uint32_t *epoch;
+    int64_t *steps, *ssa, *rej;
+    int32_t *status, *retries, *jc;
+    uint8_t *active, *mode, *clamped, *neg;
+    unsigned long long *n_done;
+};
+void launch_plain_init(const DevModel &m, int64_t R, int32_t *x, PlainState st, hipStream_t s);
+void launch_plain_select(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id,
+                         int64_t rbase, double t_final, int64_t max_iter, int32_t *x, void *a, const uint8_t *crit,
+                         const double *a0, const double *a0c, const double *tau1, PlainState st, hipStream_t s);
+void launch_plain_commit(const DevModel &m, int64_t R, int32_t *x, const int32_t *x_new, PlainState st,
+                         double t_final, hipStream_t s);
+void launch_gather_track(int64_t R, int32_t n_track, const int32_t *track, const int32_t *x, int32_t *out,
+                         hipStream_t s);
+
+// ----------------------------------------------------------- persistent --
+// One work item = one realization of one model.
+struct ModelItem {
+    DevModel dm;
+    double t_final, eps;
+    uint64_t seed;
+    int64_t max_iter;
+    int64_t item_base;   // first work-item index of this model
+    int64_t R;
+    int64_t rid_offset;  // realization id of item_base
+    uint32_t model_id;
+    int32_t n_track;
+    const int32_t *track;  // [n_track]
+    int32_t *out_x;        // [R][n_track]
+    int64_t *out_steps, *out_ssa, *out_rej;  // [R]
+    int32_t *out_status;   // [R]
+    double *score;         // [R][M] (sensitivities) or null
+    double *dtime;         // [R][M] dt/dc_j scratch (sensitivities) or null
+    int32_t n_intervals;   // > 0: record tracked species on the grid k*t_final/n_intervals
+    double *grid;          // [R][n_intervals+1][n_track]
+};
+
+struct PersistentConfig {
+    int block;     // threads per block
+    int group;     // lanes per realization (64 = one wave, == block = whole block)
+    bool a_lds;    // propensities in LDS (else per-group global scratch)
+};
+
+struct PersistentLaunch {
+    const ModelItem *items;  // device array [n_models]
+    int32_t n_models;
+    int64_t n_items;
+    int32_t n_max, m_max;
+    unsigned long long *counter;
+    void *a_glob;   // [slots][m_max] when !a_lds
+    int64_t slots;
+    int grid;
+    size_t lds_bytes;  // per block
+};
+
+// LDS bytes per block for a config; 0 if it can not fit.
+size_t persistent_lds_bytes(const PersistentConfig &c, bool is64, int32_t n_max, int32_t m_max);
+// Max resident blocks per CU for the config (0 if unsupported).
+int persistent_occupancy(const PersistentConfig &c, bool is64, bool sens, size_t lds_bytes);
+hipError_t launch_persistent(const PersistentConfig &c, bool is64, bool sens, const PersistentLaunch &L, hipStream_t s);
+void launch_grad_reduce(int64_t R, int32_t n_track, int32_t M, const int32_t *x_track, const double *score,
+                        double *mean_scratch, double *dmean, double *dmean_se, hipStream_t s);
+
+}  // namespace tl

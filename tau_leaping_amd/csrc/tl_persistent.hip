@@ -1,0 +1,769 @@
+// tl_persistent.hip -- the persistent engine: one work-stealing launch runs
+// whole trajectories of every (model, realization) work item.
+//
+// A group of lanes owns one realization at a time: GS = 64 is one wavefront
+// (several independent realizations per block, no block barriers); GS equal
+// to the block size makes the whole block cooperate on one realization (for
+// models with tens of thousands of channels).  Per group, LDS holds the
+// populations x[N], the proposed change dx[N] (integer LDS atomics, so the
+// update is exact and order-free), the criticality bits (one 64-bit ballot
+// per 64 channels) and, when it fits, the propensities a[M]; otherwise a[M]
+// lives in a per-group global scratch row.  Reductions (a0, a0c, tau min /
+// argmin, cumulative searches) are wavefront shuffles and ballots plus a
+// fixed-order combine across waves, so results do not depend on timing.
+//
+// The per-realization algorithm follows PerformCalc / Step
+// (tau_leaping/model.cu:301-413, 640-737) with the defect fixes of the CPU
+// oracle (FP64 time and cumulative sums, tau2 over a0c, exact direct events,
+// leap rejection on negative populations, clamp to t_final).
+#include "tl_internal.hpp"
+
+namespace tl {
+
+namespace {
+
+__host__ __device__ inline size_t al16(size_t v) { return (v + 15) & ~(size_t)15; }
+
+// LDS layout of one group.
+struct GroupLayout {
+    size_t off_x, off_dx, off_snap, off_crit, off_red, off_a, bytes;
+};
+__host__ __device__ inline GroupLayout group_layout(int32_t n_max, int32_t m_max, int gs, bool a_lds, size_t real_bytes) {
+    GroupLayout L;
+    const int nw = gs / 64;
+    L.off_x = 0;
+    L.off_dx = al16((size_t)n_max * 4);
+    L.off_snap = L.off_dx + al16((size_t)n_max * 4);
+    L.off_crit = L.off_snap + al16((size_t)n_max * 4);
+    L.off_red = L.off_crit + al16((size_t)((m_max + 63) / 64) * 8);
+    L.off_a = L.off_red + al16((size_t)(4 * nw + 4) * 8);
+    L.bytes = L.off_a + (a_lds ? al16((size_t)m_max * real_bytes) : 0);
+    return L;
+}
+
+// ------------------------------------------------------ uniform values --
+// Values every lane of a wave holds identically are moved to scalar
+// registers, so model pointers, times and step sizes do not occupy VGPRs and
+// branches on them are scalar branches.
+__device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ float uni(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double uni(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ unsigned long long uni(unsigned long long v) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(v & 0xffffffffull));
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+// Load a struct that is the same for every lane into scalar registers.
+template <class T>
+__device__ __forceinline__ T ld_uniform(const T *p) {
+    static_assert(sizeof(T) % 4 == 0, "dword struct");
+    T out;
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(p);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&out);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = (uint32_t)__builtin_amdgcn_readfirstlane((int)src[i]);
+    return out;
+}
+
+// ------------------------------------------------------- wave primitives --
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_incl_scan(double v, int wl) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double y = __shfl_up(v, o, 64);
+        if (wl >= o) v += y;
+    }
+    return v;
+}
+
+template <class R>
+__device__ __forceinline__ void wave_argmin(R &v, int &i, int &w) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const R ov = __shfl_xor(v, o, 64);
+        const int oi = __shfl_xor(i, o, 64);
+        const int ow = __shfl_xor(w, o, 64);
+        if (ov < v || (ov == v && oi < i)) {
+            v = ov;
+            i = oi;
+            w = ow;
+        }
+    }
+}
+
+__device__ __forceinline__ int wave_max_int(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ bool crit_bit(const uint64_t *critq, int32_t j) { return (critq[j >> 6] >> (j & 63)) & 1ull; }
+
+// ---------------------------------------------------------------- groups --
+// GS == 64: one wavefront per realization.  Only wavefront-level
+// synchronisation is used, so waves of a block never wait on each other.
+template <int GS>
+struct Grp {
+    static constexpr int NW = GS / 64;
+    int lane, wl, wid;
+    double *red;
+
+    __device__ __forceinline__ void sync() const {
+        if constexpr (GS == 64) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        } else {
+            __syncthreads();
+        }
+    }
+
+    __device__ __forceinline__ double sum(double v) const {
+        v = wave_sum(v);
+        if constexpr (GS == 64) {
+            return uni(v);
+        } else {
+            if (wl == 0) red[wid] = v;
+            __syncthreads();
+            double s = 0.0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) s += red[w];
+            __syncthreads();
+            return uni(s);
+        }
+    }
+
+    __device__ __forceinline__ void sum2(double &a, double &b) const {
+        a = wave_sum(a);
+        b = wave_sum(b);
+        if constexpr (GS != 64) {
+            if (wl == 0) {
+                red[wid] = a;
+                red[NW + wid] = b;
+            }
+            __syncthreads();
+            a = 0.0;
+            b = 0.0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                a += red[w];
+                b += red[NW + w];
+            }
+            __syncthreads();
+        }
+        a = uni(a);
+        b = uni(b);
+    }
+
+    __device__ __forceinline__ bool any(bool p) const {
+        if constexpr (GS == 64)
+            return __ballot(p) != 0ull;
+        else
+            return __syncthreads_or(p) != 0;
+    }
+
+    __device__ __forceinline__ int max_int(int v) const {
+        v = wave_max_int(v);
+        if constexpr (GS != 64) {
+            int *ri = reinterpret_cast<int *>(red);
+            if (wl == 0) ri[wid] = v;
+            __syncthreads();
+            v = ri[0];
+#pragma unroll
+            for (int w = 1; w < NW; ++w) v = max(v, ri[w]);
+            __syncthreads();
+        }
+        return uni(v);
+    }
+
+    template <class R>
+    __device__ __forceinline__ void argmin(R &v, int &i, int &w) const {
+        wave_argmin<R>(v, i, w);
+        if constexpr (GS != 64) {
+            R *rv = reinterpret_cast<R *>(red);
+            int *ri = reinterpret_cast<int *>(red + NW);
+            if (wl == 0) {
+                rv[wid] = v;
+                ri[2 * wid] = i;
+                ri[2 * wid + 1] = w;
+            }
+            __syncthreads();
+            v = rv[0];
+            i = ri[0];
+            w = ri[1];
+#pragma unroll
+            for (int q = 1; q < NW; ++q) {
+                const R ov = rv[q];
+                const int oi = ri[2 * q];
+                if (ov < v || (ov == v && oi < i)) {
+                    v = ov;
+                    i = oi;
+                    w = ri[2 * q + 1];
+                }
+            }
+            __syncthreads();
+        }
+        v = uni(v);
+        i = uni(i);
+        w = uni(w);
+    }
+
+    // Broadcast a 64-bit value from lane 0 of the group.
+    __device__ __forceinline__ unsigned long long bcast0(unsigned long long v) const {
+        if constexpr (GS == 64) {
+            return uni(v);
+        } else {
+            unsigned long long *p = reinterpret_cast<unsigned long long *>(red + 3 * NW);
+            if (lane == 0) *p = v;
+            __syncthreads();
+            v = *p;
+            __syncthreads();
+            return uni(v);
+        }
+    }
+
+    // First j with inclusive cumulative weight > target; weight = a_j, or
+    // a_j masked by the criticality bits.  -1 if the total never exceeds it.
+    template <class R>
+    __device__ int32_t search(const R *a, const uint64_t *critq, bool masked, int32_t M, double target) const {
+        const int32_t nt = (M + 63) >> 6;
+        if constexpr (GS == 64) {
+            return tile_search<R>(a, critq, masked, M, 0, nt, 0.0, target);
+        } else {
+            const int32_t tpw = (nt + NW - 1) / NW;
+            const int32_t tb = wid * tpw, te = min(nt, tb + tpw);
+            double part = 0.0;
+            for (int32_t tt = tb; tt < te; ++tt) {
+                if (masked && critq[tt] == 0ull) continue;
+                const int32_t j = tt * 64 + wl;
+                if (j < M && (!masked || ((critq[tt] >> wl) & 1ull))) part += (double)a[j];
+            }
+            part = wave_sum(part);
+            if (wl == 0) red[wid] = part;
+            __syncthreads();
+            double pre = 0.0, wbase = 0.0;
+            int ws = -1;
+            for (int w = 0; w < NW; ++w) {
+                const double nx = pre + red[w];
+                if (ws < 0 && nx > target) {
+                    ws = w;
+                    wbase = pre;
+                }
+                pre = nx;
+            }
+            int *slot = reinterpret_cast<int *>(red + 2 * NW);
+            if (wid == ws) {
+                int32_t f = tile_search<R>(a, critq, masked, M, tb, te, wbase, target);
+                if (f < 0) {  // rounding at the range end: last positive weight of this range
+                    int last = -1;
+                    for (int32_t tt = tb; tt < te; ++tt) {
+                        const int32_t j = tt * 64 + wl;
+                        if (j < M && a[j] > (R)0 && (!masked || ((critq[tt] >> wl) & 1ull))) last = j;
+                    }
+                    f = wave_max_int(last);
+                }
+                if (wl == 0) slot[0] = f;
+            }
+            __syncthreads();
+            const int32_t res = (ws < 0) ? -1 : slot[0];
+            __syncthreads();
+            return uni(res);
+        }
+    }
+
+    template <class R>
+    __device__ __forceinline__ int32_t tile_search(const R *a, const uint64_t *critq, bool masked, int32_t M, int32_t tb,
+                                                   int32_t te, double base, double target) const {
+        for (int32_t tt = tb; tt < te; ++tt) {
+            uint64_t cm = ~0ull;
+            if (masked) {
+                cm = critq[tt];
+                if (cm == 0ull) continue;
+            }
+            const int32_t j = tt * 64 + wl;
+            double w = 0.0;
+            if (j < M && ((cm >> wl) & 1ull)) w = (double)a[j];
+            const double inc = wave_incl_scan(w, wl);
+            const unsigned long long hit = __ballot(j < M && base + inc > target);
+            if (hit) return uni(tt * 64 + (int32_t)__builtin_ctzll(hit));
+            base += uni(__shfl(inc, 63, 64));
+        }
+        return -1;
+    }
+};
+
+// ----------------------------------------------------------------- passes --
+// Propensities, criticality bits, and the lane partial sums of a0 / a0c.
+template <class R, int GS>
+__device__ __forceinline__ void prop_pass(const DevModel &m, const R *__restrict__ c, const int32_t *xs, R *a,
+                                          uint64_t *critq, int lane, double &s, double &sc) {
+    s = 0.0;
+    sc = 0.0;
+    for (int32_t j0 = 0; j0 < m.M; j0 += GS) {
+        const int32_t j = j0 + lane;
+        bool cr = false;
+        if (j < m.M) {
+            const R aj = propensity<R>(m.order[j], c[j], xs[m.s0[j]], xs[m.s1[j]], cr);
+            a[j] = aj;
+            s += (double)aj;
+            if (cr) sc += (double)aj;
+        }
+        const unsigned long long bal = __ballot(cr);
+        const int32_t tile0 = j0 + (lane & ~63);
+        if ((lane & 63) == 0 && tile0 < m.M) critq[tile0 >> 6] = bal;
+    }
+}
+
+template <class R, int GS>
+__device__ __forceinline__ double prop_sum_pass(const DevModel &m, const R *__restrict__ c, const int32_t *xs, R *a,
+                                                int lane) {
+    double s = 0.0;
+    for (int32_t j = lane; j < m.M; j += GS) {
+        bool cr;
+        const R aj = propensity<R>(m.order[j], c[j], xs[m.s0[j]], xs[m.s1[j]], cr);
+        a[j] = aj;
+        s += (double)aj;
+    }
+    return s;
+}
+
+// Score term d_j * m_j of one channel (see the likelihood-ratio notes in
+// simulate_item).
+template <class R>
+__device__ __forceinline__ double score_term(bool cr, R aj, R tr, double tau, int32_t kj) {
+    const double mj = (double)aj * tau;
+    if (!cr && aj * tr >= (R)kPoissonSwitch) {
+        const double d = ((double)kj + 0.5) - mj;
+        return d + 0.5 * (d * d / mj - 1.0);
+    }
+    return (double)kj - mj;
+}
+
+// ------------------------------------------------------------ one item --
+// Tracked species on the time grid: linear interpolation between the
+// states before (snap, at t_a) and after (xs, at t_b) one main-loop step, as
+// AllStat2StepStat does between logged states (statistics.cpp:15-64).
+template <int GS>
+__device__ __forceinline__ void grid_record(const ModelItem &mi, int64_t r, int lane, const int32_t *snap,
+                                            const int32_t *xs, double t_a, double t_b, int32_t &next_g) {
+    const int32_t n = mi.n_intervals, T = mi.n_track;
+    const double h = mi.t_final / (double)n;
+    while (next_g < n) {
+        const double g = h * (double)next_g;
+        if (!(g < t_b)) break;
+        const double w = (t_b > t_a) ? (g - t_a) / (t_b - t_a) : 0.0;
+        double *out = mi.grid + ((size_t)r * (n + 1) + next_g) * T;
+        for (int32_t q = lane; q < T; q += GS) {
+            const double xa = (double)snap[q], xb = (double)xs[mi.track[q]];
+            out[q] = xa + (xb - xa) * w;
+        }
+        next_g++;
+    }
+}
+
+template <class R, int GS, bool SENS>
+__device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, int32_t *xs, int32_t *dx,
+                              int32_t *snap, uint64_t *critq, R *a) {
+    const DevModel &m = mi.dm;
+    const int lane = G.lane;
+    const R *c = Real<R>::rates(m);
+    const uint2 key = rng_key(mi.seed, mi.rid_offset + r);
+    const uint32_t c3 = mi.model_id << 2;
+    const double tf = mi.t_final;
+    const R eps = (R)mi.eps;
+    // Likelihood-ratio score S_j = dlog p(path)/dc_j, and D_j = dt/dc_j of the
+    // current time (nonzero through tau1-limited leaps; used by the clamped
+    // last leap and the truncated last direct interval).
+    double *S = SENS ? mi.score + (size_t)r * m.M : nullptr;
+    double *D = SENS ? mi.dtime + (size_t)r * m.M : nullptr;
+
+    for (int32_t i = lane; i < m.N; i += GS) {
+        xs[i] = m.x0[i];
+        dx[i] = 0;
+    }
+    if (SENS)
+        for (int32_t j = lane; j < m.M; j += GS) {
+            S[j] = 0.0;
+            D[j] = 0.0;
+        }
+    G.sync();
+
+    double t = 0.0;
+    uint32_t epoch = 0;
+    int64_t steps = 0, nssa = 0, nrej = 0;
+    int32_t status = kOk;
+    const bool grid = mi.n_intervals > 0;
+    int32_t next_g = 0;
+
+    while (t < tf) {
+        if (mi.max_iter > 0 && steps >= mi.max_iter) {
+            status = kIterCap;
+            break;
+        }
+        const double t_before = t;
+        if (grid)
+            for (int32_t q = lane; q < mi.n_track; q += GS) snap[q] = xs[mi.track[q]];
+        double a0, a0c;
+        prop_pass<R, GS>(m, c, xs, a, critq, lane, a0, a0c);
+        G.sum2(a0, a0c);
+        G.sync();
+        if (!(a0 > 0.0)) {  // absorbing state
+            t = tf;
+            steps++;
+            if (grid) grid_record<GS>(mi, r, lane, snap, xs, t_before, t, next_g);
+            break;
+        }
+        // tau1: per species mean / variance bound, min and argmin
+        R best = (R)kFltMax;
+        int bi = -1, bw = 0;
+        for (int32_t i = lane; i < m.N; i += GS) {
+            R mu = 0, sig = 0;
+            const int32_t e1 = m.row_ptr[i + 1];
+            for (int32_t e = m.row_ptr[i]; e < e1; ++e) {
+                const int32_t p = m.csr[e];
+                const int32_t j = entry_index(p);
+                const R v = (R)entry_value(p);
+                const R tmp = v * a[j] * (R)(!crit_bit(critq, j));
+                sig += v * tmp;
+                mu += tmp;
+            }
+            if (i < m.M) {
+                int w;
+                const R ti = tau_species<R>(eps, xs[i], m.g_order[i], mu, sig, w);
+                if (ti < best) {
+                    best = ti;
+                    bi = i;
+                    bw = w;
+                }
+            }
+        }
+        G.template argmin<R>(best, bi, bw);
+        double tau1 = (double)best;
+        const int32_t istar = bi;
+        const int which = bw;
+
+        int retries = 0;
+        for (;;) {
+            if (tau1 < kSsaFactor / a0) {
+                // direct branch: up to 100 exact events, propensities refreshed
+                double a0s = a0;
+                for (int s = 0; s < kSsaSteps; ++s) {
+                    if (s > 0) {
+                        a0s = G.sum(prop_sum_pass<R, GS>(m, c, xs, a, lane));
+                        G.sync();
+                    }
+                    if (!(a0s > 0.0)) {
+                        t = tf;
+                        break;
+                    }
+                    double r1, r2;
+                    step_uniforms(key, epoch, c3 | kKindSsa, r1, r2);
+                    epoch++;
+                    const double dt = -log(r2) / a0s;
+                    if (dt >= tf - t) {
+                        if (SENS) {  // survival over rem = tf - t: d/dc_j of -a0*rem
+                            const double rem = tf - t;
+                            for (int32_t j = lane; j < m.M; j += GS)
+                                if (c[j] != (R)0) S[j] += a0s * D[j] - (double)a[j] * rem / (double)c[j];
+                        }
+                        t = tf;
+                        break;
+                    }
+                    int32_t jf = G.template search<R>(a, critq, false, m.M, a0s * r1);
+                    if (jf < 0) {
+                        int last = -1;
+                        for (int32_t j = lane; j < m.M; j += GS)
+                            if (a[j] > (R)0) last = j;
+                        jf = G.max_int(last);
+                    }
+                    if (SENS) {
+                        for (int32_t j = lane; j < m.M; j += GS)
+                            if (c[j] != (R)0)
+                                S[j] += ((j == jf) ? 1.0 / (double)c[j] : 0.0) - (double)a[j] * dt / (double)c[j];
+                    }
+                    const int32_t e1 = m.col_ptr[jf + 1];
+                    for (int32_t e = m.col_ptr[jf] + lane; e < e1; e += GS) {
+                        const int32_t p = m.csc[e];
+                        atomicAdd(&xs[entry_index(p)], entry_value(p));
+                    }
+                    t += dt;
+                    nssa++;
+                    G.sync();
+                }
+                steps++;
+                break;
+            }
+
+            // ---- leap attempt ----
+            double r1, r2;
+            step_uniforms(key, epoch, c3 | kKindLeap, r1, r2);
+            int32_t jc = m.M;
+            double tau2 = (double)kFltMax;
+            if (a0c > 0.0) {
+                const int32_t f = G.template search<R>(a, critq, true, m.M, a0c * r1);
+                if (f >= 0) {
+                    jc = f;
+                    tau2 = -log(r2) / a0c;
+                }
+            }
+            double tau = tau1 < tau2 ? tau1 : tau2;
+            bool fire = (jc != m.M) && (tau2 < tau1);
+            bool clamped = false;
+            if (tau >= tf - t) {
+                tau = tf - t;
+                clamped = true;
+                fire = false;
+            }
+            const int32_t jfire = fire ? jc : -1;
+            const R tr = (R)tau;
+            const uint32_t ck = c3 | kKindPoisson;
+            double qpart = 0.0;
+            for (int32_t j = lane; j < m.M; j += GS) {
+                const bool cr = crit_bit(critq, j);
+                const R aj = a[j];
+                const int32_t kj = cr ? (j == jfire ? 1 : 0) : poisson_draw<R>(aj * tr, key, (uint32_t)j, epoch, ck);
+                if (kj) {
+                    const int32_t e1 = m.col_ptr[j + 1];
+                    for (int32_t e = m.col_ptr[j]; e < e1; ++e) {
+                        const int32_t p = m.csc[e];
+                        atomicAdd(&dx[entry_index(p)], entry_value(p) * kj);
+                    }
+                }
+                if (SENS) {
+                    const double term = score_term<R>(cr, aj, tr, tau, kj);
+                    qpart += term;
+                    if (c[j] != (R)0) S[j] += term / (double)c[j];
+                }
+            }
+            G.sync();
+            bool neg = false;
+            for (int32_t i = lane; i < m.N; i += GS) neg |= (xs[i] + dx[i]) < 0;
+            neg = G.any(neg);
+            if (neg) {
+                for (int32_t i = lane; i < m.N; i += GS) dx[i] = 0;
+                if (SENS) {  // take back this attempt's score terms (same draws)
+                    for (int32_t j = lane; j < m.M; j += GS) {
+                        const bool cr = crit_bit(critq, j);
+                        const R aj = a[j];
+                        const int32_t kj =
+                            cr ? (j == jfire ? 1 : 0) : poisson_draw<R>(aj * tr, key, (uint32_t)j, epoch, ck);
+                        if (c[j] != (R)0) S[j] -= score_term<R>(cr, aj, tr, tau, kj) / (double)c[j];
+                    }
+                }
+                epoch++;
+                nrej++;
+                G.sync();
+                if (++retries > kMaxRetries) {
+                    status = kRetryExhausted;
+                    break;
+                }
+                tau1 *= 0.5;
+                continue;
+            }
+            epoch++;
+            for (int32_t i = lane; i < m.N; i += GS) {
+                xs[i] += dx[i];
+                dx[i] = 0;
+            }
+            if (SENS && clamped) {
+                // tau = tf - t: dtau/dc_j = -D_j
+                const double Q = G.sum(qpart);
+                for (int32_t j = lane; j < m.M; j += GS) S[j] -= (Q / tau) * D[j];
+            } else if (SENS && jfire < 0 && istar >= 0) {
+                // tau = tau1: add Q * dtau1/dc_j over the argmin species row
+                const double Q = G.sum(qpart);
+                double mu = 0.0, sig = 0.0;
+                const int32_t e0 = m.row_ptr[istar], e1 = m.row_ptr[istar + 1];
+                for (int32_t e = e0 + lane; e < e1; e += GS) {
+                    const int32_t p = m.csr[e], j = entry_index(p);
+                    if (crit_bit(critq, j)) continue;
+                    const double v = (double)entry_value(p);
+                    mu += v * (double)a[j];
+                    sig += v * v * (double)a[j];
+                }
+                G.sum2(mu, sig);
+                for (int32_t e = e0 + lane; e < e1; e += GS) {
+                    const int32_t p = m.csr[e], j = entry_index(p);
+                    if (crit_bit(critq, j) || c[j] == (R)0) continue;
+                    const double v = (double)entry_value(p);
+                    const double h = (double)a[j] / (double)c[j];
+                    const double dtau = (which == 0) ? -tau * (mu > 0.0 ? 1.0 : -1.0) * v * h / fabs(mu)
+                                                     : -tau * v * v * h / sig;
+                    atomicAdd(&S[j], (Q / tau) * dtau);
+                    atomicAdd(&D[j], dtau);
+                }
+            }
+            const double tn = clamped ? tf : t + tau;
+            if (!(tn > t)) {  // time can not advance: stop instead of spinning
+                status = kStalled;
+                break;
+            }
+            t = tn;
+            steps++;
+            G.sync();
+            break;
+        }
+        if (status != kOk) break;
+        if (grid) grid_record<GS>(mi, r, lane, snap, xs, t_before, t, next_g);
+    }
+
+    if (grid && status == kOk) {  // grid point t_final (and any left) takes the final state
+        const int32_t n = mi.n_intervals, T = mi.n_track;
+        for (; next_g <= n; ++next_g) {
+            double *out = mi.grid + ((size_t)r * (n + 1) + next_g) * T;
+            for (int32_t q = lane; q < T; q += GS) out[q] = (double)xs[mi.track[q]];
+        }
+    }
+
+    // outputs
+    for (int32_t q = lane; q < mi.n_track; q += GS) mi.out_x[r * mi.n_track + q] = xs[mi.track[q]];
+    if (lane == 0) {
+        mi.out_steps[r] = steps;
+        mi.out_ssa[r] = nssa;
+        mi.out_rej[r] = nrej;
+        mi.out_status[r] = status;
+    }
+    G.sync();
+}
+
+template <class R, int BS, int GS, bool ALDS, bool SENS>
+__global__ __launch_bounds__(BS) void k_persistent(PersistentLaunch L) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    constexpr int NG = BS / GS;
+    const GroupLayout lay = group_layout(L.n_max, L.m_max, GS, ALDS, sizeof(R));
+    const int tid = threadIdx.x;
+    const int g = tid / GS;
+    Grp<GS> G;
+    G.lane = tid % GS;
+    G.wl = tid & 63;
+    G.wid = G.lane >> 6;
+    unsigned char *base = lds + (size_t)g * lay.bytes;
+    int32_t *xs = reinterpret_cast<int32_t *>(base + lay.off_x);
+    int32_t *dx = reinterpret_cast<int32_t *>(base + lay.off_dx);
+    int32_t *snap = reinterpret_cast<int32_t *>(base + lay.off_snap);
+    uint64_t *critq = reinterpret_cast<uint64_t *>(base + lay.off_crit);
+    G.red = reinterpret_cast<double *>(base + lay.off_red);
+    R *a;
+    if constexpr (ALDS)
+        a = reinterpret_cast<R *>(base + lay.off_a);
+    else
+        a = reinterpret_cast<R *>(L.a_glob) + (size_t)((int64_t)blockIdx.x * NG + g) * L.m_max;
+
+    for (;;) {
+        unsigned long long v = 0;
+        if (G.lane == 0) v = atomicAdd(L.counter, 1ull);
+        const int64_t item = (int64_t)G.bcast0(uni(v));
+        if (item >= L.n_items) break;
+        int lo = 0, hi = L.n_models - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (L.items[mid].item_base <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const ModelItem mi = ld_uniform(&L.items[uni(lo)]);
+        simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, critq, a);
+    }
+}
+
+// ---------------------------------------------------------- dispatch ----
+template <class R, bool SENS>
+const void *kernel_ptr(int bs, int gs, bool alds) {
+#define TL_K(BS_, GS_)                                                                                 \
+    if (bs == BS_ && gs == GS_)                                                                        \
+        return alds ? (const void *)k_persistent<R, BS_, GS_, true, SENS>                              \
+                    : (const void *)k_persistent<R, BS_, GS_, false, SENS>;
+    TL_K(64, 64)
+    TL_K(256, 64)
+    TL_K(256, 256)
+    TL_K(512, 512)
+    TL_K(1024, 1024)
+#undef TL_K
+    return nullptr;
+}
+
+const void *kernel_for(const PersistentConfig &c, bool is64, bool sens) {
+    if (is64) return sens ? kernel_ptr<double, true>(c.block, c.group, c.a_lds) : kernel_ptr<double, false>(c.block, c.group, c.a_lds);
+    return sens ? kernel_ptr<float, true>(c.block, c.group, c.a_lds) : kernel_ptr<float, false>(c.block, c.group, c.a_lds);
+}
+
+// ------------------------------------------------------- gradient reduce --
+__global__ __launch_bounds__(256) void k_track_mean(int64_t Rn, int32_t T, const int32_t *__restrict__ xt,
+                                                    double *__restrict__ mean) {
+    __shared__ double sh[256];
+    const int q = blockIdx.x;
+    double s = 0.0;
+    for (int64_t r = threadIdx.x; r < Rn; r += 256) s += (double)xt[r * T + q];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mean[q] = sh[0] / (double)Rn;
+}
+
+// dmean[q][j] = mean_r (X_q^r - Xbar_q) S_j^r, and its standard error.
+__global__ __launch_bounds__(256) void k_grad(int64_t Rn, int32_t T, int32_t M, const int32_t *__restrict__ xt,
+                                              const double *__restrict__ S, const double *__restrict__ mean,
+                                              double *__restrict__ dmean, double *__restrict__ se) {
+    const int32_t j = blockIdx.x * 256 + threadIdx.x;
+    const int32_t q = blockIdx.y;
+    if (j >= M) return;
+    const double mq = mean[q];
+    double acc = 0.0, acc2 = 0.0;
+    for (int64_t r = 0; r < Rn; ++r) {
+        const double d = ((double)xt[r * T + q] - mq) * S[r * M + j];
+        acc += d;
+        acc2 += d * d;
+    }
+    const double mu = acc / (double)Rn;
+    dmean[(int64_t)q * M + j] = mu;
+    if (se) {
+        const double var = (Rn > 1) ? (acc2 / (double)Rn - mu * mu) * (double)Rn / (double)(Rn - 1) : 0.0;
+        se[(int64_t)q * M + j] = sqrt(fmax(var, 0.0) / (double)Rn);
+    }
+}
+
+}  // namespace
+
+size_t persistent_lds_bytes(const PersistentConfig &c, bool is64, int32_t n_max, int32_t m_max) {
+    const GroupLayout L = group_layout(n_max, m_max, c.group, c.a_lds, is64 ? 8 : 4);
+    return L.bytes * (size_t)(c.block / c.group);
+}
+
+int persistent_occupancy(const PersistentConfig &c, bool is64, bool sens, size_t lds_bytes) {
+    const void *k = kernel_for(c, is64, sens);
+    if (!k) return 0;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, c.block, lds_bytes) != hipSuccess) return 0;
+    return n;
+}
+
+hipError_t launch_persistent(const PersistentConfig &c, bool is64, bool sens, const PersistentLaunch &L, hipStream_t s) {
+    const void *k = kernel_for(c, is64, sens);
+    if (!k) return hipErrorInvalidDeviceFunction;
+    void *args[] = {const_cast<PersistentLaunch *>(&L)};
+    return hipLaunchKernel(k, dim3((unsigned)L.grid), dim3((unsigned)c.block), args, L.lds_bytes, s);
+}
+
+void launch_grad_reduce(int64_t R, int32_t n_track, int32_t M, const int32_t *x_track, const double *score,
+                        double *mean_scratch, double *dmean, double *dmean_se, hipStream_t s) {
+    k_track_mean<<<(unsigned)n_track, 256, 0, s>>>(R, n_track, x_track, mean_scratch);
+    dim3 g((unsigned)((M + 255) / 256), (unsigned)n_track);
+    k_grad<<<g, 256, 0, s>>>(R, n_track, M, x_track, score, mean_scratch, dmean, dmean_se);
+}
+
+}  // namespace tl

@@ -1,0 +1,541 @@
+// tl_stages.hip -- stage kernels of the plain engine form.
+//
+// State is structure-of-arrays with realizations contiguous: x[i*R + r],
+// a[j*R + r], so a wavefront of 64 realizations touching the same species or
+// channel makes one coalesced access.  One host iteration of the plain form
+// launches: propensities -> tau rows -> tau min -> select (leap scalars or a
+// direct-method block) -> fire -> update -> commit.
+//
+// Reference stages (file:line): CalcPropensities model.cu:213-247; CompTau
+// CSRMatrixPacked.cu:423-462; GetReaction2Fire model.cu:555-598 and the leap
+// branch model.cu:346-390; SetNumberOfFiring model.cu:538-549; FireReactions
+// CSRMatrixPacked.cu:464-503; DirectMethodStep model.cu:427-451.
+#include "tl_internal.hpp"
+
+namespace tl {
+
+namespace {
+
+constexpr int kT = 256;  // threads per block for the realization axis
+
+inline unsigned blocks_for(int64_t n, int t = kT) { return (unsigned)((n + t - 1) / t); }
+
+// ---------------------------------------------------------- propensities --
+template <class R>
+__global__ __launch_bounds__(kT) void k_prop(DevModel m, int64_t Rn, int32_t nchunk, const int32_t *__restrict__ x,
+                                             R *__restrict__ a, uint8_t *__restrict__ crit, double *__restrict__ a0,
+                                             double *__restrict__ a0c, double *__restrict__ part,
+                                             double *__restrict__ partc, const uint8_t *__restrict__ active) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= Rn) return;
+    if (active && !active[r]) return;
+    const int32_t c = blockIdx.y;
+    const int32_t ch = (m.M + nchunk - 1) / nchunk;
+    const int32_t j0 = c * ch, j1 = min(m.M, j0 + ch);
+    const R *rc = Real<R>::rates(m);
+    double s = 0.0, sc = 0.0;
+    for (int32_t j = j0; j < j1; ++j) {
+        bool cr;
+        const R aj = propensity<R>(m.order[j], rc[j], x[(int64_t)m.s0[j] * Rn + r], x[(int64_t)m.s1[j] * Rn + r], cr);
+        a[(int64_t)j * Rn + r] = aj;
+        crit[(int64_t)j * Rn + r] = cr;
+        s += (double)aj;
+        if (cr) sc += (double)aj;
+    }
+    if (nchunk == 1) {
+        a0[r] = s;
+        a0c[r] = sc;
+    } else {
+        part[(int64_t)c * Rn + r] = s;
+        partc[(int64_t)c * Rn + r] = sc;
+    }
+}
+
+__global__ __launch_bounds__(kT) void k_sum_parts(int64_t Rn, int32_t nchunk, const double *__restrict__ part,
+                                                  const double *__restrict__ partc, double *__restrict__ a0,
+                                                  double *__restrict__ a0c, const uint8_t *__restrict__ active) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= Rn) return;
+    if (active && !active[r]) return;
+    double s = 0.0, sc = 0.0;
+    for (int32_t c = 0; c < nchunk; ++c) {
+        s += part[(int64_t)c * Rn + r];
+        sc += partc[(int64_t)c * Rn + r];
+    }
+    a0[r] = s;
+    a0c[r] = sc;
+}
+
+// ------------------------------------------------------------------ tau --
+// tau_i for species row i = blockIdx.y (+ gridDim.y strides).
+template <class R>
+__global__ __launch_bounds__(kT) void k_tau_rows(DevModel m, int64_t Rn, R eps, const int32_t *__restrict__ x,
+                                                 const R *__restrict__ a, const uint8_t *__restrict__ crit,
+                                                 R *__restrict__ tau_i, uint8_t *__restrict__ which_i,
+                                                 const uint8_t *__restrict__ active) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= Rn) return;
+    if (active && !active[r]) return;
+    for (int32_t i = blockIdx.y; i < m.N; i += gridDim.y) {
+        R mu = 0, sig = 0;
+        const int32_t e1 = m.row_ptr[i + 1];
+        for (int32_t e = m.row_ptr[i]; e < e1; ++e) {
+            const int32_t p = m.csr[e];
+            const int32_t j = entry_index(p);
+            const R v = (R)entry_value(p);
+            const R tmp = v * a[(int64_t)j * Rn + r] * (R)(!crit[(int64_t)j * Rn + r]);
+            sig += v * tmp;
+            mu += tmp;
+        }
+        R ti;
+        int w = 0;
+        if (i < m.M)
+            ti = tau_species<R>(eps, x[(int64_t)i * Rn + r], m.g_order[i], mu, sig, w);
+        else
+            ti = (R)INFINITY;
+        tau_i[(int64_t)i * Rn + r] = ti;
+        which_i[(int64_t)i * Rn + r] = (uint8_t)w;
+    }
+}
+
+template <class R>
+__global__ __launch_bounds__(kT) void k_tau_min(int32_t N, int64_t Rn, const R *__restrict__ tau_i,
+                                                const uint8_t *__restrict__ which_i, double *__restrict__ tau1,
+                                                int32_t *__restrict__ argmin, int32_t *__restrict__ which,
+                                                const uint8_t *__restrict__ active) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= Rn) return;
+    if (active && !active[r]) return;
+    R best = (R)kFltMax;
+    int32_t bi = -1, bw = 0;
+    for (int32_t i = 0; i < N; ++i) {
+        const R ti = tau_i[(int64_t)i * Rn + r];
+        if (ti < best) {
+            best = ti;
+            bi = i;
+            bw = which_i[(int64_t)i * Rn + r];
+        }
+    }
+    tau1[r] = (double)best;
+    if (argmin) argmin[r] = bi;
+    if (which) which[r] = bw;
+}
+
+// --------------------------------------------------------------- select --
+template <class R>
+__device__ void select_leap(const DevModel &m, uint2 key, uint32_t c3, uint32_t epoch, const R *a, const uint8_t *crit,
+                            int64_t Rn, int64_t r, double a0c, double tau1, double t_rem, double &tau, int32_t &jc_out,
+                            uint8_t &clamped) {
+    double r1, r2;
+    step_uniforms(key, epoch, c3 | kKindLeap, r1, r2);
+    int32_t jc = m.M;
+    double tau2 = (double)kFltMax;
+    if (a0c > 0.0) {
+        const double target = a0c * r1;
+        double cum = 0.0;
+        for (int32_t j = 0; j < m.M; ++j) {
+            if (!crit[(int64_t)j * Rn + r]) continue;
+            cum += (double)a[(int64_t)j * Rn + r];
+            if (cum > target) {
+                jc = j;
+                break;
+            }
+        }
+        if (jc != m.M) tau2 = -log(r2) / a0c;
+    }
+    tau = tau1 < tau2 ? tau1 : tau2;
+    bool fire = (jc != m.M) && (tau2 < tau1);
+    clamped = 0;
+    if (tau >= t_rem) {
+        tau = t_rem;
+        clamped = 1;
+        fire = false;
+    }
+    jc_out = fire ? jc : -1;
+}
+
+template <class R>
+__global__ __launch_bounds__(kT) void k_select(DevModel m, int64_t Rn, uint64_t seed, uint32_t model_id, int64_t rbase,
+                                               const uint32_t *__restrict__ epoch, const R *__restrict__ a,
+                                               const uint8_t *__restrict__ crit, const double *__restrict__ a0c,
+                                               const double *__restrict__ tau1, const double *__restrict__ t_rem,
+                                               double *__restrict__ tau, int32_t *__restrict__ jc,
+                                               uint8_t *__restrict__ clamped) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= Rn) return;
+    select_leap<R>(m, rng_key(seed, rbase + r), model_id << 2, epoch[r], a, crit, Rn, r, a0c[r], tau1[r], t_rem[r],
+                   tau[r], jc[r], clamped[r]);
+}
+
+// ----------------------------------------------------------------- fire --
+// One block covers kT realizations of one channel: blockIdx.x = j*rt + tile.
+template <class R>
+__global__ __launch_bounds__(kT) void k_fire(DevModel m, int64_t Rn, uint64_t seed, uint32_t model_id, int64_t rbase,
+                                             const uint32_t *__restrict__ epoch, const R *__restrict__ a,
+                                             const uint8_t *__restrict__ crit, const double *__restrict__ tau,
+                                             const int32_t *__restrict__ jc, int32_t *__restrict__ k,
+                                             const uint8_t *__restrict__ mode) {
+    const int64_t rt = (Rn + kT - 1) / kT;
+    const int32_t j = (int32_t)(blockIdx.x / rt);
+    const int64_t r = (int64_t)(blockIdx.x % rt) * kT + threadIdx.x;
+    if (r >= Rn) return;
+    if (mode && mode[r] != 1) return;
+    const int64_t o = (int64_t)j * Rn + r;
+    int32_t kj;
+    if (crit[o])
+        kj = (jc[r] == j) ? 1 : 0;
+    else
+        kj = poisson_draw<R>(a[o] * (R)tau[r], rng_key(seed, rbase + r), (uint32_t)j, epoch[r],
+                             (model_id << 2) | kKindPoisson);
+    k[o] = kj;
+}
+
+// --------------------------------------------------------------- update --
+// x_out_i = x_i + sum over row i of v * k_j (all N species rows).
+__global__ __launch_bounds__(kT) void k_update(DevModel m, int64_t Rn, const int32_t *__restrict__ x,
+                                               const int32_t *__restrict__ k, int32_t *__restrict__ x_out,
+                                               uint8_t *__restrict__ neg, const uint8_t *__restrict__ mode) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= Rn) return;
+    if (mode && mode[r] != 1) return;
+    for (int32_t i = blockIdx.y; i < m.N; i += gridDim.y) {
+        int32_t s = 0;
+        const int32_t e1 = m.row_ptr[i + 1];
+        for (int32_t e = m.row_ptr[i]; e < e1; ++e) {
+            const int32_t p = m.csr[e];
+            s += entry_value(p) * k[(int64_t)entry_index(p) * Rn + r];
+        }
+        const int32_t xo = x[(int64_t)i * Rn + r] + s;
+        x_out[(int64_t)i * Rn + r] = xo;
+        if (xo < 0) neg[r] = 1;
+    }
+}
+
+// ------------------------------------------------------------ direct SSA --
+template <class R>
+__device__ double prop_soa(const DevModel &m, const int32_t *x, R *a, uint8_t *crit, int64_t Rn, int64_t r,
+                           double *a0c) {
+    const R *rc = Real<R>::rates(m);
+    double s = 0.0, sc = 0.0;
+    for (int32_t j = 0; j < m.M; ++j) {
+        bool cr;
+        const R aj = propensity<R>(m.order[j], rc[j], x[(int64_t)m.s0[j] * Rn + r], x[(int64_t)m.s1[j] * Rn + r], cr);
+        a[(int64_t)j * Rn + r] = aj;
+        if (crit) crit[(int64_t)j * Rn + r] = cr;
+        s += (double)aj;
+        if (cr) sc += (double)aj;
+    }
+    if (a0c) *a0c = sc;
+    return s;
+}
+
+// One direct-method event on column r (the fixed DirectMethodStep): returns
+// false when the realization reached t_final instead.
+template <class R>
+__device__ bool ssa_event_soa(const DevModel &m, uint2 key, uint32_t c3, uint32_t &epoch, int32_t *x, int64_t Rn,
+                              int64_t r, double &t, double tf, const R *a, double a0) {
+    if (!(a0 > 0.0)) {
+        t = tf;
+        return false;
+    }
+    double r1, r2;
+    step_uniforms(key, epoch, c3 | kKindSsa, r1, r2);
+    epoch++;
+    const double dt = -log(r2) / a0;
+    if (dt >= tf - t) {
+        t = tf;
+        return false;
+    }
+    const double target = a0 * r1;
+    double cum = 0.0;
+    int32_t jf = -1, last = -1;
+    for (int32_t j = 0; j < m.M; ++j) {
+        const R aj = a[(int64_t)j * Rn + r];
+        if (aj > 0) last = j;
+        cum += (double)aj;
+        if (cum > target) {
+            jf = j;
+            break;
+        }
+    }
+    if (jf < 0) jf = last;
+    for (int32_t e = m.col_ptr[jf]; e < m.col_ptr[jf + 1]; ++e) {
+        const int32_t p = m.csc[e];
+        x[(int64_t)entry_index(p) * Rn + r] += entry_value(p);
+    }
+    t += dt;
+    return true;
+}
+
+template <class R>
+__global__ __launch_bounds__(kT) void k_ssa(DevModel m, int64_t Rn, uint64_t seed, uint32_t model_id, int64_t rbase,
+                                            int32_t *__restrict__ x, double *__restrict__ t, uint32_t *__restrict__ epoch,
+                                            double tf, int32_t n_events, int32_t *__restrict__ events,
+                                            R *__restrict__ a) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= Rn) return;
+    const uint2 key = rng_key(seed, rbase + r);
+    double tr = t[r];
+    uint32_t ep = epoch[r];
+    int32_t n = 0;
+    for (int32_t s = 0; s < n_events && tr < tf; ++s) {
+        const double a0 = prop_soa<R>(m, x, a, nullptr, Rn, r, nullptr);
+        if (!ssa_event_soa<R>(m, key, model_id << 2, ep, x, Rn, r, tr, tf, a, a0)) break;
+        ++n;
+    }
+    t[r] = tr;
+    epoch[r] = ep;
+    events[r] = n;
+}
+
+// ------------------------------------------------------ plain bookkeeping --
+__global__ __launch_bounds__(kT) void k_plain_init(DevModel m, int64_t Rn, int32_t *__restrict__ x, PlainState st) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= Rn) return;
+    for (int32_t i = 0; i < m.N; ++i) x[(int64_t)i * Rn + r] = m.x0[i];
+    st.t[r] = 0.0;
+    st.override_tau1[r] = -1.0;
+    st.tau1_used[r] = 0.0;
+    st.tau[r] = 0.0;
+    st.epoch[r] = 0;
+    st.steps[r] = 0;
+    st.ssa[r] = 0;
+    st.rej[r] = 0;
+    st.status[r] = kOk;
+    st.retries[r] = 0;
+    st.jc[r] = -1;
+    st.active[r] = 1;
+    st.mode[r] = 0;
+    st.clamped[r] = 0;
+    st.neg[r] = 0;
+}
+
+template <class R>
+__global__ __launch_bounds__(kT) void k_plain_select(DevModel m, int64_t Rn, uint64_t seed, uint32_t model_id,
+                                                     int64_t rbase, double tf, int64_t max_iter,
+                                                     int32_t *__restrict__ x, R *__restrict__ a,
+                                                     const uint8_t *__restrict__ crit, const double *__restrict__ a0,
+                                                     const double *__restrict__ a0c, const double *__restrict__ tau1,
+                                                     PlainState st) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= Rn) return;
+    st.neg[r] = 0;
+    if (!st.active[r]) {
+        st.mode[r] = 0;
+        return;
+    }
+    if (max_iter > 0 && st.steps[r] >= max_iter) {
+        st.status[r] = kIterCap;
+        st.active[r] = 0;
+        st.mode[r] = 0;
+        atomicAdd(st.n_done, 1ull);
+        return;
+    }
+    const double a0r = a0[r];
+    double t = st.t[r];
+    if (!(a0r > 0.0)) {  // absorbing state
+        st.t[r] = tf;
+        st.steps[r] += 1;
+        st.active[r] = 0;
+        st.mode[r] = 0;
+        atomicAdd(st.n_done, 1ull);
+        return;
+    }
+    const double ov = st.override_tau1[r];
+    const double tau1r = ov >= 0.0 ? ov : tau1[r];
+    st.tau1_used[r] = tau1r;
+    const uint2 key = rng_key(seed, rbase + r);
+    const uint32_t c3 = model_id << 2;
+    if (tau1r < kSsaFactor / a0r) {  // direct block
+        uint32_t ep = st.epoch[r];
+        double a0s = a0r;
+        int64_t n = 0;
+        for (int s = 0; s < kSsaSteps; ++s) {
+            if (s > 0) a0s = prop_soa<R>(m, x, a, nullptr, Rn, r, nullptr);
+            if (!ssa_event_soa<R>(m, key, c3, ep, x, Rn, r, t, tf, a, a0s)) break;
+            ++n;
+        }
+        st.t[r] = t;
+        st.epoch[r] = ep;
+        st.ssa[r] += n;
+        st.steps[r] += 1;
+        st.retries[r] = 0;
+        st.override_tau1[r] = -1.0;
+        st.mode[r] = 2;
+        if (t >= tf) {
+            st.active[r] = 0;
+            atomicAdd(st.n_done, 1ull);
+        }
+        return;
+    }
+    double tau;
+    int32_t jc;
+    uint8_t cl;
+    select_leap<R>(m, key, c3, st.epoch[r], a, crit, Rn, r, a0c[r], tau1r, tf - t, tau, jc, cl);
+    st.tau[r] = tau;
+    st.jc[r] = jc;
+    st.clamped[r] = cl;
+    st.mode[r] = 1;
+}
+
+// Accept or reject the leap of each realization (rows i = blockIdx.y...).
+__global__ __launch_bounds__(kT) void k_plain_commit(DevModel m, int64_t Rn, int32_t *__restrict__ x,
+                                                     const int32_t *__restrict__ x_new, PlainState st, double tf) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= Rn) return;
+    if (st.mode[r] != 1) return;
+    const bool neg = st.neg[r] != 0;
+    if (!neg)
+        for (int32_t i = blockIdx.y; i < m.N; i += gridDim.y) x[(int64_t)i * Rn + r] = x_new[(int64_t)i * Rn + r];
+    if (blockIdx.y != 0) return;
+    st.epoch[r] += 1;
+    if (neg) {
+        st.rej[r] += 1;
+        const int32_t rt = st.retries[r] + 1;
+        st.retries[r] = rt;
+        if (rt > kMaxRetries) {
+            st.status[r] = kRetryExhausted;
+            st.active[r] = 0;
+            atomicAdd(st.n_done, 1ull);
+        } else {
+            st.override_tau1[r] = st.tau1_used[r] * 0.5;
+        }
+    } else {
+        const double t = st.clamped[r] ? tf : st.t[r] + st.tau[r];
+        if (!(t > st.t[r])) {  // time can not advance: stop instead of spinning
+            st.status[r] = kStalled;
+            st.active[r] = 0;
+            atomicAdd(st.n_done, 1ull);
+            return;
+        }
+        st.t[r] = t;
+        st.steps[r] += 1;
+        st.retries[r] = 0;
+        st.override_tau1[r] = -1.0;
+        if (t >= tf) {
+            st.active[r] = 0;
+            atomicAdd(st.n_done, 1ull);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kT) void k_gather_track(int64_t Rn, int32_t T, const int32_t *__restrict__ track,
+                                                     const int32_t *__restrict__ x, int32_t *__restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= Rn) return;
+    for (int32_t q = 0; q < T; ++q) out[r * T + q] = x[(int64_t)track[q] * Rn + r];
+}
+
+__global__ __launch_bounds__(kT) void k_fill_u8(uint8_t *p, uint8_t v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ __launch_bounds__(kT) void k_fill_u64(unsigned long long *p, unsigned long long v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+unsigned row_grid(int32_t n) { return (unsigned)(n < 65535 ? n : 65535); }
+
+}  // namespace
+
+// ----------------------------------------------------------- launchers ----
+void launch_prop(bool is64, const DevModel &m, int64_t R, int32_t nchunk, const int32_t *x, void *a, uint8_t *crit,
+                 double *a0, double *a0c, double *part, double *partc, const uint8_t *active, hipStream_t s) {
+    dim3 g(blocks_for(R), (unsigned)nchunk);
+    if (is64)
+        k_prop<double><<<g, kT, 0, s>>>(m, R, nchunk, x, (double *)a, crit, a0, a0c, part, partc, active);
+    else
+        k_prop<float><<<g, kT, 0, s>>>(m, R, nchunk, x, (float *)a, crit, a0, a0c, part, partc, active);
+    if (nchunk > 1) k_sum_parts<<<blocks_for(R), kT, 0, s>>>(R, nchunk, part, partc, a0, a0c, active);
+}
+
+void launch_tau(bool is64, const DevModel &m, int64_t R, double eps, const int32_t *x, const void *a,
+                const uint8_t *crit, void *tau_i, uint8_t *which_i, double *tau1, int32_t *argmin, int32_t *which,
+                const uint8_t *active, hipStream_t s) {
+    dim3 g(blocks_for(R), row_grid(m.N));
+    if (is64) {
+        k_tau_rows<double><<<g, kT, 0, s>>>(m, R, eps, x, (const double *)a, crit, (double *)tau_i, which_i, active);
+        k_tau_min<double><<<blocks_for(R), kT, 0, s>>>(m.N, R, (const double *)tau_i, which_i, tau1, argmin, which,
+                                                       active);
+    } else {
+        k_tau_rows<float><<<g, kT, 0, s>>>(m, R, (float)eps, x, (const float *)a, crit, (float *)tau_i, which_i, active);
+        k_tau_min<float><<<blocks_for(R), kT, 0, s>>>(m.N, R, (const float *)tau_i, which_i, tau1, argmin, which,
+                                                      active);
+    }
+}
+
+void launch_select(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase,
+                   const uint32_t *epoch, const void *a, const uint8_t *crit, const double *a0c, const double *tau1,
+                   const double *t_rem, double *tau, int32_t *jc, uint8_t *clamped, hipStream_t s) {
+    if (is64)
+        k_select<double><<<blocks_for(R), kT, 0, s>>>(m, R, seed, model_id, rbase, epoch, (const double *)a, crit, a0c,
+                                                      tau1, t_rem, tau, jc, clamped);
+    else
+        k_select<float><<<blocks_for(R), kT, 0, s>>>(m, R, seed, model_id, rbase, epoch, (const float *)a, crit, a0c,
+                                                     tau1, t_rem, tau, jc, clamped);
+}
+
+void launch_fire(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase,
+                 const uint32_t *epoch, const void *a, const uint8_t *crit, const double *tau, const int32_t *jc,
+                 int32_t *k, const uint8_t *mode, hipStream_t s) {
+    const int64_t nb = (int64_t)blocks_for(R) * m.M;
+    if (is64)
+        k_fire<double><<<(unsigned)nb, kT, 0, s>>>(m, R, seed, model_id, rbase, epoch, (const double *)a, crit, tau, jc,
+                                                   k, mode);
+    else
+        k_fire<float><<<(unsigned)nb, kT, 0, s>>>(m, R, seed, model_id, rbase, epoch, (const float *)a, crit, tau, jc,
+                                                  k, mode);
+}
+
+void launch_update(const DevModel &m, int64_t R, const int32_t *x, const int32_t *k, int32_t *x_out, uint8_t *neg,
+                   const uint8_t *mode, hipStream_t s) {
+    dim3 g(blocks_for(R), row_grid(m.N));
+    k_update<<<g, kT, 0, s>>>(m, R, x, k, x_out, neg, mode);
+}
+
+void launch_ssa(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase, int32_t *x,
+                double *t, uint32_t *epoch, double t_final, int32_t n_events, int32_t *events, void *a,
+                hipStream_t s) {
+    if (is64)
+        k_ssa<double><<<blocks_for(R), kT, 0, s>>>(m, R, seed, model_id, rbase, x, t, epoch, t_final, n_events, events,
+                                                   (double *)a);
+    else
+        k_ssa<float><<<blocks_for(R), kT, 0, s>>>(m, R, seed, model_id, rbase, x, t, epoch, t_final, n_events, events,
+                                                  (float *)a);
+}
+
+void launch_fill_u8(uint8_t *p, uint8_t v, int64_t n, hipStream_t s) {
+    if (n > 0) k_fill_u8<<<blocks_for(n), kT, 0, s>>>(p, v, n);
+}
+void launch_fill_u64(unsigned long long *p, unsigned long long v, int64_t n, hipStream_t s) {
+    if (n > 0) k_fill_u64<<<blocks_for(n), kT, 0, s>>>(p, v, n);
+}
+
+void launch_plain_init(const DevModel &m, int64_t R, int32_t *x, PlainState st, hipStream_t s) {
+    k_plain_init<<<blocks_for(R), kT, 0, s>>>(m, R, x, st);
+}
+
+void launch_plain_select(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase,
+                         double t_final, int64_t max_iter, int32_t *x, void *a, const uint8_t *crit, const double *a0,
+                         const double *a0c, const double *tau1, PlainState st, hipStream_t s) {
+    if (is64)
+        k_plain_select<double><<<blocks_for(R), kT, 0, s>>>(m, R, seed, model_id, rbase, t_final, max_iter, x,
+                                                            (double *)a, crit, a0, a0c, tau1, st);
+    else
+        k_plain_select<float><<<blocks_for(R), kT, 0, s>>>(m, R, seed, model_id, rbase, t_final, max_iter, x,
+                                                           (float *)a, crit, a0, a0c, tau1, st);
+}
+
+void launch_plain_commit(const DevModel &m, int64_t R, int32_t *x, const int32_t *x_new, PlainState st,
+                         double t_final, hipStream_t s) {
+    dim3 g(blocks_for(R), row_grid(m.N));
+    k_plain_commit<<<g, kT, 0, s>>>(m, R, x, x_new, st, t_final);
+}
+
+void launch_gather_track(int64_t R, int32_t n_track, const int32_t *track, const int32_t *x, int32_t *out,
+                         hipStream_t s) {
+    k_gather_track<<<blocks_for(R), kT, 0, s>>>(R, n_track, track, x, out);
+}
+
+}  // namespace tl

@@ -1,0 +1,394 @@
+"""Pins of the CPU oracle (no GPU).
+
+The reference ships no tests or golden outputs, so the oracle is pinned by:
+  * known-answer vectors: Philox4x32-10 (Random123 KATs and rocRAND's
+    independent implementation), the dimer-decay stage values K1/K2;
+  * exact rational arithmetic on random tiny models built from the reaction
+    lists (not from the oracle's packed layouts);
+  * distributions: the Knuth branch against scipy's Poisson pmf, the normal
+    branch against the exact law of floor|N(m, m)|;
+  * closed forms: direct-method trajectories of the linear cyclic chain
+    against exp(T L), and likelihood-ratio sensitivities against the
+    Frechet derivative of exp(T L) and against finite differences with common
+    random numbers.
+"""
+import json
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+from scipy import linalg, stats
+
+import tl_oracle as O
+from helpers import oracle_of
+from tau_leaping_amd import models
+from tau_leaping_amd.models import Network, Reaction
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+# ------------------------------------------------------------------ RNG --
+def test_philox_random123_kat():
+    # Random123 kat_vectors for philox4x32-10
+    assert O.philox([0, 0, 0, 0], [0, 0]).tolist() == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert O.philox([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2).tolist() == [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert O.philox([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0]).tolist() == [
+        0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_philox_matches_rocrand():
+    cases = json.load(open(os.path.join(HERE, "golden", "philox_rocrand.json")))["cases"]
+    assert len(cases) == 64
+    for c in cases:
+        assert O.philox(c["ctr"], c["key"]).tolist() == c["out"]
+
+
+# ------------------------------------------------------------ dimer KATs --
+def test_dimer_k1():
+    """K1: S1 = 100000, t = 0 (values computed by hand from functions.h:100-123
+    and CSRMatrixPacked.cu:311-355)."""
+    net = models.dimer_decay(100000)
+    om = oracle_of(net)
+    x = net.x0.reshape(3, 1)
+    a, crit, a0, a0c = om.propensities(x, "fp64")
+    assert a.ravel().tolist() == [100000.0, 9999900.0, 0.0, 0.0]
+    assert crit.ravel().tolist() == [0, 0, 1, 1]
+    assert a0[0] == 10099900.0 and a0c[0] == 0.0
+    a32, _, _, _ = om.propensities(x, "fp32")
+    assert a32.ravel()[1] == np.float32(9999901.0)  # FP32 rounding of 0.002*1e5*99999/2
+    tau1, am, wh = om.tau(x, a, crit, 0.03)
+    # tau_S1 = 3000/20099800 = 1.49255e-4, tau_S2 = 1/9999900, tau_S3 = inf
+    assert tau1[0] == pytest.approx(1.000010000100001e-7, rel=1e-15)
+    assert am[0] == 1
+    assert 10.0 / a0[0] == pytest.approx(9.901088129585442e-7, rel=1e-15)
+    assert tau1[0] < 10.0 / a0[0]  # the direct branch is taken at t = 0
+
+
+def test_dimer_k2():
+    """K2: the file's S1 = 10000."""
+    net = models.dimer_decay(10000)
+    om = oracle_of(net)
+    x = net.x0.reshape(3, 1)
+    a, crit, a0, _ = om.propensities(x)
+    assert a0[0] == 109990.0
+    tau1, _, _ = om.tau(x, a, crit, 0.03)
+    assert tau1[0] == pytest.approx(1.0001000100010001e-5, rel=1e-15)
+    assert 10.0 / a0[0] == pytest.approx(9.091735612328394e-5, rel=1e-15)
+
+
+def test_dimer_tau_s1_species_value():
+    """tau_S1 of K1 becomes the minimum once S2 is non-critical and large."""
+    net = models.dimer_decay(100000)
+    om = oracle_of(net)
+    x = np.array([[100000], [0], [0]], np.int32)
+    a, crit, _, _ = om.propensities(x)
+    crit2 = crit.copy()
+    a2 = a.copy()
+    # drop S2's production row contribution by zeroing channel 2 (S2 consumer
+    # channels are already critical), so tau_S2 = inf and tau_S1 is the min
+    a2[1] = 0.0
+    tau1, am, _ = om.tau(x, a2, crit2, 0.03)
+    # mu_S1 = -1e5, sigma_S1 = 1e5 -> tau = min(3000/1e5, 9e6/1e5) = 0.03
+    assert am[0] == 0 and tau1[0] == pytest.approx(0.03, rel=1e-15)
+    tau1, _, _ = om.tau(x, a, crit, 0.03)
+    assert tau1[0] < 1.4925521646981562e-4  # tau_S1 itself is larger than tau_S2
+
+
+# ---------------------------------------------- exact rational stage checks --
+def _random_tiny(seed, with_catalyst=True):
+    rng = np.random.default_rng(seed)
+    N = int(rng.integers(3, 7))
+    M = int(rng.integers(4, 9))
+    rx = []
+    for _ in range(M):
+        t = int(rng.integers(1, 4))
+        sp = rng.permutation(N)
+        if t == 1:
+            reac = {int(sp[0]): 1}
+        elif t == 2:
+            reac = {int(sp[0]): 1, int(sp[1]): 1}
+        else:
+            reac = {int(sp[0]): 2}
+        prod = {}
+        for s in sp[2:2 + int(rng.integers(0, 3))]:
+            prod[int(s)] = int(rng.integers(1, 3))
+        if with_catalyst and rng.random() < 0.25:
+            s0 = next(iter(reac))
+            prod[s0] = prod.get(s0, 0) + 1
+        rx.append(Reaction(reac, prod, float(np.exp(rng.uniform(np.log(1e-3), 0)))))
+    return Network(N, rx, rng.integers(0, 50, size=N).astype(np.int32))
+
+
+def _exact_prop(rx, x):
+    c = Fraction(rx.rate)
+    items = list(rx.reactants.items())
+    if len(items) == 2:
+        (i, _), (j, _) = items
+        return c * x[i] * x[j], x[i] < 5 or x[j] < 5
+    (i, v), = items
+    if v == 2:
+        return c * x[i] * (x[i] - 1) / 2, 2 * x[i] < 5
+    return c * x[i], x[i] < 5
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_stages_exact_rational(seed):
+    net = _random_tiny(seed)
+    om = oracle_of(net)
+    rng = np.random.default_rng(100 + seed)
+    R = 16
+    x = rng.integers(0, 12, size=(net.n_species, R)).astype(np.int32)
+    a, crit, a0, a0c = om.propensities(x)
+    eps = Fraction(3, 100)
+    for r in range(R):
+        xr = [int(v) for v in x[:, r]]
+        ex = [_exact_prop(rx, xr) for rx in net.reactions]
+        for j, (aj, cj) in enumerate(ex):
+            assert a[j, r] == pytest.approx(float(aj), rel=5e-16, abs=0)  # two roundings
+            assert crit[j, r] == int(cj)
+        assert a0[r] == pytest.approx(float(sum(e[0] for e in ex)), rel=1e-15)
+        # mean / variance bound per species from the reaction lists; a species
+        # on both sides contributes a reactant and a product entry
+        best = None
+        for i in range(net.n_species):
+            mu = Fraction(0)
+            sig = Fraction(0)
+            for j, rx in enumerate(net.reactions):
+                if crit[j, r]:
+                    continue
+                aj = Fraction(float(a[j, r]))
+                vs = []
+                if i in rx.reactants:
+                    vs.append(-rx.reactants[i])
+                if i in rx.products:
+                    vs.append(rx.products[i])
+                for v in vs:
+                    mu += v * aj
+                    sig += v * v * aj
+            if i >= net.n_channels:
+                continue
+            o = net.reactions[i]  # GetG is indexed by species through the channel order array
+            kinds = list(o.reactants.values())
+            order = 2 if len(kinds) == 2 else (3 if kinds[0] == 2 else 1)
+            g = {1: Fraction(1), 2: Fraction(2), 3: None}[order]
+            xi = xr[i]
+            if order == 3:
+                g = None if xi == 1 else Fraction(2) + Fraction(1, xi - 1)
+            q = Fraction(0) if g is None else eps * xi / g
+            tmp = max(q, Fraction(1))
+            t1 = None if mu == 0 else tmp / abs(mu)
+            t2 = None if sig == 0 else tmp * tmp / abs(sig)
+            cands = [v for v in (t1, t2) if v is not None]
+            if cands:
+                ti = min(cands)
+                best = ti if best is None or ti < best else best
+        tau1, _, _ = om.tau(x[:, r:r + 1], a[:, r:r + 1], crit[:, r:r + 1], 0.03)
+        if best is None:
+            assert tau1[0] == pytest.approx(3.4028234663852886e38)
+        else:
+            assert tau1[0] == pytest.approx(float(best), rel=1e-13)
+    # integer update from the reaction lists
+    k = rng.integers(0, 5, size=(net.n_channels, R)).astype(np.int32)
+    xo, neg = om.update(x, k)
+    S = np.zeros((net.n_species, net.n_channels), np.int64)
+    for j, rx in enumerate(net.reactions):
+        for s, v in rx.reactants.items():
+            S[s, j] -= v
+        for s, v in rx.products.items():
+            S[s, j] += v
+    assert np.array_equal(xo, x + S @ k)
+    assert np.array_equal(neg, (xo < 0).any(axis=0).astype(np.uint8))
+
+
+def test_select_is_inverse_cdf():
+    net = models.toy_network(12, 30, seed=3)
+    om = oracle_of(net)
+    R = 400
+    rng = np.random.default_rng(2)
+    x = rng.integers(0, 9, size=(12, R)).astype(np.int32)
+    a, crit, a0, a0c = om.propensities(x)
+    tau1, _, _ = om.tau(x, a, crit, 0.03)
+    epoch = np.arange(R, dtype=np.uint32)
+    tau, jc, cl = om.select(7, epoch, a, crit, a0c, tau1, np.full(R, 1e9))
+    for r in range(R):
+        ctr = [0, 0, int(epoch[r]), 1]
+        key = [7, r]
+        u = O.philox(ctr, key)
+        r1 = (float(u[0]) + 0.5) / 2 ** 32
+        r2 = (float(u[1]) + 0.5) / 2 ** 32
+        w = np.where(crit[:, r] == 1, a[:, r], 0.0)
+        if a0c[r] > 0:
+            j = int(np.searchsorted(np.cumsum(w), a0c[r] * r1, side="right"))
+            tau2 = -np.log(r2) / a0c[r]
+        else:
+            j, tau2 = net.n_channels, 3.4028234663852886e38
+        fire = j < net.n_channels and tau2 < tau1[r]
+        assert jc[r] == (j if fire else -1)
+        assert tau[r] == pytest.approx(min(tau1[r], tau2), rel=1e-14)
+        assert cl[r] == 0
+
+
+# ----------------------------------------------------------- sampler laws --
+@pytest.mark.parametrize("mean", [0.3, 2.5, 11.0, 37.0, 79.5])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_knuth_branch_is_poisson(mean, prec):
+    om = oracle_of(models.dimer_decay())
+    n = 20000
+    k = np.array([om.poisson(mean, 11, rid=e % 97, epoch=e, channel=e % 5, precision=prec) for e in range(n)])
+    lo, hi = stats.poisson.ppf([1e-4, 1 - 1e-4], mean).astype(int)
+    edges = np.arange(lo, hi + 2)
+    obs = np.histogram(np.clip(k, lo, hi), bins=edges)[0]
+    pmf = stats.poisson.pmf(np.arange(lo, hi + 1), mean)
+    pmf[0] += stats.poisson.cdf(lo - 1, mean)
+    pmf[-1] += stats.poisson.sf(hi, mean)
+    exp = pmf * n
+    keep = exp >= 5
+    obs2 = np.append(obs[keep], obs[~keep].sum())
+    exp2 = np.append(exp[keep], exp[~keep].sum())
+    if exp2[-1] < 5:
+        obs2[-2] += obs2[-1]
+        exp2[-2] += exp2[-1]
+        obs2, exp2 = obs2[:-1], exp2[:-1]
+    p = stats.chisquare(obs2, exp2 * obs2.sum() / exp2.sum()).pvalue
+    assert p > 1e-3
+
+
+@pytest.mark.parametrize("mean", [80.0, 150.0, 500.0])
+def test_normal_branch_law(mean):
+    """k = floor(|Y|), Y ~ N(m, m) (model.cu:517): compare the sample mean and
+    variance with the exact moments of that law."""
+    om = oracle_of(models.dimer_decay())
+    n = 20000
+    k = np.array([om.poisson(mean, 5, rid=e % 89, epoch=e, channel=1) for e in range(n)], float)
+    s = np.sqrt(mean)
+    ks = np.arange(0, int(mean + 12 * s) + 2)
+    p = (stats.norm.cdf(ks + 1, mean, s) - stats.norm.cdf(ks, mean, s)
+         + stats.norm.cdf(-ks, mean, s) - stats.norm.cdf(-ks - 1, mean, s))
+    m1 = (ks * p).sum()
+    var = ((ks - m1) ** 2 * p).sum()
+    assert abs(k.mean() - m1) < 4 * np.sqrt(var / n)
+    assert abs(k.var() / var - 1) < 0.06
+
+
+# ---------------------------------------------------- cyclic chain closed form --
+def _cyclic_generator(c):
+    N = len(c)
+    L = np.zeros((N, N))
+    for i in range(N):
+        L[i, i] -= c[i]
+        L[i, (i + 1) % N] += c[i]
+    return L
+
+
+def _cyclic_net(N, c, x0):
+    rx = [Reaction({i: 1}, {(i + 1) % N: 1}, float(c[i])) for i in range(N)]
+    return Network(N, rx, np.asarray(x0, np.int32))
+
+
+def test_direct_method_cyclic_closed_form():
+    """Exact SSA (the fixed DirectMethodStep) on a linear cyclic chain: each
+    molecule walks the ring independently, so E[X_k(T)] = sum_i x0_i P_ik and
+    Var[X_k(T)] = sum_i x0_i P_ik (1 - P_ik) with P = exp(T L)."""
+    N, T = 6, 0.4
+    c = np.array([1.0, 0.5, 2.0, 1.5, 0.8, 1.2])
+    x0 = np.array([300, 0, 120, 50, 0, 200])
+    om = oracle_of(_cyclic_net(N, c, x0))
+    R = 4000
+    x = np.ascontiguousarray(np.repeat(x0[:, None], R, axis=1).astype(np.int32))
+    xo, to, _, ev = om.ssa(17, x, np.zeros(R), np.zeros(R, np.uint32), T, 10 ** 6)
+    assert (to == T).all()
+    P = linalg.expm(T * _cyclic_generator(c))
+    mean = x0 @ P
+    var = x0 @ (P * (1 - P))
+    se = np.sqrt(var / R)
+    assert np.all(np.abs(xo.mean(axis=1) - mean) < 4.5 * se)
+    assert np.all(np.abs(xo.var(axis=1, ddof=1) / var - 1) < 0.12)
+
+
+def test_tau_leap_cyclic_mean():
+    """Tau-leaping (eps = 0.03) stays within the Euler-step bias of the exact
+    mean on the cyclic chain."""
+    net = models.cyclic_chain(32, seed=8)
+    om = oracle_of(net)
+    res = om.simulate(3000, 0.5, seed=4)
+    P = linalg.expm(0.5 * _cyclic_generator(np.ones(32)))
+    mean = net.x0 @ P
+    var = net.x0 @ (P * (1 - P))
+    err = np.abs(res["x_final"].mean(axis=0) - mean)
+    assert np.all(err < 4.5 * np.sqrt(var / 3000) + 0.02 * mean)
+    assert (res["x_final"].sum(axis=1) == net.x0.sum()).all()
+
+
+def test_score_function_cyclic():
+    """Likelihood-ratio dE[X_k(T)]/dc_j against the Frechet derivative of
+    exp(T L) (the mean of the linear chain solves dm/dt = L^T m) and against
+    central finite differences with common random numbers."""
+    N, T = 5, 0.5
+    c = np.array([1.0, 0.7, 1.3, 0.9, 1.1])
+    x0 = np.array([900, 400, 1200, 600, 800])
+    net = _cyclic_net(N, c, x0)
+    om = oracle_of(net)
+    R = 20000
+    res = om.simulate(R, T, seed=21, sensitivities=True)
+    X = res["x_final"].astype(float)
+    S = res["score"]
+    lr = ((X - X.mean(0)).T @ S) / R  # [k][j]
+    lr_se = np.sqrt((((X - X.mean(0))[:, :, None] * S[:, None, :]) ** 2).mean(0) / R)
+    Lg = _cyclic_generator(c)
+    exact = np.zeros((N, N))
+    for j in range(N):
+        E = np.zeros((N, N))
+        E[j, j] = -1.0
+        E[j, (j + 1) % N] = 1.0
+        _, dP = linalg.expm_frechet(T * Lg, T * E)
+        exact[:, j] = x0 @ dP
+    assert np.all(np.abs(lr - exact) < 4.5 * lr_se + 0.03 * np.abs(exact).max())
+    # central differences with common random numbers on two channels
+    for j in (0, 2):
+        h = 1e-3 * c[j]
+        outs = []
+        for sgn in (+1, -1):
+            cc = c.copy()
+            cc[j] += sgn * h
+            o = oracle_of(_cyclic_net(N, cc, x0)).simulate(R, T, seed=21)
+            outs.append(o["x_final"].astype(float))
+        fd = (outs[0] - outs[1]) / (2 * h)
+        fd_mean = fd.mean(0)
+        fd_se = fd.std(0, ddof=1) / np.sqrt(R)
+        assert np.all(np.abs(lr[:, j] - fd_mean) < 4.5 * np.hypot(lr_se[:, j], fd_se) + 0.05 * np.abs(fd_mean).max())
+
+
+# --------------------------------------------------------------- misc --
+def test_grid_endpoints():
+    net = models.toy_network(10, 20, seed=1)
+    om = oracle_of(net)
+    res = om.simulate(50, 0.05, seed=2, n_intervals=4, track=[0, 1, 2])
+    g = res["grid"]
+    assert np.all(g[:, 0, :] == net.x0[[0, 1, 2]])
+    assert np.all(g[:, -1, :] == res["x_final"][:, [0, 1, 2]])
+
+
+def test_model_validation():
+    def build(reacs):
+        net = Network(3, reacs, np.array([1, 2, 3], np.int32))
+        return O.OracleModel(3, len(reacs), *net.to_csc())
+
+    build([Reaction({0: 1}, {1: 1}, 1.0)])
+    with pytest.raises(ValueError, match="status 3"):
+        build([Reaction({}, {1: 1}, 1.0)])  # zero-order source
+    with pytest.raises(ValueError, match="status 3"):
+        build([Reaction({0: 1, 1: 1, 2: 1}, {}, 1.0)])  # three reactants
+    with pytest.raises(ValueError, match="status 3"):
+        build([Reaction({0: 1, 1: 2}, {}, 1.0)])  # S + 2S'
+    with pytest.raises(ValueError, match="status 4"):
+        build([Reaction({0: 3}, {}, 1.0)])  # stoichiometry -3 does not pack in 3 bits
+    cat = Network(2, [Reaction({0: 1}, {0: 1, 1: 1}, 1.0)], np.array([5, 0], np.int32))
+    col_ptr, row_ind, stoich, _, _ = cat.to_csc()
+    assert row_ind.tolist() == [0, 0, 1] and stoich.tolist() == [-1, 1, 1]
+
+
+def test_iteration_cap_status():
+    om = oracle_of(models.toy_network(16, 32, seed=2))
+    res = om.simulate(20, 0.1, seed=1, max_iter=3)
+    assert (res["status"] == 8).all() and (res["steps"] == 3).all()
