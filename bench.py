@@ -61,6 +61,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1500)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 1 GiB device-to-device copy first (byte-counter calibration for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (realizations shard trivially across GPUs, "
@@ -74,6 +76,12 @@ def main():
     from tau_leaping_amd import models
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     net = models.toy_network(128, 256, seed=42)
     m = T.Model.from_network(net)
     R, tf = args.realizations, 0.1
@@ -120,7 +128,7 @@ def main():
     if not args.no_extras:
         extras = {}
         # plain per-model launch loop, bounded sample of the same workload
-        Rp = 20000
+        Rp = 2000
         t0 = time.perf_counter()
         rp = m.simulate(tf, Rp, mode="plain", **kw)
         dtp = time.perf_counter() - t0

@@ -50,6 +50,16 @@ static inline void tlo_key(uint64_t seed, int64_t rid, uint32_t key[2]) {
 
 static inline double tlo_unif64(uint32_t b) { return ((double)b + 0.5) * 2.3283064365386962890625e-10; }
 
+/* First draw of channel j at an epoch: output (j & 3) of the Philox block
+ * shared by channels 4b..4b+3, b = j >> 2, at counter (b, 0, epoch,
+ * ctr3 | TLO_KIND_HEAD). */
+static inline uint32_t tlo_head_draw(const uint32_t key[2], uint32_t ctr3, uint32_t epoch, int32_t j) {
+    uint32_t ctr[4] = {(uint32_t)j >> 2, 0u, epoch, ctr3 | TLO_KIND_HEAD};
+    uint32_t o[4];
+    tlo_philox4x32_10(ctr, key, o);
+    return o[j & 3];
+}
+
 /* ---- model build ---- */
 void tlo_model_destroy(tlo_model *m) {
     if (!m) return;

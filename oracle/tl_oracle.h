@@ -75,6 +75,7 @@ enum {
 #define TLO_KIND_POISSON 0u
 #define TLO_KIND_LEAP 1u
 #define TLO_KIND_SSA 2u
+#define TLO_KIND_HEAD 3u /* first draw of channels 4b..4b+3 */
 
 typedef struct tlo_model {
     int32_t N, M, nnz;

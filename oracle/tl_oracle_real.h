@@ -117,14 +117,22 @@ static double FN(tau_one)(const tlo_model *m, REAL eps, const int32_t *x, ptrdif
 }
 
 /* Knuth product (mean < 80, cap 300) or |polar Box-Muller| truncated
- * (model.cu:477-520, functions.h:37-54), drawing from the channel's stream. */
-static int32_t FN(poisson_stream)(REAL mean, const uint32_t key[2], uint32_t channel, uint32_t epoch,
-                                  uint32_t ctr3) {
+ * (model.cu:477-520, functions.h:37-54).  Draw 1 of the channel is `head`
+ * (see head_draw); draws 2, 3, ... come from the channel's own stream, output
+ * ((n-2) & 3) of Philox block (n-2) >> 2 at counter (block, channel, epoch,
+ * ctr3). */
+static int32_t FN(poisson_stream)(REAL mean, uint32_t head, const uint32_t key[2], uint32_t channel,
+                                  uint32_t epoch, uint32_t ctr3) {
     uint32_t blk[4];
     uint32_t ctr[4] = {0u, channel, epoch, ctr3};
-    int have = 0, pos = 0;
+    int have = 0, pos = 0, first = 1;
 #define NEXT_U(dst)                                   \
     do {                                              \
+        if (first) {                                  \
+            first = 0;                                \
+            (dst) = FN(unif)(head);                   \
+            break;                                    \
+        }                                             \
         if (pos == have) {                            \
             tlo_philox4x32_10(ctr, key, blk);         \
             ctr[0]++;                                 \
@@ -164,7 +172,8 @@ int32_t FN(tlo_poisson)(REAL mean, uint64_t seed, uint32_t model_id, int64_t rid
                         int32_t channel) {
     uint32_t key[2];
     tlo_key(seed, rid, key);
-    return FN(poisson_stream)(mean, key, (uint32_t)channel, epoch, (model_id << 2) | TLO_KIND_POISSON);
+    uint32_t head = tlo_head_draw(key, model_id << 2, epoch, channel);
+    return FN(poisson_stream)(mean, head, key, (uint32_t)channel, epoch, (model_id << 2) | TLO_KIND_POISSON);
 }
 
 /* Critical pick (GetReaction2Fire, model.cu:555-598): upper_bound over the
@@ -213,7 +222,8 @@ static void FN(fire_one)(const tlo_model *m, const uint32_t key[2], uint32_t ctr
         if (crit[(ptrdiff_t)j * as])
             kj = (j == jc) ? 1 : 0;
         else
-            kj = FN(poisson_stream)(a[(ptrdiff_t)j * as] * tr, key, (uint32_t)j, epoch, ctr3 | TLO_KIND_POISSON);
+            kj = FN(poisson_stream)(a[(ptrdiff_t)j * as] * tr, tlo_head_draw(key, ctr3, epoch, j), key, (uint32_t)j,
+                                    epoch, ctr3 | TLO_KIND_POISSON);
         k[(ptrdiff_t)j * ks] = kj;
     }
 }

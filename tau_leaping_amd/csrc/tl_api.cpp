@@ -131,6 +131,7 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     add(N * 4);
     add(M);
     add(N);
+    add(M * 8);  // rs: (s0 | order << 29, s1)
     std::vector<unsigned char> host(total, 0);
     std::vector<float> c32(M);
     for (size_t j = 0; j < M; ++j) c32[j] = (float)m.rates[j];
@@ -145,6 +146,13 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     memcpy(&host[off[8]], m.x0.data(), N * 4);
     memcpy(&host[off[9]], m.order.data(), M);
     memcpy(&host[off[10]], m.g_order.data(), N);
+    {
+        int32_t *rs = reinterpret_cast<int32_t *>(&host[off[11]]);
+        for (size_t j = 0; j < M; ++j) {
+            rs[2 * j] = m.s0[j] | ((int32_t)m.order[j] << 29);
+            rs[2 * j + 1] = m.s1[j];
+        }
+    }
 
     DeviceModelCache *c;
     if (it != m.dev.end()) {
@@ -178,6 +186,7 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     d.x0 = (const int32_t *)(b + off[8]);
     d.order = (const int8_t *)(b + off[9]);
     d.g_order = (const int8_t *)(b + off[10]);
+    d.rs = (const int2 *)(b + off[11]);
     c->version = m.version;
     out = d;
     if (upload_seconds) *upload_seconds = now_s() - t0;

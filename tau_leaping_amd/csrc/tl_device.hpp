@@ -25,9 +25,10 @@ constexpr int kKnuthCap = 300;         // Knuth returns 300 after 299 draws
 constexpr int kMaxRetries = 32;        // rejected leaps in a row before giving up
 constexpr float kFltMax = 3.402823466e+38f;
 
-constexpr uint32_t kKindPoisson = 0u;  // ctr[3] low bits: per-channel firing stream
+constexpr uint32_t kKindPoisson = 0u;  // ctr[3] low bits: per-channel firing stream (draws 2, 3, ...)
 constexpr uint32_t kKindLeap = 1u;     //                  leap scalars (r1, r2)
 constexpr uint32_t kKindSsa = 2u;      //                  direct-event scalars
+constexpr uint32_t kKindHead = 3u;     //                  first draw of channels 4b..4b+3 (block b)
 
 // Status codes (mirror tl_api.h)
 constexpr int kOk = 0;
@@ -49,6 +50,7 @@ struct DevModel {
     const int32_t *csc;      // [nnz] (species << 3) | ((v + 2) & 7)
     const int8_t *g_order;   // [N] order[i] for i < M else 0
     const int32_t *x0;       // [N]
+    const int2 *rs;          // [M] (s0 | order << 29, s1): one load per channel
 };
 
 // ---- Philox4x32-10 ----
@@ -133,18 +135,34 @@ __device__ __forceinline__ R tau_species(R eps, int32_t xi, int order_i, R mu, R
     return t1 < t2 ? t1 : t2;
 }
 
+// Head draw of channel j at an epoch: output (j & 3) of the Philox block
+// shared by channels 4b..4b+3, b = j >> 2.
+__device__ __forceinline__ uint4 head_block(uint2 key, uint32_t b, uint32_t epoch, uint32_t ctr3_model) {
+    return philox4x32_10(make_uint4(b, 0u, epoch, ctr3_model | kKindHead), key);
+}
+__device__ __forceinline__ uint32_t quad_pick(uint4 o, int s) {
+    return s == 0 ? o.x : (s == 1 ? o.y : (s == 2 ? o.z : o.w));
+}
+
 // Firing count for a non-critical channel: Knuth product below 80 (capped at
 // 300), else |mean + sqrt(mean) * z| truncated, z from the polar method.
-// Uniform n of the stream is output (n & 3) of Philox block n >> 2 at
-// counter (block, channel, epoch, ctr3).
+// Draw 1 of the channel is `head`; draw n >= 2 is output ((n-2) & 3) of the
+// channel's own Philox block (n-2) >> 2 at counter (block, channel, epoch,
+// ctr3).  The Knuth loop returns 0 on its first draw without evaluating exp
+// when head < 1 - mean - margin, a bound below the rounded 1/exp(mean) (the
+// result is the one the full loop returns).
 template <class R>
-__device__ __forceinline__ int32_t poisson_draw(R mean, uint2 key, uint32_t channel, uint32_t epoch,
-                                                uint32_t ctr3) {
+__device__ __forceinline__ int32_t poisson_draw(R mean, uint32_t head, uint2 key, uint32_t channel,
+                                                uint32_t epoch, uint32_t ctr3) {
     uint32_t blk = 0;
+    const R u1 = Real<R>::unif(head);
     if (mean < (R)kPoissonSwitch) {
+        const R margin = sizeof(R) == 8 ? (R)9.094947017729282e-13 : (R)9.5367431640625e-07;  // 2^-40 / 2^-20
+        if (mean < (R)1 && u1 < (R)1 - mean - margin) return 0;
         const R lam = (R)1 / Real<R>::exp_(mean);
-        R val = (R)1;
-        int32_t n = 1;
+        R val = u1;
+        if (val < lam) return 0;
+        int32_t n = 2;
         for (;;) {
             const uint4 o = philox4x32_10(make_uint4(blk++, channel, epoch, ctr3), key);
             val *= Real<R>::unif(o.x);
@@ -161,17 +179,24 @@ __device__ __forceinline__ int32_t poisson_draw(R mean, uint2 key, uint32_t chan
             if (++n == kKnuthCap) return kKnuthCap;
         }
     } else {
-        R u, s;
-        for (;;) {
-            const uint4 o = philox4x32_10(make_uint4(blk++, channel, epoch, ctr3), key);
-            u = 2 * Real<R>::unif(o.x) - 1;
-            R v = 2 * Real<R>::unif(o.y) - 1;
-            s = u * u + v * v;
-            if (s < 1) break;
-            u = 2 * Real<R>::unif(o.z) - 1;
-            v = 2 * Real<R>::unif(o.w) - 1;
-            s = u * u + v * v;
-            if (s < 1) break;
+        // pairs (head, t0), (t1, t2), (t3, t4), ... of the channel's draws
+        R u = 2 * u1 - 1;
+        uint4 o = philox4x32_10(make_uint4(blk++, channel, epoch, ctr3), key);
+        R v = 2 * Real<R>::unif(o.x) - 1;
+        R s = u * u + v * v;
+        if (s >= 1) {
+            for (;;) {
+                u = 2 * Real<R>::unif(o.y) - 1;
+                v = 2 * Real<R>::unif(o.z) - 1;
+                s = u * u + v * v;
+                if (s < 1) break;
+                const uint32_t w = o.w;
+                o = philox4x32_10(make_uint4(blk++, channel, epoch, ctr3), key);
+                u = 2 * Real<R>::unif(w) - 1;
+                v = 2 * Real<R>::unif(o.x) - 1;
+                s = u * u + v * v;
+                if (s < 1) break;
+            }
         }
         const R z = u * Real<R>::sqrt_(-2 * Real<R>::log_(s) / s);
         const R y = mean + Real<R>::sqrt_(mean) * z;

@@ -7,10 +7,11 @@
 // models with tens of thousands of channels).  Per group, LDS holds the
 // populations x[N], the proposed change dx[N] (integer LDS atomics, so the
 // update is exact and order-free), the criticality bits (one 64-bit ballot
-// per 64 channels) and, when it fits, the propensities a[M]; otherwise a[M]
-// lives in a per-group global scratch row.  Reductions (a0, a0c, tau min /
-// argmin, cumulative searches) are wavefront shuffles and ballots plus a
-// fixed-order combine across waves, so results do not depend on timing.
+// per 64 channels), a staging buffer for the first random draw of 4*GS
+// channels and, when it fits, the propensities a[M]; otherwise a[M] lives in
+// a per-group global scratch row.  Reductions (a0, a0c, tau min / argmin,
+// cumulative searches) are wavefront shuffles and ballots plus a fixed-order
+// combine across waves, so results do not depend on timing.
 //
 // The per-realization algorithm follows PerformCalc / Step
 // (tau_leaping/model.cu:301-413, 640-737) with the defect fixes of the CPU
@@ -26,15 +27,17 @@ __host__ __device__ inline size_t al16(size_t v) { return (v + 15) & ~(size_t)15
 
 // LDS layout of one group.
 struct GroupLayout {
-    size_t off_x, off_dx, off_snap, off_crit, off_red, off_a, bytes;
+    size_t off_x, off_dx, off_snap, off_head, off_crit, off_red, off_a, bytes;
 };
-__host__ __device__ inline GroupLayout group_layout(int32_t n_max, int32_t m_max, int gs, bool a_lds, size_t real_bytes) {
+__host__ __device__ inline GroupLayout group_layout(int32_t n_max, int32_t m_max, int gs, bool a_lds,
+                                                    size_t real_bytes) {
     GroupLayout L;
     const int nw = gs / 64;
     L.off_x = 0;
     L.off_dx = al16((size_t)n_max * 4);
     L.off_snap = L.off_dx + al16((size_t)n_max * 4);
-    L.off_crit = L.off_snap + al16((size_t)n_max * 4);
+    L.off_head = L.off_snap + al16((size_t)n_max * 4);
+    L.off_crit = L.off_head + (size_t)gs * 16;
     L.off_red = L.off_crit + al16((size_t)((m_max + 63) / 64) * 8);
     L.off_a = L.off_red + al16((size_t)(4 * nw + 4) * 8);
     L.bytes = L.off_a + (a_lds ? al16((size_t)m_max * real_bytes) : 0);
@@ -60,6 +63,13 @@ __device__ __forceinline__ unsigned long long uni(unsigned long long v) {
     const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(v & 0xffffffffull));
     const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(v >> 32));
     return ((unsigned long long)hi << 32) | lo;
+}
+// Value of lane `l` (uniform) of a wave-varying double.
+__device__ __forceinline__ double readlane(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 // Load a struct that is the same for every lane into scalar registers.
 template <class T>
@@ -235,77 +245,88 @@ struct Grp {
         }
     }
 
-    // First j with inclusive cumulative weight > target; weight = a_j, or
-    // a_j masked by the criticality bits.  -1 if the total never exceeds it.
+    // First j with inclusive cumulative weight > target; weight = a_j, or a_j
+    // of critical channels only.  -1 if no such j (total <= target).  Lane l
+    // sums the contiguous channels [l*C, (l+1)*C); one group scan over the
+    // lane sums picks the lane, which then walks its own channels.
     template <class R>
     __device__ int32_t search(const R *a, const uint64_t *critq, bool masked, int32_t M, double target) const {
-        const int32_t nt = (M + 63) >> 6;
+        const int32_t C = (M + GS - 1) / GS;
+        const int32_t j0 = min(M, lane * C), j1 = min(M, j0 + C);
+        double s = 0.0;
+        for (int32_t j = j0; j < j1; ++j)
+            if (!masked || crit_bit(critq, j)) s += (double)a[j];
+        const double incl_w = wave_incl_scan(s, wl);
+        double base = 0.0;
+        if constexpr (GS != 64) {
+            if (wl == 63) red[wid] = incl_w;
+            __syncthreads();
+            for (int w = 0; w < wid; ++w) base += red[w];
+            __syncthreads();
+        }
+        const double incl = base + incl_w;
+        // the shuffle runs on every lane: a lane that is masked off does not
+        // provide its value to the others
+        const double up = __shfl_up(incl_w, 1, 64);
+        const double excl = base + (wl == 0 ? 0.0 : up);
+        const unsigned long long hit = __ballot(incl > target);
+        int32_t res = -1;
         if constexpr (GS == 64) {
-            return tile_search<R>(a, critq, masked, M, 0, nt, 0.0, target);
-        } else {
-            const int32_t tpw = (nt + NW - 1) / NW;
-            const int32_t tb = wid * tpw, te = min(nt, tb + tpw);
-            double part = 0.0;
-            for (int32_t tt = tb; tt < te; ++tt) {
-                if (masked && critq[tt] == 0ull) continue;
-                const int32_t j = tt * 64 + wl;
-                if (j < M && (!masked || ((critq[tt] >> wl) & 1ull))) part += (double)a[j];
-            }
-            part = wave_sum(part);
-            if (wl == 0) red[wid] = part;
-            __syncthreads();
-            double pre = 0.0, wbase = 0.0;
-            int ws = -1;
-            for (int w = 0; w < NW; ++w) {
-                const double nx = pre + red[w];
-                if (ws < 0 && nx > target) {
-                    ws = w;
-                    wbase = pre;
-                }
-                pre = nx;
-            }
-            int *slot = reinterpret_cast<int *>(red + 2 * NW);
-            if (wid == ws) {
-                int32_t f = tile_search<R>(a, critq, masked, M, tb, te, wbase, target);
-                if (f < 0) {  // rounding at the range end: last positive weight of this range
-                    int last = -1;
-                    for (int32_t tt = tb; tt < te; ++tt) {
-                        const int32_t j = tt * 64 + wl;
-                        if (j < M && a[j] > (R)0 && (!masked || ((critq[tt] >> wl) & 1ull))) last = j;
+            if (hit == 0ull) return -1;
+            const int L = (int)__builtin_ctzll(hit);
+            if (wl == L) {
+                double cum = excl;
+                int32_t last = -1;
+                for (int32_t j = j0; j < j1; ++j) {
+                    if (masked && !crit_bit(critq, j)) continue;
+                    const double w = (double)a[j];
+                    if (w > 0.0) last = j;
+                    cum += w;
+                    if (cum > target) {
+                        res = j;
+                        break;
                     }
-                    f = wave_max_int(last);
                 }
-                if (wl == 0) slot[0] = f;
+                if (res < 0) res = last;  // rounding at the chunk end
+            }
+            return uni(__builtin_amdgcn_readlane(res, L));
+        } else {
+            int *slot = reinterpret_cast<int *>(red + 2 * NW);
+            if (lane == 0) slot[0] = 1 << 30;
+            __syncthreads();
+            if (hit != 0ull && wl == 0) atomicMin(&slot[0], wid * 64 + (int)__builtin_ctzll(hit));
+            __syncthreads();
+            const int L = slot[0];
+            if (lane == L) {
+                double cum = excl;
+                int32_t last = -1;
+                for (int32_t j = j0; j < j1; ++j) {
+                    if (masked && !crit_bit(critq, j)) continue;
+                    const double w = (double)a[j];
+                    if (w > 0.0) last = j;
+                    cum += w;
+                    if (cum > target) {
+                        res = j;
+                        break;
+                    }
+                }
+                slot[1] = (res < 0) ? last : res;
             }
             __syncthreads();
-            const int32_t res = (ws < 0) ? -1 : slot[0];
+            const int32_t out = (L >= (1 << 30)) ? -1 : slot[1];
             __syncthreads();
-            return uni(res);
+            return uni(out);
         }
-    }
-
-    template <class R>
-    __device__ __forceinline__ int32_t tile_search(const R *a, const uint64_t *critq, bool masked, int32_t M, int32_t tb,
-                                                   int32_t te, double base, double target) const {
-        for (int32_t tt = tb; tt < te; ++tt) {
-            uint64_t cm = ~0ull;
-            if (masked) {
-                cm = critq[tt];
-                if (cm == 0ull) continue;
-            }
-            const int32_t j = tt * 64 + wl;
-            double w = 0.0;
-            if (j < M && ((cm >> wl) & 1ull)) w = (double)a[j];
-            const double inc = wave_incl_scan(w, wl);
-            const unsigned long long hit = __ballot(j < M && base + inc > target);
-            if (hit) return uni(tt * 64 + (int32_t)__builtin_ctzll(hit));
-            base += uni(__shfl(inc, 63, 64));
-        }
-        return -1;
     }
 };
 
 // ----------------------------------------------------------------- passes --
+__device__ __forceinline__ void chan_unpack(int2 p, int &order, int32_t &s0, int32_t &s1) {
+    order = (int)((uint32_t)p.x >> 29);
+    s0 = p.x & 0x1fffffff;
+    s1 = p.y;
+}
+
 // Propensities, criticality bits, and the lane partial sums of a0 / a0c.
 template <class R, int GS>
 __device__ __forceinline__ void prop_pass(const DevModel &m, const R *__restrict__ c, const int32_t *xs, R *a,
@@ -316,7 +337,10 @@ __device__ __forceinline__ void prop_pass(const DevModel &m, const R *__restrict
         const int32_t j = j0 + lane;
         bool cr = false;
         if (j < m.M) {
-            const R aj = propensity<R>(m.order[j], c[j], xs[m.s0[j]], xs[m.s1[j]], cr);
+            int order;
+            int32_t s0, s1;
+            chan_unpack(m.rs[j], order, s0, s1);
+            const R aj = propensity<R>(order, c[j], xs[s0], xs[s1], cr);
             a[j] = aj;
             s += (double)aj;
             if (cr) sc += (double)aj;
@@ -333,7 +357,10 @@ __device__ __forceinline__ double prop_sum_pass(const DevModel &m, const R *__re
     double s = 0.0;
     for (int32_t j = lane; j < m.M; j += GS) {
         bool cr;
-        const R aj = propensity<R>(m.order[j], c[j], xs[m.s0[j]], xs[m.s1[j]], cr);
+        int order;
+        int32_t s0, s1;
+        chan_unpack(m.rs[j], order, s0, s1);
+        const R aj = propensity<R>(order, c[j], xs[s0], xs[s1], cr);
         a[j] = aj;
         s += (double)aj;
     }
@@ -350,6 +377,34 @@ __device__ __forceinline__ double score_term(bool cr, R aj, R tr, double tau, in
         return d + 0.5 * (d * d / mj - 1.0);
     }
     return (double)kj - mj;
+}
+
+// Visit every channel of one leap attempt with its firing count k_j.  The
+// first draws of channels [c0, c0 + 4*GS) are produced one Philox block per
+// lane (4 channels per block) into the LDS staging buffer, then each lane
+// samples its channels c0 + lane + GS*i.
+template <class R, int GS, class F>
+__device__ __forceinline__ void for_each_firing(const DevModel &m, const Grp<GS> &G, const R *a, const uint64_t *critq,
+                                                uint4 *head, uint2 key, uint32_t epoch, uint32_t c3, R tr,
+                                                int32_t jfire, F &&f) {
+    const uint32_t *hflat = reinterpret_cast<const uint32_t *>(head);
+    const uint32_t ck = c3 | kKindPoisson;
+    for (int32_t c0 = 0; c0 < m.M; c0 += 4 * GS) {
+        if (c0 + 4 * G.lane < m.M) head[G.lane] = head_block(key, (uint32_t)((c0 >> 2) + G.lane), epoch, c3);
+        G.sync();
+#pragma unroll 1
+        for (int i = 0; i < 4; ++i) {
+            const int32_t j = c0 + G.lane + GS * i;
+            if (j < m.M) {
+                const bool cr = crit_bit(critq, j);
+                const R aj = a[j];
+                const int32_t kj = cr ? (j == jfire ? 1 : 0)
+                                      : poisson_draw<R>(aj * tr, hflat[G.lane + GS * i], key, (uint32_t)j, epoch, ck);
+                f(j, cr, aj, kj);
+            }
+        }
+        G.sync();
+    }
 }
 
 // ------------------------------------------------------------ one item --
@@ -376,7 +431,7 @@ __device__ __forceinline__ void grid_record(const ModelItem &mi, int64_t r, int 
 
 template <class R, int GS, bool SENS>
 __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, int32_t *xs, int32_t *dx,
-                              int32_t *snap, uint64_t *critq, R *a) {
+                              int32_t *snap, uint4 *head, uint64_t *critq, R *a) {
     const DevModel &m = mi.dm;
     const int lane = G.lane;
     const R *c = Real<R>::rates(m);
@@ -407,6 +462,11 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
     int32_t status = kOk;
     const bool grid = mi.n_intervals > 0;
     int32_t next_g = 0;
+    // direct-event uniforms, 64 events at a time: lane l holds r1 and -ln(r2)
+    // of the event at epoch ssa_base + l
+    uint32_t ssa_base = 0;
+    bool ssa_valid = false;
+    double ssa_r1 = 0.0, ssa_nl = 0.0;
 
     while (t < tf) {
         if (mi.max_iter > 0 && steps >= mi.max_iter) {
@@ -469,10 +529,18 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                         t = tf;
                         break;
                     }
-                    double r1, r2;
-                    step_uniforms(key, epoch, c3 | kKindSsa, r1, r2);
+                    if (!ssa_valid || epoch - ssa_base >= 64u) {
+                        ssa_base = epoch;
+                        ssa_valid = true;
+                        double r1l, r2l;
+                        step_uniforms(key, epoch + (uint32_t)G.wl, c3 | kKindSsa, r1l, r2l);
+                        ssa_r1 = r1l;
+                        ssa_nl = -log(r2l);
+                    }
+                    const int li = (int)uni(epoch - ssa_base);
+                    const double r1 = uni(readlane(ssa_r1, li));
+                    const double dt = uni(readlane(ssa_nl, li)) / a0s;
                     epoch++;
-                    const double dt = -log(r2) / a0s;
                     if (dt >= tf - t) {
                         if (SENS) {  // survival over rem = tf - t: d/dc_j of -a0*rem
                             const double rem = tf - t;
@@ -529,40 +597,32 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
             }
             const int32_t jfire = fire ? jc : -1;
             const R tr = (R)tau;
-            const uint32_t ck = c3 | kKindPoisson;
             double qpart = 0.0;
-            for (int32_t j = lane; j < m.M; j += GS) {
-                const bool cr = crit_bit(critq, j);
-                const R aj = a[j];
-                const int32_t kj = cr ? (j == jfire ? 1 : 0) : poisson_draw<R>(aj * tr, key, (uint32_t)j, epoch, ck);
-                if (kj) {
-                    const int32_t e1 = m.col_ptr[j + 1];
-                    for (int32_t e = m.col_ptr[j]; e < e1; ++e) {
-                        const int32_t p = m.csc[e];
-                        atomicAdd(&dx[entry_index(p)], entry_value(p) * kj);
-                    }
-                }
-                if (SENS) {
-                    const double term = score_term<R>(cr, aj, tr, tau, kj);
-                    qpart += term;
-                    if (c[j] != (R)0) S[j] += term / (double)c[j];
-                }
-            }
-            G.sync();
+            for_each_firing<R, GS>(m, G, a, critq, head, key, epoch, c3, tr, jfire,
+                                   [&](int32_t j, bool cr, R aj, int32_t kj) {
+                                       if (kj) {
+                                           const int32_t e1 = m.col_ptr[j + 1];
+                                           for (int32_t e = m.col_ptr[j]; e < e1; ++e) {
+                                               const int32_t p = m.csc[e];
+                                               atomicAdd(&dx[entry_index(p)], entry_value(p) * kj);
+                                           }
+                                       }
+                                       if (SENS) {
+                                           const double term = score_term<R>(cr, aj, tr, tau, kj);
+                                           qpart += term;
+                                           if (c[j] != (R)0) S[j] += term / (double)c[j];
+                                       }
+                                   });
             bool neg = false;
             for (int32_t i = lane; i < m.N; i += GS) neg |= (xs[i] + dx[i]) < 0;
             neg = G.any(neg);
             if (neg) {
                 for (int32_t i = lane; i < m.N; i += GS) dx[i] = 0;
-                if (SENS) {  // take back this attempt's score terms (same draws)
-                    for (int32_t j = lane; j < m.M; j += GS) {
-                        const bool cr = crit_bit(critq, j);
-                        const R aj = a[j];
-                        const int32_t kj =
-                            cr ? (j == jfire ? 1 : 0) : poisson_draw<R>(aj * tr, key, (uint32_t)j, epoch, ck);
-                        if (c[j] != (R)0) S[j] -= score_term<R>(cr, aj, tr, tau, kj) / (double)c[j];
-                    }
-                }
+                if (SENS)  // take back this attempt's score terms (same draws)
+                    for_each_firing<R, GS>(m, G, a, critq, head, key, epoch, c3, tr, jfire,
+                                           [&](int32_t j, bool cr, R aj, int32_t kj) {
+                                               if (c[j] != (R)0) S[j] -= score_term<R>(cr, aj, tr, tau, kj) / (double)c[j];
+                                           });
                 epoch++;
                 nrej++;
                 G.sync();
@@ -654,6 +714,7 @@ __global__ __launch_bounds__(BS) void k_persistent(PersistentLaunch L) {
     int32_t *xs = reinterpret_cast<int32_t *>(base + lay.off_x);
     int32_t *dx = reinterpret_cast<int32_t *>(base + lay.off_dx);
     int32_t *snap = reinterpret_cast<int32_t *>(base + lay.off_snap);
+    uint4 *head = reinterpret_cast<uint4 *>(base + lay.off_head);
     uint64_t *critq = reinterpret_cast<uint64_t *>(base + lay.off_crit);
     G.red = reinterpret_cast<double *>(base + lay.off_red);
     R *a;
@@ -674,7 +735,7 @@ __global__ __launch_bounds__(BS) void k_persistent(PersistentLaunch L) {
             else hi = mid - 1;
         }
         const ModelItem mi = ld_uniform(&L.items[uni(lo)]);
-        simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, critq, a);
+        simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, head, critq, a);
     }
 }
 

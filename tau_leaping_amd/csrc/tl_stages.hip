@@ -182,11 +182,13 @@ __global__ __launch_bounds__(kT) void k_fire(DevModel m, int64_t Rn, uint64_t se
     if (mode && mode[r] != 1) return;
     const int64_t o = (int64_t)j * Rn + r;
     int32_t kj;
-    if (crit[o])
+    if (crit[o]) {
         kj = (jc[r] == j) ? 1 : 0;
-    else
-        kj = poisson_draw<R>(a[o] * (R)tau[r], rng_key(seed, rbase + r), (uint32_t)j, epoch[r],
-                             (model_id << 2) | kKindPoisson);
+    } else {
+        const uint2 key = rng_key(seed, rbase + r);
+        const uint32_t head = quad_pick(head_block(key, (uint32_t)j >> 2, epoch[r], model_id << 2), j & 3);
+        kj = poisson_draw<R>(a[o] * (R)tau[r], head, key, (uint32_t)j, epoch[r], (model_id << 2) | kKindPoisson);
+    }
     k[o] = kj;
 }
 
