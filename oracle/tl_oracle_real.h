@@ -88,16 +88,21 @@ static double FN(tau_one)(const tlo_model *m, REAL eps, const int32_t *x, ptrdif
     REAL best = (REAL)FLT_MAX;
     int32_t bi = -1, bw = 0;
     for (int32_t i = 0; i < m->N; ++i) {
-        REAL mu = 0, sig = 0;
+        /* the row sums run in FP64 in both variants and are rounded once to
+         * REAL (the FP32 variant's only departure from all-FP32 arithmetic:
+         * v * a_j and v^2 * a_j are exact in FP64, so the rounded sums no
+         * longer depend on the order in which a device adds them up) */
+        double mu_d = 0, sig_d = 0;
         for (int32_t e = m->row_ptr[i]; e < m->row_ptr[i + 1]; ++e) {
             int32_t p = m->csr_packed[e];
             int32_t j = p >> 3;
-            REAL v = (REAL)((p & 7) - 2);
-            REAL tmp = v * a[(ptrdiff_t)j * as] * (REAL)(!crit[(ptrdiff_t)j * as]);
-            sig += v * tmp;
-            mu += tmp;
+            double v = (double)((p & 7) - 2);
+            double tmp = v * (double)a[(ptrdiff_t)j * as] * (double)(!crit[(ptrdiff_t)j * as]);
+            sig_d += v * tmp;
+            mu_d += tmp;
         }
         if (i >= m->M) continue; /* tau_i = +inf */
+        REAL mu = (REAL)mu_d, sig = (REAL)sig_d;
         int32_t xi = x[(ptrdiff_t)i * xs];
         REAL g = FN(getg)(m->g_order[i], xi);
         REAL q = eps * (REAL)xi / g;

@@ -112,7 +112,8 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
         return TL_OK;
     }
     const double t0 = now_s();
-    // one allocation: c64 | c32 | s0 | s1 | row_ptr | csr | col_ptr | csc | x0 | order | g_order
+    // one allocation: c64 | c32 | s0 | s1 | row_ptr | csr | col_ptr | csc | x0 | order | g_order | rs |
+    // dep_ptr | dep | colq | aff_ptr | aff
     const size_t M = (size_t)m.M, N = (size_t)m.N, Z = (size_t)std::max(m.nnz, 1);
     std::vector<size_t> off;
     size_t total = 0;
@@ -132,6 +133,12 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     add(M);
     add(N);
     add(M * 8);  // rs: (s0 | order << 29, s1)
+    const size_t ND = std::max<size_t>(m.dep.size(), 1);
+    add((N + 1) * 4);  // dep_ptr
+    add(ND * 4);       // dep
+    add(std::max<size_t>(m.colq.size(), 1) * 4);     // colq
+    add((size_t)std::max<size_t>(m.aff_ptr.size(), 1) * 4);  // aff_ptr
+    add(std::max<size_t>(m.aff.size(), 1) * 4);      // aff
     std::vector<unsigned char> host(total, 0);
     std::vector<float> c32(M);
     for (size_t j = 0; j < M; ++j) c32[j] = (float)m.rates[j];
@@ -153,6 +160,11 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
             rs[2 * j + 1] = m.s1[j];
         }
     }
+    memcpy(&host[off[12]], m.dep_ptr.data(), (N + 1) * 4);
+    if (!m.dep.empty()) memcpy(&host[off[13]], m.dep.data(), m.dep.size() * 4);
+    if (!m.colq.empty()) memcpy(&host[off[14]], m.colq.data(), m.colq.size() * 4);
+    if (!m.aff_ptr.empty()) memcpy(&host[off[15]], m.aff_ptr.data(), m.aff_ptr.size() * 4);
+    if (!m.aff.empty()) memcpy(&host[off[16]], m.aff.data(), m.aff.size() * 4);
 
     DeviceModelCache *c;
     if (it != m.dev.end()) {
@@ -187,6 +199,12 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     d.order = (const int8_t *)(b + off[9]);
     d.g_order = (const int8_t *)(b + off[10]);
     d.rs = (const int2 *)(b + off[11]);
+    d.wave_rows = (m.nnz > 32 * m.N) ? 1 : 0;  // mean row length above 32 entries
+    d.dep_ptr = (const int32_t *)(b + off[12]);
+    d.dep = (const int32_t *)(b + off[13]);
+    d.colq = m.colq.empty() ? nullptr : (const int4 *)(b + off[14]);
+    d.aff_ptr = m.aff_ptr.empty() ? nullptr : (const int32_t *)(b + off[15]);
+    d.aff = m.aff_ptr.empty() ? nullptr : (const int32_t *)(b + off[16]);
     c->version = m.version;
     out = d;
     if (upload_seconds) *upload_seconds = now_s() - t0;
@@ -530,11 +548,14 @@ tl_status autotune(DeviceContext &dc, bool is64, bool sens, const std::vector<Mo
     const int64_t sample = std::min<int64_t>(n_items, (int64_t)dc.n_cu * 16);
     if (cands.size() > 1 && n_items >= 4 * sample) {
         std::vector<ModelItem> trial = items_in;
-        for (auto &mi : trial) mi.max_iter = (mi.max_iter > 0) ? std::min<int64_t>(mi.max_iter, 24) : 24;
+        for (auto &mi : trial) mi.max_iter = (mi.max_iter > 0) ? std::min<int64_t>(mi.max_iter, 128) : 128;
         ModelItem *d_trial = nullptr;
         TL_HIP(hipMalloc(&d_trial, trial.size() * sizeof(ModelItem)));
         hipMemcpy(d_trial, trial.data(), trial.size() * sizeof(ModelItem), hipMemcpyHostToDevice);
-        double best_rate = -1.0;
+        // the heuristic shape is kept unless another is clearly faster: the
+        // trial covers only the first steps of a sample
+        const PersistentConfig first = best;
+        double best_rate = -1.0, first_rate = -1.0;
         for (auto &c : cands) {
             float ms = 0.f;
             tl_status st = launch_batch(dc, c, is64, sens, d_trial, (int32_t)trial.size(), sample, n_max, m_max,
@@ -552,12 +573,14 @@ tl_status autotune(DeviceContext &dc, bool is64, bool sens, const std::vector<Mo
                 left -= take;
             }
             const double rate = ms > 0 ? (double)done / ms : 0.0;
+            if (c.block == first.block && c.group == first.group && c.a_lds == first.a_lds) first_rate = rate;
             if (rate > best_rate) {
                 best_rate = rate;
                 best = c;
             }
         }
         hipFree(d_trial);
+        if (first_rate > 0 && best_rate < 1.15 * first_rate) best = first;
     }
     (void)d_steps_all;
     std::lock_guard<std::mutex> g(g_tune_mu);

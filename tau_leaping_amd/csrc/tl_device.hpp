@@ -51,17 +51,23 @@ struct DevModel {
     const int8_t *g_order;   // [N] order[i] for i < M else 0
     const int32_t *x0;       // [N]
     const int2 *rs;          // [M] (s0 | order << 29, s1): one load per channel
+    int32_t wave_rows;       // species rows are long: reduce each row with a whole wavefront
+    const int32_t *dep_ptr;  // [N+1] species -> channels with it as a reactant
+    const int32_t *dep;      // [dep_ptr[N]]
+    const int4 *colq;        // [M] column entries padded with v = 0, or null (a column longer than 4)
+    const int32_t *aff_ptr;  // [M+1] channel -> channels whose propensity it can change, or null
+    const int32_t *aff;      // [aff_ptr[M]]
 };
 
 // ---- Philox4x32-10 ----
 __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        const uint32_t lo0 = 0xD2511F53u * c.x;
-        const uint32_t hi0 = __umulhi(0xD2511F53u, c.x);
-        const uint32_t lo1 = 0xCD9E8D57u * c.z;
-        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z);
-        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+        // one 32x32->64 product per multiplier (v_mad_u64_u32)
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        c = make_uint4((uint32_t)(p1 >> 32) ^ c.y ^ k.x, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k.y,
+                       (uint32_t)p0);
         k.x += 0x9E3779B9u;
         k.y += 0xBB67AE85u;
     }
@@ -100,18 +106,17 @@ template <> struct Real<float> {
 
 // Propensity of one channel; operation order (c*x0)*x1 and ((c*x0)*(x0-1))/2
 // as in IsCriticalAndPropensity.
+// Written without branches (channels of different orders share a wave); the
+// roundings are those of the three formulas: c*x0, (c*x0)*f, then an exact
+// halving for order 3.
 template <class R>
 __device__ __forceinline__ R propensity(int order, R c, int32_t x0, int32_t x1, bool &crit) {
-    if (order == 1) {
-        crit = x0 < kNc;
-        return c * (R)x0;
-    } else if (order == 2) {
-        crit = (x0 < kNc) || (x1 < kNc);
-        return c * (R)x0 * (R)x1;
-    } else {
-        crit = x0 * 2 < kNc;
-        return c * (R)x0 * (R)(x0 - 1) / (R)2;
-    }
+    const bool o2 = order == 2, o3 = order == 3;
+    crit = o3 ? (x0 * 2 < kNc) : ((x0 < kNc) || (o2 && x1 < kNc));
+    const R t = c * (R)x0;
+    const R f = (R)(o2 ? x1 : x0 - 1);
+    const R t2 = (o2 || o3) ? t * f : t;
+    return o3 ? t2 * (R)0.5 : t2;
 }
 
 template <class R>

@@ -464,6 +464,57 @@ tl_status build_model(tl_model &m, std::string &msg) {
         }
     m.g_order.assign(m.N, 0);
     for (int32_t i = 0; i < m.N && i < m.M; ++i) m.g_order[i] = m.order[i];
+    // reactant-of lists: the channels whose propensity changes with x_i
+    m.dep_ptr.assign(m.N + 1, 0);
+    for (int32_t j = 0; j < m.M; ++j) {
+        m.dep_ptr[m.s0[j] + 1]++;
+        if (m.s1[j] != m.s0[j]) m.dep_ptr[m.s1[j] + 1]++;
+    }
+    for (int32_t i = 0; i < m.N; ++i) m.dep_ptr[i + 1] += m.dep_ptr[i];
+    m.dep.assign(m.dep_ptr[m.N], 0);
+    {
+        std::vector<int32_t> f(m.dep_ptr.begin(), m.dep_ptr.end() - 1);
+        for (int32_t j = 0; j < m.M; ++j) {
+            m.dep[f[m.s0[j]]++] = j;
+            if (m.s1[j] != m.s0[j]) m.dep[f[m.s1[j]]++] = j;
+        }
+    }
+    // fixed-width columns
+    int32_t max_col = 0;
+    for (int32_t j = 0; j < m.M; ++j) max_col = std::max(max_col, m.col_ptr[j + 1] - m.col_ptr[j]);
+    m.colq.clear();
+    if (max_col <= 4) {
+        m.colq.assign((size_t)m.M * 4, 2);
+        for (int32_t j = 0; j < m.M; ++j)
+            for (int32_t e = m.col_ptr[j]; e < m.col_ptr[j + 1]; ++e) m.colq[(size_t)j * 4 + (e - m.col_ptr[j])] = m.csc[e];
+    }
+    // affected-channel lists (union of the reactant-of lists of the column)
+    {
+        constexpr size_t kMaxAffected = (size_t)1 << 24;
+        std::vector<int32_t> stamp(m.M, -1);
+        std::vector<int32_t> ptr(m.M + 1, 0), list;
+        bool ok = true;
+        for (int32_t j = 0; j < m.M && ok; ++j) {
+            const size_t start = list.size();
+            for (int32_t e = m.col_ptr[j]; e < m.col_ptr[j + 1]; ++e) {
+                const int32_t i = m.row_ind[e];
+                for (int32_t d = m.dep_ptr[i]; d < m.dep_ptr[i + 1]; ++d)
+                    if (stamp[m.dep[d]] != j) {
+                        stamp[m.dep[d]] = j;
+                        list.push_back(m.dep[d]);
+                    }
+            }
+            std::sort(list.begin() + start, list.end());
+            ptr[j + 1] = (int32_t)list.size();
+            if (list.size() > kMaxAffected) ok = false;
+        }
+        m.aff_ptr.clear();
+        m.aff.clear();
+        if (ok) {
+            m.aff_ptr.swap(ptr);
+            m.aff.swap(list);
+        }
+    }
     if (m.names.empty()) {
         m.names.resize(m.N);
         for (int32_t i = 0; i < m.N; ++i) m.names[i] = "S" + std::to_string(i + 1);

@@ -54,6 +54,14 @@ struct tl_model {
     std::vector<int32_t> row_ptr, csr;  // species-major, (channel << 3) | ((v + 2) & 7)
     std::vector<int32_t> csc;           // channel-major, (species << 3) | ((v + 2) & 7)
     std::vector<int8_t> g_order;        // order[i] for i < M else 0
+    std::vector<int32_t> dep_ptr, dep;  // species i -> channels with i as a reactant
+    // [M][4] column entries padded with (0 << 3) | 2 (v = 0); empty unless
+    // every column has at most 4 entries
+    std::vector<int32_t> colq;
+    // channel j -> channels with a reactant among j's column species (the
+    // propensities one firing of j can change); empty when it would exceed
+    // kMaxAffected entries
+    std::vector<int32_t> aff_ptr, aff;
 
     uint64_t version = 1;  // bumped by set_rates / set_x0; invalidates device copies
     mutable std::mutex dev_mu;

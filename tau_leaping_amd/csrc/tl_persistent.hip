@@ -27,8 +27,9 @@ __host__ __device__ inline size_t al16(size_t v) { return (v + 15) & ~(size_t)15
 
 // LDS layout of one group.
 struct GroupLayout {
-    size_t off_x, off_dx, off_snap, off_head, off_crit, off_red, off_a, bytes;
+    size_t off_x, off_dx, off_snap, off_head, off_crit, off_red, off_mu, off_sig, off_a, bytes;
 };
+// One-wavefront groups also carry the species accumulators mu[N], sig[N].
 __host__ __device__ inline GroupLayout group_layout(int32_t n_max, int32_t m_max, int gs, bool a_lds,
                                                     size_t real_bytes) {
     GroupLayout L;
@@ -39,7 +40,10 @@ __host__ __device__ inline GroupLayout group_layout(int32_t n_max, int32_t m_max
     L.off_head = L.off_snap + al16((size_t)n_max * 4);
     L.off_crit = L.off_head + (size_t)gs * 16;
     L.off_red = L.off_crit + al16((size_t)((m_max + 63) / 64) * 8);
-    L.off_a = L.off_red + al16((size_t)(4 * nw + 4) * 8);
+    L.off_mu = L.off_red + al16((size_t)(4 * nw + 4) * 8);
+    const size_t musig = (gs == 64) ? al16((size_t)n_max * 8) : 0;
+    L.off_sig = L.off_mu + musig;
+    L.off_a = L.off_sig + musig;
     L.bytes = L.off_a + (a_lds ? al16((size_t)m_max * real_bytes) : 0);
     return L;
 }
@@ -84,40 +88,118 @@ __device__ __forceinline__ T ld_uniform(const T *p) {
 }
 
 // ------------------------------------------------------- wave primitives --
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+// Scans and reductions over a wavefront with DPP lane moves (no LDS round
+// trip): an inclusive scan shifts within rows of 16 lanes by 1, 2, 4, 8, then
+// broadcasts lane 15 into the next row (row_bcast:15 on rows 1 and 3) and
+// lane 31 into rows 2-3 (row_bcast:31); lane 63 then holds the total.  Lanes
+// without a source read the identity.  The order of additions is fixed, so
+// every run sums identically.
+constexpr int kDppRowShr = 0x110;
+constexpr int kDppBcast15 = 0x142;
+constexpr int kDppBcast31 = 0x143;
+constexpr int kDppWaveShr1 = 0x138;
+
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ int dpp_i(int v, int ident) {
+    return __builtin_amdgcn_update_dpp(ident, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ double dpp_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_i<CTRL, ROWS>((int)(b & 0xffffffffll), 0);
+    const int hi = dpp_i<CTRL, ROWS>((int)(b >> 32), 0);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ float dpp_f(float v, float ident) {
+    return __int_as_float(dpp_i<CTRL, ROWS>(__float_as_int(v), __float_as_int(ident)));
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ double dpp_d_ident(double v, double ident) {
+    const long long b = __double_as_longlong(v), id = __double_as_longlong(ident);
+    const int lo = dpp_i<CTRL, ROWS>((int)(b & 0xffffffffll), (int)(id & 0xffffffffll));
+    const int hi = dpp_i<CTRL, ROWS>((int)(b >> 32), (int)(id >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-__device__ __forceinline__ double wave_incl_scan(double v, int wl) {
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const double y = __shfl_up(v, o, 64);
-        if (wl >= o) v += y;
-    }
+__device__ __forceinline__ double wave_incl_scan(double v, int /*wl*/) {
+    v += dpp_d<kDppRowShr | 1>(v);
+    v += dpp_d<kDppRowShr | 2>(v);
+    v += dpp_d<kDppRowShr | 4>(v);
+    v += dpp_d<kDppRowShr | 8>(v);
+    v += dpp_d<kDppBcast15, 0xa>(v);
+    v += dpp_d<kDppBcast31, 0xc>(v);
     return v;
+}
+// Value of the previous lane (0.0 in lane 0).
+__device__ __forceinline__ double wave_shift_up(double v) { return dpp_d<kDppWaveShr1>(v); }
+
+// Wave total, the same in every lane (scalar register).
+__device__ __forceinline__ double wave_sum(double v) { return readlane(wave_incl_scan(v, 0), 63); }
+
+__device__ __forceinline__ float wave_incl_scan_f(float v) {
+    v += dpp_f<kDppRowShr | 1>(v, 0.f);
+    v += dpp_f<kDppRowShr | 2>(v, 0.f);
+    v += dpp_f<kDppRowShr | 4>(v, 0.f);
+    v += dpp_f<kDppRowShr | 8>(v, 0.f);
+    v += dpp_f<kDppBcast15, 0xa>(v, 0.f);
+    v += dpp_f<kDppBcast31, 0xc>(v, 0.f);
+    return v;
+}
+template <class R>
+__device__ __forceinline__ R wave_total(R v) {
+    if constexpr (sizeof(R) == 8)
+        return wave_sum(v);
+    else
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wave_incl_scan_f(v)), 63));
 }
 
 template <class R>
-__device__ __forceinline__ void wave_argmin(R &v, int &i, int &w) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const R ov = __shfl_xor(v, o, 64);
-        const int oi = __shfl_xor(i, o, 64);
-        const int ow = __shfl_xor(w, o, 64);
-        if (ov < v || (ov == v && oi < i)) {
-            v = ov;
-            i = oi;
-            w = ow;
-        }
+__device__ __forceinline__ void argmin_step(R &v, int &i, int &w, R ov, int oi, int ow) {
+    if (ov < v || (ov == v && oi < i)) {
+        v = ov;
+        i = oi;
+        w = ow;
     }
+}
+template <int CTRL, int ROWS, class R>
+__device__ __forceinline__ void argmin_dpp(R &v, int &i, int &w) {
+    constexpr int kImax = 0x7fffffff;
+    R ov;
+    if constexpr (sizeof(R) == 8)
+        ov = dpp_d_ident<CTRL, ROWS>(v, (double)INFINITY);
+    else
+        ov = dpp_f<CTRL, ROWS>(v, INFINITY);
+    const int oi = dpp_i<CTRL, ROWS>(i, kImax);
+    const int ow = dpp_i<CTRL, ROWS>(w, 0);
+    argmin_step<R>(v, i, w, ov, oi, ow);
+}
+// Minimum value with the smallest index among ties; result in every lane.
+template <class R>
+__device__ __forceinline__ void wave_argmin(R &v, int &i, int &w) {
+    argmin_dpp<kDppRowShr | 1, 0xf, R>(v, i, w);
+    argmin_dpp<kDppRowShr | 2, 0xf, R>(v, i, w);
+    argmin_dpp<kDppRowShr | 4, 0xf, R>(v, i, w);
+    argmin_dpp<kDppRowShr | 8, 0xf, R>(v, i, w);
+    argmin_dpp<kDppBcast15, 0xa, R>(v, i, w);
+    argmin_dpp<kDppBcast31, 0xc, R>(v, i, w);
+    if constexpr (sizeof(R) == 8)
+        v = readlane((double)v, 63);
+    else
+        v = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+    i = __builtin_amdgcn_readlane(i, 63);
+    w = __builtin_amdgcn_readlane(w, 63);
 }
 
 __device__ __forceinline__ int wave_max_int(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
+    constexpr int kImin = (int)0x80000000;
+    v = max(v, dpp_i<kDppRowShr | 1>(v, kImin));
+    v = max(v, dpp_i<kDppRowShr | 2>(v, kImin));
+    v = max(v, dpp_i<kDppRowShr | 4>(v, kImin));
+    v = max(v, dpp_i<kDppRowShr | 8>(v, kImin));
+    v = max(v, dpp_i<kDppBcast15, 0xa>(v, kImin));
+    v = max(v, dpp_i<kDppBcast31, 0xc>(v, kImin));
+    return __builtin_amdgcn_readlane(v, 63);
 }
 
 __device__ __forceinline__ bool crit_bit(const uint64_t *critq, int32_t j) { return (critq[j >> 6] >> (j & 63)) & 1ull; }
@@ -245,30 +327,57 @@ struct Grp {
         }
     }
 
-    // First j with inclusive cumulative weight > target; weight = a_j, or a_j
-    // of critical channels only.  -1 if no such j (total <= target).  Lane l
-    // sums the contiguous channels [l*C, (l+1)*C); one group scan over the
-    // lane sums picks the lane, which then walks its own channels.
+    // Cumulative weights in channel order: lane l owns the contiguous channels
+    // [l*C, (l+1)*C) and sums them; a group scan over the lane sums gives each
+    // lane's inclusive / exclusive prefix.  Weight = a_j, or a_j of critical
+    // channels only.  Returns the total (the same in every lane).
+    struct Scan {
+        double incl, excl;
+        int32_t j0, j1;
+    };
     template <class R>
-    __device__ int32_t search(const R *a, const uint64_t *critq, bool masked, int32_t M, double target) const {
+    __device__ double scan_chunks(const R *a, const uint64_t *critq, bool masked, int32_t M, Scan &sc) const {
         const int32_t C = (M + GS - 1) / GS;
-        const int32_t j0 = min(M, lane * C), j1 = min(M, j0 + C);
+        sc.j0 = min(M, lane * C);
+        sc.j1 = min(M, sc.j0 + C);
         double s = 0.0;
-        for (int32_t j = j0; j < j1; ++j)
+        for (int32_t j = sc.j0; j < sc.j1; ++j)
             if (!masked || crit_bit(critq, j)) s += (double)a[j];
         const double incl_w = wave_incl_scan(s, wl);
-        double base = 0.0;
+        double base = 0.0, total;
         if constexpr (GS != 64) {
             if (wl == 63) red[wid] = incl_w;
             __syncthreads();
-            for (int w = 0; w < wid; ++w) base += red[w];
+            total = 0.0;
+            for (int w = 0; w < NW; ++w) {
+                if (w == wid) base = total;
+                total += red[w];
+            }
             __syncthreads();
+        } else {
+            total = readlane(incl_w, 63);
         }
-        const double incl = base + incl_w;
-        // the shuffle runs on every lane: a lane that is masked off does not
-        // provide its value to the others
-        const double up = __shfl_up(incl_w, 1, 64);
-        const double excl = base + (wl == 0 ? 0.0 : up);
+        sc.incl = base + incl_w;
+        // runs on every lane: a lane that is masked off does not provide its
+        // value to the others
+        sc.excl = base + wave_shift_up(incl_w);
+        return uni(total);
+    }
+
+    // First j with inclusive cumulative weight > target; -1 if no such j
+    // (total <= target).  The first lane whose prefix passes the target walks
+    // its own channels.
+    template <class R>
+    __device__ int32_t search(const R *a, const uint64_t *critq, bool masked, int32_t M, double target) const {
+        Scan sc;
+        scan_chunks<R>(a, critq, masked, M, sc);
+        return pick<R>(a, critq, masked, sc, target);
+    }
+
+    template <class R>
+    __device__ int32_t pick(const R *a, const uint64_t *critq, bool masked, const Scan &sc, double target) const {
+        const int32_t j0 = sc.j0, j1 = sc.j1;
+        const double incl = sc.incl, excl = sc.excl;
         const unsigned long long hit = __ballot(incl > target);
         int32_t res = -1;
         if constexpr (GS == 64) {
@@ -327,10 +436,30 @@ __device__ __forceinline__ void chan_unpack(int2 p, int &order, int32_t &s0, int
     s1 = p.y;
 }
 
+// Adds of one column entry p of a channel with propensity aj to the species
+// sums (v * aj, v^2 * aj; padding entries have v = 0).
+__device__ __forceinline__ void musig_add(double *mu, double *sig, int32_t p, double aj) {
+    const int32_t vi = entry_value(p);
+    if (vi != 0) {
+        const double v = (double)vi, tmp = v * aj;
+        atomicAdd(&mu[entry_index(p)], tmp);
+        atomicAdd(&sig[entry_index(p)], v * tmp);
+    }
+}
+
+// x[i] += v * k for one column entry p.
+__device__ __forceinline__ void column_add(int32_t *x, int32_t p, int32_t k) {
+    const int32_t v = entry_value(p);
+    if (v != 0) atomicAdd(&x[entry_index(p)], v * k);
+}
+
 // Propensities, criticality bits, and the lane partial sums of a0 / a0c.
-template <class R, int GS>
+// With MUSIG, each non-critical channel also adds v*a_j and v*(v*a_j) of its
+// column entries to the species accumulators mu[i], sig[i] (LDS atomics;
+// within one wavefront the order of the adds is fixed).
+template <class R, int GS, bool MUSIG>
 __device__ __forceinline__ void prop_pass(const DevModel &m, const R *__restrict__ c, const int32_t *xs, R *a,
-                                          uint64_t *critq, int lane, double &s, double &sc) {
+                                          uint64_t *critq, double *mu, double *sig, int lane, double &s, double &sc) {
     s = 0.0;
     sc = 0.0;
     for (int32_t j0 = 0; j0 < m.M; j0 += GS) {
@@ -344,6 +473,18 @@ __device__ __forceinline__ void prop_pass(const DevModel &m, const R *__restrict
             a[j] = aj;
             s += (double)aj;
             if (cr) sc += (double)aj;
+            if (MUSIG && !cr && aj != (R)0) {
+                if (m.colq) {
+                    const int4 q = m.colq[j];
+                    musig_add(mu, sig, q.x, (double)aj);
+                    musig_add(mu, sig, q.y, (double)aj);
+                    musig_add(mu, sig, q.z, (double)aj);
+                    musig_add(mu, sig, q.w, (double)aj);
+                } else {
+                    const int32_t e1 = m.col_ptr[j + 1];
+                    for (int32_t e = m.col_ptr[j]; e < e1; ++e) musig_add(mu, sig, m.csc[e], (double)aj);
+                }
+            }
         }
         const unsigned long long bal = __ballot(cr);
         const int32_t tile0 = j0 + (lane & ~63);
@@ -351,21 +492,39 @@ __device__ __forceinline__ void prop_pass(const DevModel &m, const R *__restrict
     }
 }
 
+// Propensities of the channels whose reactants include a species of
+// channel jf's column (after jf fired once); the values equal a full
+// recomputation.
 template <class R, int GS>
-__device__ __forceinline__ double prop_sum_pass(const DevModel &m, const R *__restrict__ c, const int32_t *xs, R *a,
-                                                int lane) {
-    double s = 0.0;
-    for (int32_t j = lane; j < m.M; j += GS) {
-        bool cr;
-        int order;
-        int32_t s0, s1;
-        chan_unpack(m.rs[j], order, s0, s1);
-        const R aj = propensity<R>(order, c[j], xs[s0], xs[s1], cr);
-        a[j] = aj;
-        s += (double)aj;
+__device__ __forceinline__ void prop_update(const DevModel &m, const R *__restrict__ c, const int32_t *xs, R *a,
+                                            int32_t jf, int lane) {
+    if (m.aff_ptr) {
+        const int32_t d1 = m.aff_ptr[jf + 1];
+        for (int32_t d = m.aff_ptr[jf] + lane; d < d1; d += GS) {
+            const int32_t k = m.aff[d];
+            int order;
+            int32_t s0, s1;
+            bool cr;
+            chan_unpack(m.rs[k], order, s0, s1);
+            a[k] = propensity<R>(order, c[k], xs[s0], xs[s1], cr);
+        }
+        return;
     }
-    return s;
+    const int32_t e1 = m.col_ptr[jf + 1];
+    for (int32_t e = m.col_ptr[jf]; e < e1; ++e) {
+        const int32_t i = entry_index(m.csc[e]);
+        const int32_t d1 = m.dep_ptr[i + 1];
+        for (int32_t d = m.dep_ptr[i] + lane; d < d1; d += GS) {
+            const int32_t k = m.dep[d];
+            int order;
+            int32_t s0, s1;
+            bool cr;
+            chan_unpack(m.rs[k], order, s0, s1);
+            a[k] = propensity<R>(order, c[k], xs[s0], xs[s1], cr);
+        }
+    }
 }
+
 
 // Score term d_j * m_j of one channel (see the likelihood-ratio notes in
 // simulate_item).
@@ -431,7 +590,9 @@ __device__ __forceinline__ void grid_record(const ModelItem &mi, int64_t r, int 
 
 template <class R, int GS, bool SENS>
 __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, int32_t *xs, int32_t *dx,
-                              int32_t *snap, uint4 *head, uint64_t *critq, R *a) {
+                              int32_t *snap, uint4 *head, uint64_t *critq, double *mu, double *sig, R *a) {
+    // one-wavefront groups scatter the species sums during the propensity pass
+    constexpr bool MUSIG = (GS == 64);
     const DevModel &m = mi.dm;
     const int lane = G.lane;
     const R *c = Real<R>::rates(m);
@@ -448,6 +609,10 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
     for (int32_t i = lane; i < m.N; i += GS) {
         xs[i] = m.x0[i];
         dx[i] = 0;
+        if (MUSIG) {
+            mu[i] = 0.0;
+            sig[i] = 0.0;
+        }
     }
     if (SENS)
         for (int32_t j = lane; j < m.M; j += GS) {
@@ -477,7 +642,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
         if (grid)
             for (int32_t q = lane; q < mi.n_track; q += GS) snap[q] = xs[mi.track[q]];
         double a0, a0c;
-        prop_pass<R, GS>(m, c, xs, a, critq, lane, a0, a0c);
+        prop_pass<R, GS, MUSIG>(m, c, xs, a, critq, mu, sig, lane, a0, a0c);
         G.sum2(a0, a0c);
         G.sync();
         if (!(a0 > 0.0)) {  // absorbing state
@@ -489,24 +654,69 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
         // tau1: per species mean / variance bound, min and argmin
         R best = (R)kFltMax;
         int bi = -1, bw = 0;
-        for (int32_t i = lane; i < m.N; i += GS) {
-            R mu = 0, sig = 0;
-            const int32_t e1 = m.row_ptr[i + 1];
-            for (int32_t e = m.row_ptr[i]; e < e1; ++e) {
-                const int32_t p = m.csr[e];
-                const int32_t j = entry_index(p);
-                const R v = (R)entry_value(p);
-                const R tmp = v * a[j] * (R)(!crit_bit(critq, j));
-                sig += v * tmp;
-                mu += tmp;
+        if constexpr (MUSIG) {
+            // species sums were scattered by the propensity pass
+            for (int32_t i = lane; i < m.N; i += GS) {
+                const R mui = (R)mu[i], sgi = (R)sig[i];
+                mu[i] = 0.0;
+                sig[i] = 0.0;
+                if (i < m.M) {
+                    int w;
+                    const R ti = tau_species<R>(eps, xs[i], m.g_order[i], mui, sgi, w);
+                    if (ti < best) {
+                        best = ti;
+                        bi = i;
+                        bw = w;
+                    }
+                }
             }
-            if (i < m.M) {
-                int w;
-                const R ti = tau_species<R>(eps, xs[i], m.g_order[i], mu, sig, w);
-                if (ti < best) {
-                    best = ti;
-                    bi = i;
-                    bw = w;
+        } else if (m.wave_rows) {
+            // long rows: one wavefront per species row, lanes stride its entries
+            for (int32_t i = G.wid; i < m.N; i += Grp<GS>::NW) {
+                const int32_t e0 = m.row_ptr[i], e1 = m.row_ptr[i + 1];
+                double mu = 0, sig = 0;
+                for (int32_t e = e0 + G.wl; e < e1; e += 64) {
+                    const int32_t p = m.csr[e];
+                    const int32_t j = entry_index(p);
+                    const double v = (double)entry_value(p);
+                    const double tmp = v * (double)a[j] * (double)(!crit_bit(critq, j));
+                    sig += v * tmp;
+                    mu += tmp;
+                }
+                mu = wave_total<double>(mu);
+                sig = wave_total<double>(sig);
+                if (i < m.M) {
+                    int w;
+                    const R ti = tau_species<R>(eps, xs[i], m.g_order[i], (R)mu, (R)sig, w);
+                    if (ti < best) {
+                        best = ti;
+                        bi = i;
+                        bw = w;
+                    }
+                }
+            }
+        } else {
+            // short rows: one lane per species row, in the row's channel order
+            for (int32_t i = lane; i < m.N; i += GS) {
+                double mu = 0, sig = 0;
+                const int32_t e0 = m.row_ptr[i], e1 = m.row_ptr[i + 1];
+#pragma unroll 4
+                for (int32_t e = e0; e < e1; ++e) {
+                    const int32_t p = m.csr[e];
+                    const int32_t j = entry_index(p);
+                    const double v = (double)entry_value(p);
+                    const double tmp = v * (double)a[j] * (double)(!crit_bit(critq, j));
+                    sig += v * tmp;
+                    mu += tmp;
+                }
+                if (i < m.M) {
+                    int w;
+                    const R ti = tau_species<R>(eps, xs[i], m.g_order[i], (R)mu, (R)sig, w);
+                    if (ti < best) {
+                        best = ti;
+                        bi = i;
+                        bw = w;
+                    }
                 }
             }
         }
@@ -518,13 +728,11 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
         int retries = 0;
         for (;;) {
             if (tau1 < kSsaFactor / a0) {
-                // direct branch: up to 100 exact events, propensities refreshed
-                double a0s = a0;
+                // direct branch: up to 100 exact events; after each event only
+                // the channels that read a changed species are recomputed
                 for (int s = 0; s < kSsaSteps; ++s) {
-                    if (s > 0) {
-                        a0s = G.sum(prop_sum_pass<R, GS>(m, c, xs, a, lane));
-                        G.sync();
-                    }
+                    typename Grp<GS>::Scan sc;
+                    const double a0s = G.template scan_chunks<R>(a, critq, false, m.M, sc);
                     if (!(a0s > 0.0)) {
                         t = tf;
                         break;
@@ -550,7 +758,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                         t = tf;
                         break;
                     }
-                    int32_t jf = G.template search<R>(a, critq, false, m.M, a0s * r1);
+                    int32_t jf = G.template pick<R>(a, critq, false, sc, a0s * r1);
                     if (jf < 0) {
                         int last = -1;
                         for (int32_t j = lane; j < m.M; j += GS)
@@ -562,11 +770,14 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                             if (c[j] != (R)0)
                                 S[j] += ((j == jf) ? 1.0 / (double)c[j] : 0.0) - (double)a[j] * dt / (double)c[j];
                     }
-                    const int32_t e1 = m.col_ptr[jf + 1];
-                    for (int32_t e = m.col_ptr[jf] + lane; e < e1; e += GS) {
-                        const int32_t p = m.csc[e];
-                        atomicAdd(&xs[entry_index(p)], entry_value(p));
+                    if (m.colq) {
+                        if (lane < 4) column_add(xs, reinterpret_cast<const int32_t *>(m.colq)[(size_t)jf * 4 + lane], 1);
+                    } else {
+                        const int32_t e1 = m.col_ptr[jf + 1];
+                        for (int32_t e = m.col_ptr[jf] + lane; e < e1; e += GS) column_add(xs, m.csc[e], 1);
                     }
+                    G.sync();
+                    prop_update<R, GS>(m, c, xs, a, jf, lane);
                     t += dt;
                     nssa++;
                     G.sync();
@@ -601,10 +812,15 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
             for_each_firing<R, GS>(m, G, a, critq, head, key, epoch, c3, tr, jfire,
                                    [&](int32_t j, bool cr, R aj, int32_t kj) {
                                        if (kj) {
-                                           const int32_t e1 = m.col_ptr[j + 1];
-                                           for (int32_t e = m.col_ptr[j]; e < e1; ++e) {
-                                               const int32_t p = m.csc[e];
-                                               atomicAdd(&dx[entry_index(p)], entry_value(p) * kj);
+                                           if (m.colq) {
+                                               const int4 q = m.colq[j];
+                                               column_add(dx, q.x, kj);
+                                               column_add(dx, q.y, kj);
+                                               column_add(dx, q.z, kj);
+                                               column_add(dx, q.w, kj);
+                                           } else {
+                                               const int32_t e1 = m.col_ptr[j + 1];
+                                               for (int32_t e = m.col_ptr[j]; e < e1; ++e) column_add(dx, m.csc[e], kj);
                                            }
                                        }
                                        if (SENS) {
@@ -700,7 +916,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
 }
 
 template <class R, int BS, int GS, bool ALDS, bool SENS>
-__global__ __launch_bounds__(BS) void k_persistent(PersistentLaunch L) {
+__global__ __launch_bounds__(BS, (GS == 64 ? 4 : 1)) void k_persistent(PersistentLaunch L) {
     extern __shared__ __align__(16) unsigned char lds[];
     constexpr int NG = BS / GS;
     const GroupLayout lay = group_layout(L.n_max, L.m_max, GS, ALDS, sizeof(R));
@@ -717,6 +933,8 @@ __global__ __launch_bounds__(BS) void k_persistent(PersistentLaunch L) {
     uint4 *head = reinterpret_cast<uint4 *>(base + lay.off_head);
     uint64_t *critq = reinterpret_cast<uint64_t *>(base + lay.off_crit);
     G.red = reinterpret_cast<double *>(base + lay.off_red);
+    double *mu = reinterpret_cast<double *>(base + lay.off_mu);
+    double *sig = reinterpret_cast<double *>(base + lay.off_sig);
     R *a;
     if constexpr (ALDS)
         a = reinterpret_cast<R *>(base + lay.off_a);
@@ -735,7 +953,7 @@ __global__ __launch_bounds__(BS) void k_persistent(PersistentLaunch L) {
             else hi = mid - 1;
         }
         const ModelItem mi = ld_uniform(&L.items[uni(lo)]);
-        simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, head, critq, a);
+        simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, head, critq, mu, sig, a);
     }
 }
 

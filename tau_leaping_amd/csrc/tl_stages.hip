@@ -77,20 +77,20 @@ __global__ __launch_bounds__(kT) void k_tau_rows(DevModel m, int64_t Rn, R eps, 
     if (r >= Rn) return;
     if (active && !active[r]) return;
     for (int32_t i = blockIdx.y; i < m.N; i += gridDim.y) {
-        R mu = 0, sig = 0;
+        double mu = 0, sig = 0;  // FP64 sums, rounded once to R
         const int32_t e1 = m.row_ptr[i + 1];
         for (int32_t e = m.row_ptr[i]; e < e1; ++e) {
             const int32_t p = m.csr[e];
             const int32_t j = entry_index(p);
-            const R v = (R)entry_value(p);
-            const R tmp = v * a[(int64_t)j * Rn + r] * (R)(!crit[(int64_t)j * Rn + r]);
+            const double v = (double)entry_value(p);
+            const double tmp = v * (double)a[(int64_t)j * Rn + r] * (double)(!crit[(int64_t)j * Rn + r]);
             sig += v * tmp;
             mu += tmp;
         }
         R ti;
         int w = 0;
         if (i < m.M)
-            ti = tau_species<R>(eps, x[(int64_t)i * Rn + r], m.g_order[i], mu, sig, w);
+            ti = tau_species<R>(eps, x[(int64_t)i * Rn + r], m.g_order[i], (R)mu, (R)sig, w);
         else
             ti = (R)INFINITY;
         tau_i[(int64_t)i * Rn + r] = ti;
