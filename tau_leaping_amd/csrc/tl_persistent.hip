@@ -25,15 +25,21 @@ namespace {
 
 __host__ __device__ inline size_t al16(size_t v) { return (v + 15) & ~(size_t)15; }
 
+// LDS budget of the per-wavefront species accumulators of one group.
+constexpr size_t kMusigBudget = 65536;
+
 // LDS layout of one group.
 struct GroupLayout {
     size_t off_x, off_dx, off_snap, off_head, off_crit, off_red, off_mu, off_sig, off_a, bytes;
+    bool musig;  // species sums scattered by the propensity pass (else gathered by rows)
 };
-// One-wavefront groups also carry the species accumulators mu[N], sig[N].
+// Groups carry one pair of species accumulators mu[N], sig[N] per wavefront
+// when they fit the budget (always for one-wavefront groups).
 __host__ __device__ inline GroupLayout group_layout(int32_t n_max, int32_t m_max, int gs, bool a_lds,
                                                     size_t real_bytes) {
     GroupLayout L;
     const int nw = gs / 64;
+    L.musig = gs == 64 || (size_t)nw * (size_t)n_max * 16 <= kMusigBudget;
     L.off_x = 0;
     L.off_dx = al16((size_t)n_max * 4);
     L.off_snap = L.off_dx + al16((size_t)n_max * 4);
@@ -41,7 +47,7 @@ __host__ __device__ inline GroupLayout group_layout(int32_t n_max, int32_t m_max
     L.off_crit = L.off_head + (size_t)gs * 16;
     L.off_red = L.off_crit + al16((size_t)((m_max + 63) / 64) * 8);
     L.off_mu = L.off_red + al16((size_t)(4 * nw + 4) * 8);
-    const size_t musig = (gs == 64) ? al16((size_t)n_max * 8) : 0;
+    const size_t musig = L.musig ? al16((size_t)nw * (size_t)n_max * 8) : 0;
     L.off_sig = L.off_mu + musig;
     L.off_a = L.off_sig + musig;
     L.bytes = L.off_a + (a_lds ? al16((size_t)m_max * real_bytes) : 0);
@@ -455,11 +461,13 @@ __device__ __forceinline__ void column_add(int32_t *x, int32_t p, int32_t k) {
 
 // Propensities, criticality bits, and the lane partial sums of a0 / a0c.
 // With MUSIG, each non-critical channel also adds v*a_j and v*(v*a_j) of its
-// column entries to the species accumulators mu[i], sig[i] (LDS atomics;
-// within one wavefront the order of the adds is fixed).
-template <class R, int GS, bool MUSIG>
+// column entries to its wavefront's species accumulators mu[i], sig[i] (LDS
+// atomics; within one wavefront the order of the adds is fixed, so the sums
+// repeat bit for bit).
+template <class R, int GS>
 __device__ __forceinline__ void prop_pass(const DevModel &m, const R *__restrict__ c, const int32_t *xs, R *a,
-                                          uint64_t *critq, double *mu, double *sig, int lane, double &s, double &sc) {
+                                          uint64_t *critq, bool musig, double *mu, double *sig, int lane, double &s,
+                                          double &sc) {
     s = 0.0;
     sc = 0.0;
     for (int32_t j0 = 0; j0 < m.M; j0 += GS) {
@@ -473,7 +481,7 @@ __device__ __forceinline__ void prop_pass(const DevModel &m, const R *__restrict
             a[j] = aj;
             s += (double)aj;
             if (cr) sc += (double)aj;
-            if (MUSIG && !cr && aj != (R)0) {
+            if (musig && !cr && aj != (R)0) {
                 if (m.colq) {
                     const int4 q = m.colq[j];
                     musig_add(mu, sig, q.x, (double)aj);
@@ -590,10 +598,12 @@ __device__ __forceinline__ void grid_record(const ModelItem &mi, int64_t r, int 
 
 template <class R, int GS, bool SENS>
 __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, int32_t *xs, int32_t *dx,
-                              int32_t *snap, uint4 *head, uint64_t *critq, double *mu, double *sig, R *a) {
-    // one-wavefront groups scatter the species sums during the propensity pass
-    constexpr bool MUSIG = (GS == 64);
+                              int32_t *snap, uint4 *head, uint64_t *critq, bool musig, double *mu, double *sig,
+                              R *a) {
     const DevModel &m = mi.dm;
+    // this wavefront's species accumulators (NW copies of [N] each)
+    double *muw = mu + (size_t)G.wid * m.N;
+    double *sigw = sig + (size_t)G.wid * m.N;
     const int lane = G.lane;
     const R *c = Real<R>::rates(m);
     const uint2 key = rng_key(mi.seed, mi.rid_offset + r);
@@ -609,11 +619,12 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
     for (int32_t i = lane; i < m.N; i += GS) {
         xs[i] = m.x0[i];
         dx[i] = 0;
-        if (MUSIG) {
+    }
+    if (musig)
+        for (int32_t i = lane; i < Grp<GS>::NW * m.N; i += GS) {
             mu[i] = 0.0;
             sig[i] = 0.0;
         }
-    }
     if (SENS)
         for (int32_t j = lane; j < m.M; j += GS) {
             S[j] = 0.0;
@@ -642,7 +653,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
         if (grid)
             for (int32_t q = lane; q < mi.n_track; q += GS) snap[q] = xs[mi.track[q]];
         double a0, a0c;
-        prop_pass<R, GS, MUSIG>(m, c, xs, a, critq, mu, sig, lane, a0, a0c);
+        prop_pass<R, GS>(m, c, xs, a, critq, musig, muw, sigw, lane, a0, a0c);
         G.sum2(a0, a0c);
         G.sync();
         if (!(a0 > 0.0)) {  // absorbing state
@@ -654,12 +665,19 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
         // tau1: per species mean / variance bound, min and argmin
         R best = (R)kFltMax;
         int bi = -1, bw = 0;
-        if constexpr (MUSIG) {
-            // species sums were scattered by the propensity pass
+        if (musig) {
+            // species sums were scattered by the propensity pass; wavefront
+            // copies are added in a fixed order
             for (int32_t i = lane; i < m.N; i += GS) {
-                const R mui = (R)mu[i], sgi = (R)sig[i];
-                mu[i] = 0.0;
-                sig[i] = 0.0;
+                double smu = 0.0, ssg = 0.0;
+#pragma unroll 1
+                for (int w = 0; w < Grp<GS>::NW; ++w) {
+                    smu += mu[(size_t)w * m.N + i];
+                    ssg += sig[(size_t)w * m.N + i];
+                    mu[(size_t)w * m.N + i] = 0.0;
+                    sig[(size_t)w * m.N + i] = 0.0;
+                }
+                const R mui = (R)smu, sgi = (R)ssg;
                 if (i < m.M) {
                     int w;
                     const R ti = tau_species<R>(eps, xs[i], m.g_order[i], mui, sgi, w);
@@ -953,7 +971,7 @@ __global__ __launch_bounds__(BS, (GS == 64 ? 4 : 1)) void k_persistent(Persisten
             else hi = mid - 1;
         }
         const ModelItem mi = ld_uniform(&L.items[uni(lo)]);
-        simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, head, critq, mu, sig, a);
+        simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, head, critq, lay.musig, mu, sig, a);
     }
 }
 
