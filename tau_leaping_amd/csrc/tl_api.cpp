@@ -113,7 +113,7 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     }
     const double t0 = now_s();
     // one allocation: c64 | c32 | s0 | s1 | row_ptr | csr | col_ptr | csc | x0 | order | g_order | rs |
-    // dep_ptr | dep | colq | aff_ptr | aff
+    // dep_ptr | dep | colq | aff_ptr | aff | tile_ptr
     const size_t M = (size_t)m.M, N = (size_t)m.N, Z = (size_t)std::max(m.nnz, 1);
     std::vector<size_t> off;
     size_t total = 0;
@@ -139,6 +139,7 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     add(std::max<size_t>(m.colq.size(), 1) * 4);     // colq
     add((size_t)std::max<size_t>(m.aff_ptr.size(), 1) * 4);  // aff_ptr
     add(std::max<size_t>(m.aff.size(), 1) * 4);      // aff
+    add(m.tile_ptr.size() * 4);                       // tile_ptr
     std::vector<unsigned char> host(total, 0);
     std::vector<float> c32(M);
     for (size_t j = 0; j < M; ++j) c32[j] = (float)m.rates[j];
@@ -165,6 +166,7 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     if (!m.colq.empty()) memcpy(&host[off[14]], m.colq.data(), m.colq.size() * 4);
     if (!m.aff_ptr.empty()) memcpy(&host[off[15]], m.aff_ptr.data(), m.aff_ptr.size() * 4);
     if (!m.aff.empty()) memcpy(&host[off[16]], m.aff.data(), m.aff.size() * 4);
+    memcpy(&host[off[17]], m.tile_ptr.data(), m.tile_ptr.size() * 4);
 
     DeviceModelCache *c;
     if (it != m.dev.end()) {
@@ -199,12 +201,12 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     d.order = (const int8_t *)(b + off[9]);
     d.g_order = (const int8_t *)(b + off[10]);
     d.rs = (const int2 *)(b + off[11]);
-    d.wave_rows = (m.nnz > 32 * m.N) ? 1 : 0;  // mean row length above 32 entries
     d.dep_ptr = (const int32_t *)(b + off[12]);
     d.dep = (const int32_t *)(b + off[13]);
     d.colq = m.colq.empty() ? nullptr : (const int4 *)(b + off[14]);
     d.aff_ptr = m.aff_ptr.empty() ? nullptr : (const int32_t *)(b + off[15]);
     d.aff = m.aff_ptr.empty() ? nullptr : (const int32_t *)(b + off[16]);
+    d.tile_ptr = (const int32_t *)(b + off[17]);
     c->version = m.version;
     out = d;
     if (upload_seconds) *upload_seconds = now_s() - t0;
@@ -492,8 +494,6 @@ tl_status launch_batch(DeviceContext &dc, const PersistentConfig &cfg, bool is64
     L.slots = grid * ng;
     L.grid = (int)grid;
     L.lds_bytes = lds;
-    L.a_glob = nullptr;
-    if (!cfg.a_lds) TL_TRY(scratch(dc, 0, (size_t)L.slots * m_max * (is64 ? 8 : 4), L.a_glob));
     hipEvent_t e0, e1;
     TL_HIP(hipEventCreate(&e0));
     TL_HIP(hipEventCreate(&e1));

@@ -24,6 +24,7 @@ constexpr int kPoissonSwitch = 80;     // Knuth below, normal approximation at o
 constexpr int kKnuthCap = 300;         // Knuth returns 300 after 299 draws
 constexpr int kMaxRetries = 32;        // rejected leaps in a row before giving up
 constexpr float kFltMax = 3.402823466e+38f;
+constexpr int32_t kTileChannels = 8192;  // channels per propensity tile of the block-group tau pass
 
 constexpr uint32_t kKindPoisson = 0u;  // ctr[3] low bits: per-channel firing stream (draws 2, 3, ...)
 constexpr uint32_t kKindLeap = 1u;     //                  leap scalars (r1, r2)
@@ -51,12 +52,12 @@ struct DevModel {
     const int8_t *g_order;   // [N] order[i] for i < M else 0
     const int32_t *x0;       // [N]
     const int2 *rs;          // [M] (s0 | order << 29, s1): one load per channel
-    int32_t wave_rows;       // species rows are long: reduce each row with a whole wavefront
     const int32_t *dep_ptr;  // [N+1] species -> channels with it as a reactant
     const int32_t *dep;      // [dep_ptr[N]]
     const int4 *colq;        // [M] column entries padded with v = 0, or null (a column longer than 4)
     const int32_t *aff_ptr;  // [M+1] channel -> channels whose propensity it can change, or null
     const int32_t *aff;      // [aff_ptr[M]]
+    const int32_t *tile_ptr; // [N][n_tiles + 1] CSR offset of each species row at channel t * kTileChannels
 };
 
 // ---- Philox4x32-10 ----

@@ -116,7 +116,7 @@ struct ModelItem {
 struct PersistentConfig {
     int block;     // threads per block
     int group;     // lanes per realization (64 = one wave, == block = whole block)
-    bool a_lds;    // propensities in LDS (else per-group global scratch)
+    bool a_lds;    // propensities in LDS (else recomputed where they are read)
 };
 
 struct PersistentLaunch {
@@ -125,7 +125,6 @@ struct PersistentLaunch {
     int64_t n_items;
     int32_t n_max, m_max;
     unsigned long long *counter;
-    void *a_glob;   // [slots][m_max] when !a_lds
     int64_t slots;
     int grid;
     size_t lds_bytes;  // per block

@@ -515,6 +515,20 @@ tl_status build_model(tl_model &m, std::string &msg) {
             m.aff.swap(list);
         }
     }
+    // row offsets at propensity-tile boundaries (rows are sorted by channel)
+    {
+        constexpr int32_t kTile = 8192;
+        const int32_t nt = std::max(1, (m.M + kTile - 1) / kTile);
+        m.tile_ptr.assign((size_t)m.N * (nt + 1), 0);
+        for (int32_t i = 0; i < m.N; ++i) {
+            int32_t e = m.row_ptr[i];
+            for (int32_t t = 0; t <= nt; ++t) {
+                const int32_t bound = (t == nt) ? INT32_MAX : t * kTile;
+                while (e < m.row_ptr[i + 1] && (m.csr[e] >> 3) < bound) ++e;
+                m.tile_ptr[(size_t)i * (nt + 1) + t] = e;
+            }
+        }
+    }
     if (m.names.empty()) {
         m.names.resize(m.N);
         for (int32_t i = 0; i < m.N; ++i) m.names[i] = "S" + std::to_string(i + 1);

@@ -8,10 +8,13 @@
 // populations x[N], the proposed change dx[N] (integer LDS atomics, so the
 // update is exact and order-free), the criticality bits (one 64-bit ballot
 // per 64 channels), a staging buffer for the first random draw of 4*GS
-// channels and, when it fits, the propensities a[M]; otherwise a[M] lives in
-// a per-group global scratch row.  Reductions (a0, a0c, tau min / argmin,
-// cumulative searches) are wavefront shuffles and ballots plus a fixed-order
-// combine across waves, so results do not depend on timing.
+// channels, the species sums of the tau bound (one copy per wavefront) and,
+// when it fits, the propensities a[M]; otherwise every pass recomputes a_j
+// from x and the channel constants (shared by all realizations, so they stay
+// in cache), and no per-realization array goes to global memory.  Reductions
+// (a0, a0c, tau min / argmin, cumulative searches) are wavefront shuffles and
+// ballots plus a fixed-order combine across waves, so results do not depend
+// on timing.
 //
 // The per-realization algorithm follows PerformCalc / Step
 // (tau_leaping/model.cu:301-413, 640-737) with the defect fixes of the CPU
@@ -25,21 +28,80 @@ namespace {
 
 __host__ __device__ inline size_t al16(size_t v) { return (v + 15) & ~(size_t)15; }
 
-// LDS budget of the per-wavefront species accumulators of one group.
-constexpr size_t kMusigBudget = 65536;
+// Global-memory pointers.  Pointers read from memory (the work-item table)
+// are generic to the compiler, which then emits flat instructions; a flat
+// load also counts against the LDS wait counter, so every wait for an LDS
+// result would wait for it too.  Cast to the global address space, they give
+// global loads.
+template <class T>
+using gptr = T __attribute__((address_space(1))) *;
+template <class T>
+__device__ __forceinline__ gptr<T> as_global(T *p) {
+    return (gptr<T>)p;
+}
 
-// LDS layout of one group.
-struct GroupLayout {
-    size_t off_x, off_dx, off_snap, off_head, off_crit, off_red, off_mu, off_sig, off_a, bytes;
-    bool musig;  // species sums scattered by the propensity pass (else gathered by rows)
+// Native vector types (the HIP vector classes do not load through
+// address-space-qualified pointers).
+typedef int v2i __attribute__((ext_vector_type(2)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// The device model with global-address-space pointers.
+struct GModel {
+    int32_t N, M, nnz;
+    gptr<const double> c64;
+    gptr<const float> c32;
+    gptr<const int32_t> row_ptr, csr, col_ptr, csc, x0, dep_ptr, dep, aff_ptr, aff, tile_ptr;
+    gptr<const int8_t> g_order;
+    gptr<const v2i> rs;
+    gptr<const v4i> colq;  // null when a column has more than 4 entries
+    __device__ __forceinline__ int2 rs_at(int32_t j) const {
+        const v2i v = rs[j];
+        return make_int2(v.x, v.y);
+    }
+    __device__ __forceinline__ int4 colq_at(int32_t j) const {
+        const v4i v = colq[j];
+        return make_int4(v.x, v.y, v.z, v.w);
+    }
+    template <class R>
+    __device__ __forceinline__ gptr<const R> rates() const {
+        if constexpr (sizeof(R) == 8)
+            return c64;
+        else
+            return c32;
+    }
 };
-// Groups carry one pair of species accumulators mu[N], sig[N] per wavefront
-// when they fit the budget (always for one-wavefront groups).
+__device__ __forceinline__ GModel global_model(const DevModel &d) {
+    GModel g;
+    g.N = d.N;
+    g.M = d.M;
+    g.nnz = d.nnz;
+    g.c64 = as_global(d.c64);
+    g.c32 = as_global(d.c32);
+    g.row_ptr = as_global(d.row_ptr);
+    g.csr = as_global(d.csr);
+    g.col_ptr = as_global(d.col_ptr);
+    g.csc = as_global(d.csc);
+    g.x0 = as_global(d.x0);
+    g.dep_ptr = as_global(d.dep_ptr);
+    g.dep = as_global(d.dep);
+    g.aff_ptr = as_global(d.aff_ptr);
+    g.aff = as_global(d.aff);
+    g.tile_ptr = as_global(d.tile_ptr);
+    g.g_order = as_global(d.g_order);
+    g.rs = as_global(reinterpret_cast<const v2i *>(d.rs));
+    g.colq = as_global(reinterpret_cast<const v4i *>(d.colq));
+    return g;
+}
+
+// LDS layout of one group.  Species sums mu[N], sig[N] of the tau bound;
+// block groups without a[M] in LDS also hold one propensity tile.
+struct GroupLayout {
+    size_t off_x, off_dx, off_snap, off_head, off_crit, off_red, off_mu, off_sig, off_tile, off_a, bytes;
+};
 __host__ __device__ inline GroupLayout group_layout(int32_t n_max, int32_t m_max, int gs, bool a_lds,
                                                     size_t real_bytes) {
     GroupLayout L;
     const int nw = gs / 64;
-    L.musig = gs == 64 || (size_t)nw * (size_t)n_max * 16 <= kMusigBudget;
     L.off_x = 0;
     L.off_dx = al16((size_t)n_max * 4);
     L.off_snap = L.off_dx + al16((size_t)n_max * 4);
@@ -47,9 +109,10 @@ __host__ __device__ inline GroupLayout group_layout(int32_t n_max, int32_t m_max
     L.off_crit = L.off_head + (size_t)gs * 16;
     L.off_red = L.off_crit + al16((size_t)((m_max + 63) / 64) * 8);
     L.off_mu = L.off_red + al16((size_t)(4 * nw + 4) * 8);
-    const size_t musig = L.musig ? al16((size_t)nw * (size_t)n_max * 8) : 0;
-    L.off_sig = L.off_mu + musig;
-    L.off_a = L.off_sig + musig;
+    L.off_sig = L.off_mu + al16((size_t)n_max * 8);
+    L.off_tile = L.off_sig + al16((size_t)n_max * 8);
+    const bool tile = gs > 64 && !a_lds;
+    L.off_a = L.off_tile + (tile ? al16((size_t)(m_max < kTileChannels ? m_max : kTileChannels) * real_bytes) : 0);
     L.bytes = L.off_a + (a_lds ? al16((size_t)m_max * real_bytes) : 0);
     return L;
 }
@@ -341,14 +404,14 @@ struct Grp {
         double incl, excl;
         int32_t j0, j1;
     };
-    template <class R>
-    __device__ double scan_chunks(const R *a, const uint64_t *critq, bool masked, int32_t M, Scan &sc) const {
+    template <class P>
+    __device__ double scan_chunks(const P &pa, const uint64_t *critq, bool masked, int32_t M, Scan &sc) const {
         const int32_t C = (M + GS - 1) / GS;
         sc.j0 = min(M, lane * C);
         sc.j1 = min(M, sc.j0 + C);
         double s = 0.0;
         for (int32_t j = sc.j0; j < sc.j1; ++j)
-            if (!masked || crit_bit(critq, j)) s += (double)a[j];
+            if (!masked || crit_bit(critq, j)) s += (double)pa.get(j);
         const double incl_w = wave_incl_scan(s, wl);
         double base = 0.0, total;
         if constexpr (GS != 64) {
@@ -373,15 +436,15 @@ struct Grp {
     // First j with inclusive cumulative weight > target; -1 if no such j
     // (total <= target).  The first lane whose prefix passes the target walks
     // its own channels.
-    template <class R>
-    __device__ int32_t search(const R *a, const uint64_t *critq, bool masked, int32_t M, double target) const {
+    template <class P>
+    __device__ int32_t search(const P &pa, const uint64_t *critq, bool masked, int32_t M, double target) const {
         Scan sc;
-        scan_chunks<R>(a, critq, masked, M, sc);
-        return pick<R>(a, critq, masked, sc, target);
+        scan_chunks<P>(pa, critq, masked, M, sc);
+        return pick<P>(pa, critq, masked, sc, target);
     }
 
-    template <class R>
-    __device__ int32_t pick(const R *a, const uint64_t *critq, bool masked, const Scan &sc, double target) const {
+    template <class P>
+    __device__ int32_t pick(const P &pa, const uint64_t *critq, bool masked, const Scan &sc, double target) const {
         const int32_t j0 = sc.j0, j1 = sc.j1;
         const double incl = sc.incl, excl = sc.excl;
         const unsigned long long hit = __ballot(incl > target);
@@ -394,7 +457,7 @@ struct Grp {
                 int32_t last = -1;
                 for (int32_t j = j0; j < j1; ++j) {
                     if (masked && !crit_bit(critq, j)) continue;
-                    const double w = (double)a[j];
+                    const double w = (double)pa.get(j);
                     if (w > 0.0) last = j;
                     cum += w;
                     if (cum > target) {
@@ -417,7 +480,7 @@ struct Grp {
                 int32_t last = -1;
                 for (int32_t j = j0; j < j1; ++j) {
                     if (masked && !crit_bit(critq, j)) continue;
-                    const double w = (double)a[j];
+                    const double w = (double)pa.get(j);
                     if (w > 0.0) last = j;
                     cum += w;
                     if (cum > target) {
@@ -442,6 +505,34 @@ __device__ __forceinline__ void chan_unpack(int2 p, int &order, int32_t &s0, int
     s1 = p.y;
 }
 
+// Where the passes read propensities: an array (kept in LDS, written by the
+// propensity pass and the direct-event updates), or a recomputation from the
+// populations in LDS and the shared channel constants.  Both give the same
+// values: a_j is a function of (c_j, x) evaluated by one formula.
+template <class R>
+struct PropStored {
+    static constexpr bool kStored = true;
+    R *a;
+    __device__ __forceinline__ R get(int32_t j) const { return a[j]; }
+    __device__ __forceinline__ void set(int32_t j, R v) const { a[j] = v; }
+};
+template <class R>
+struct PropRecomputed {
+    static constexpr bool kStored = false;
+    gptr<const v2i> rs;
+    gptr<const R> c;
+    const int32_t *xs;
+    __device__ __forceinline__ R get(int32_t j) const {
+        int order;
+        int32_t s0, s1;
+        bool cr;
+        const v2i p = rs[j];
+        chan_unpack(make_int2(p.x, p.y), order, s0, s1);
+        return propensity<R>(order, c[j], xs[s0], xs[s1], cr);
+    }
+    __device__ __forceinline__ void set(int32_t, R) const {}
+};
+
 // Adds of one column entry p of a channel with propensity aj to the species
 // sums (v * aj, v^2 * aj; padding entries have v = 0).
 __device__ __forceinline__ void musig_add(double *mu, double *sig, int32_t p, double aj) {
@@ -459,53 +550,157 @@ __device__ __forceinline__ void column_add(int32_t *x, int32_t p, int32_t k) {
     if (v != 0) atomicAdd(&x[entry_index(p)], v * k);
 }
 
-// Propensities, criticality bits, and the lane partial sums of a0 / a0c.
-// With MUSIG, each non-critical channel also adds v*a_j and v*(v*a_j) of its
-// column entries to its wavefront's species accumulators mu[i], sig[i] (LDS
+// Propensities, criticality bits, and the lane partial sums of a0 / a0c
+// (one-wavefront groups).  Each non-critical channel also adds v*a_j and
+// v*(v*a_j) of its column entries to the species sums mu[i], sig[i] (LDS
 // atomics; within one wavefront the order of the adds is fixed, so the sums
 // repeat bit for bit).
-template <class R, int GS>
-__device__ __forceinline__ void prop_pass(const DevModel &m, const R *__restrict__ c, const int32_t *xs, R *a,
-                                          uint64_t *critq, bool musig, double *mu, double *sig, int lane, double &s,
+template <class R, class P>
+__device__ __forceinline__ bool prop_one(const GModel &m, int32_t j, int2 rsj, R cj, int4 q, const int32_t *xs,
+                                         const P &pa, double *mu, double *sig, double &s, double &sc) {
+    int order;
+    int32_t s0, s1;
+    bool cr;
+    chan_unpack(rsj, order, s0, s1);
+    const R aj = propensity<R>(order, cj, xs[s0], xs[s1], cr);
+    pa.set(j, aj);
+    s += (double)aj;
+    if (cr) sc += (double)aj;
+    if (!cr && aj != (R)0) {
+        if (m.colq) {
+            musig_add(mu, sig, q.x, (double)aj);
+            musig_add(mu, sig, q.y, (double)aj);
+            musig_add(mu, sig, q.z, (double)aj);
+            musig_add(mu, sig, q.w, (double)aj);
+        } else {
+            const int32_t e1 = m.col_ptr[j + 1];
+            for (int32_t e = m.col_ptr[j]; e < e1; ++e) musig_add(mu, sig, m.csc[e], (double)aj);
+        }
+    }
+    return cr;
+}
+
+// Two channels per lane per round, with the constants of both loaded up
+// front (more loads in flight per wavefront).
+template <class R, int GS, class P>
+__device__ __forceinline__ void prop_pass(const GModel &m, gptr<const R> c, const int32_t *xs, const P &pa,
+                                          uint64_t *critq, double *mu, double *sig, int lane, double &s,
                                           double &sc) {
     s = 0.0;
     sc = 0.0;
-    for (int32_t j0 = 0; j0 < m.M; j0 += GS) {
-        const int32_t j = j0 + lane;
-        bool cr = false;
-        if (j < m.M) {
-            int order;
-            int32_t s0, s1;
-            chan_unpack(m.rs[j], order, s0, s1);
-            const R aj = propensity<R>(order, c[j], xs[s0], xs[s1], cr);
-            a[j] = aj;
-            s += (double)aj;
-            if (cr) sc += (double)aj;
-            if (musig && !cr && aj != (R)0) {
-                if (m.colq) {
-                    const int4 q = m.colq[j];
-                    musig_add(mu, sig, q.x, (double)aj);
-                    musig_add(mu, sig, q.y, (double)aj);
-                    musig_add(mu, sig, q.z, (double)aj);
-                    musig_add(mu, sig, q.w, (double)aj);
-                } else {
-                    const int32_t e1 = m.col_ptr[j + 1];
-                    for (int32_t e = m.col_ptr[j]; e < e1; ++e) musig_add(mu, sig, m.csc[e], (double)aj);
+    const bool wantq = m.colq != nullptr;
+    for (int32_t j0 = 0; j0 < m.M; j0 += 2 * GS) {
+        const int32_t ja = j0 + lane, jb = ja + GS;
+        const bool va = ja < m.M, vb = jb < m.M;
+        int2 ra = make_int2(0, 0), rb = make_int2(0, 0);
+        R ca = (R)0, cb = (R)0;
+        int4 qa = make_int4(2, 2, 2, 2), qb = make_int4(2, 2, 2, 2);
+        if (va) {
+            ra = m.rs_at(ja);
+            ca = c[ja];
+            if (wantq) qa = m.colq_at(ja);
+        }
+        if (vb) {
+            rb = m.rs_at(jb);
+            cb = c[jb];
+            if (wantq) qb = m.colq_at(jb);
+        }
+        bool cra = false, crb = false;
+        if (va) cra = prop_one<R>(m, ja, ra, ca, qa, xs, pa, mu, sig, s, sc);
+        if (vb) crb = prop_one<R>(m, jb, rb, cb, qb, xs, pa, mu, sig, s, sc);
+        const unsigned long long bala = __ballot(cra), balb = __ballot(crb);
+        const int32_t ta = j0 + (lane & ~63), tb = ta + GS;
+        if ((lane & 63) == 0) {
+            if (ta < m.M) critq[ta >> 6] = bala;
+            if (tb < m.M) critq[tb >> 6] = balb;
+        }
+    }
+}
+
+// Block groups: propensities, criticality bits and lane partial sums of
+// a0 / a0c, then the species sums gathered along the CSR rows, one tile of
+// kTileChannels channels at a time (the tile's propensities sit in LDS: the
+// whole a[M] when it fits, else a tile buffer).  A species row belongs to L
+// consecutive lanes (L a power of two, about GS / N) that stride its entries
+// in the tile and combine by butterfly; the row's first lane adds the tile's
+// part to mu[i], sig[i].  No atomics, and a fixed order of additions.
+template <class R, int GS, class P>
+__device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c, const int32_t *xs, const P &pa,
+                                               R *tile, uint64_t *critq, double *mu, double *sig, const Grp<GS> &G,
+                                               double &s, double &sc) {
+    const int lane = G.lane;
+    s = 0.0;
+    sc = 0.0;
+    int L = 1;
+    while (L < 64 && 2 * L * m.N <= GS) L *= 2;
+    const int32_t nt = (m.M + kTileChannels - 1) / kTileChannels;
+    const int32_t rows_per_pass = GS / L;
+    const int k = lane & (L - 1);
+    for (int32_t t = 0; t < nt; ++t) {
+        const int32_t t0 = t * kTileChannels;
+        const int32_t t1 = min(m.M, t0 + kTileChannels);
+        R *A;  // A[j - aoff] holds a_j for j in the tile
+        int32_t aoff;
+        if constexpr (P::kStored) {
+            A = pa.a;
+            aoff = 0;
+        } else {
+            A = tile;
+            aoff = t0;
+        }
+        for (int32_t j0 = t0; j0 < t1; j0 += GS) {
+            const int32_t j = j0 + lane;
+            bool cr = false;
+            if (j < t1) {
+                int order;
+                int32_t s0, s1;
+                chan_unpack(m.rs_at(j), order, s0, s1);
+                const R aj = propensity<R>(order, c[j], xs[s0], xs[s1], cr);
+                A[j - aoff] = aj;
+                s += (double)aj;
+                if (cr) sc += (double)aj;
+            }
+            const unsigned long long bal = __ballot(cr);
+            const int32_t tile0 = j0 + (lane & ~63);
+            if ((lane & 63) == 0 && tile0 < t1) critq[tile0 >> 6] = bal;
+        }
+        G.sync();
+        for (int32_t i0 = 0; i0 < m.N; i0 += rows_per_pass) {
+            const int32_t i = i0 + lane / L;
+            double pm = 0.0, ps = 0.0;
+            if (i < m.N) {
+                const int32_t e0 = m.tile_ptr[(size_t)i * (nt + 1) + t], e1 = m.tile_ptr[(size_t)i * (nt + 1) + t + 1];
+                for (int32_t e = e0 + k; e < e1; e += L) {
+                    const int32_t p = m.csr[e];
+                    const int32_t j = entry_index(p);
+                    if (!crit_bit(critq, j)) {
+                        const double v = (double)entry_value(p);
+                        const double tmp = v * (double)A[j - aoff];
+                        pm += tmp;
+                        ps += v * tmp;
+                    }
                 }
             }
+            for (int o = L >> 1; o > 0; o >>= 1) {
+                pm += __shfl_xor(pm, o, 64);
+                ps += __shfl_xor(ps, o, 64);
+            }
+            if (i < m.N && k == 0) {
+                mu[i] = (t == 0) ? pm : mu[i] + pm;
+                sig[i] = (t == 0) ? ps : sig[i] + ps;
+            }
         }
-        const unsigned long long bal = __ballot(cr);
-        const int32_t tile0 = j0 + (lane & ~63);
-        if ((lane & 63) == 0 && tile0 < m.M) critq[tile0 >> 6] = bal;
+        G.sync();
     }
 }
 
 // Propensities of the channels whose reactants include a species of
 // channel jf's column (after jf fired once); the values equal a full
 // recomputation.
-template <class R, int GS>
-__device__ __forceinline__ void prop_update(const DevModel &m, const R *__restrict__ c, const int32_t *xs, R *a,
+template <class R, int GS, class P>
+__device__ __forceinline__ void prop_update(const GModel &m, gptr<const R> c, const int32_t *xs, const P &pa,
                                             int32_t jf, int lane) {
+    if constexpr (!P::kStored) return;
     if (m.aff_ptr) {
         const int32_t d1 = m.aff_ptr[jf + 1];
         for (int32_t d = m.aff_ptr[jf] + lane; d < d1; d += GS) {
@@ -513,8 +708,8 @@ __device__ __forceinline__ void prop_update(const DevModel &m, const R *__restri
             int order;
             int32_t s0, s1;
             bool cr;
-            chan_unpack(m.rs[k], order, s0, s1);
-            a[k] = propensity<R>(order, c[k], xs[s0], xs[s1], cr);
+            chan_unpack(m.rs_at(k), order, s0, s1);
+            pa.set(k, propensity<R>(order, c[k], xs[s0], xs[s1], cr));
         }
         return;
     }
@@ -527,8 +722,8 @@ __device__ __forceinline__ void prop_update(const DevModel &m, const R *__restri
             int order;
             int32_t s0, s1;
             bool cr;
-            chan_unpack(m.rs[k], order, s0, s1);
-            a[k] = propensity<R>(order, c[k], xs[s0], xs[s1], cr);
+            chan_unpack(m.rs_at(k), order, s0, s1);
+            pa.set(k, propensity<R>(order, c[k], xs[s0], xs[s1], cr));
         }
     }
 }
@@ -550,21 +745,28 @@ __device__ __forceinline__ double score_term(bool cr, R aj, R tr, double tau, in
 // first draws of channels [c0, c0 + 4*GS) are produced one Philox block per
 // lane (4 channels per block) into the LDS staging buffer, then each lane
 // samples its channels c0 + lane + GS*i.
-template <class R, int GS, class F>
-__device__ __forceinline__ void for_each_firing(const DevModel &m, const Grp<GS> &G, const R *a, const uint64_t *critq,
+template <class R, int GS, class P, class F>
+__device__ __forceinline__ void for_each_firing(const GModel &m, const Grp<GS> &G, const P &pa, const uint64_t *critq,
                                                 uint4 *head, uint2 key, uint32_t epoch, uint32_t c3, R tr,
                                                 int32_t jfire, F &&f) {
     const uint32_t *hflat = reinterpret_cast<const uint32_t *>(head);
     const uint32_t ck = c3 | kKindPoisson;
     for (int32_t c0 = 0; c0 < m.M; c0 += 4 * GS) {
         if (c0 + 4 * G.lane < m.M) head[G.lane] = head_block(key, (uint32_t)((c0 >> 2) + G.lane), epoch, c3);
+        // the four propensities first (their loads overlap)
+        R a4[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int32_t j = c0 + G.lane + GS * i;
+            a4[i] = (j < m.M) ? pa.get(j) : (R)0;
+        }
         G.sync();
 #pragma unroll 1
         for (int i = 0; i < 4; ++i) {
             const int32_t j = c0 + G.lane + GS * i;
             if (j < m.M) {
                 const bool cr = crit_bit(critq, j);
-                const R aj = a[j];
+                const R aj = i == 0 ? a4[0] : (i == 1 ? a4[1] : (i == 2 ? a4[2] : a4[3]));
                 const int32_t kj = cr ? (j == jfire ? 1 : 0)
                                       : poisson_draw<R>(aj * tr, hflat[G.lane + GS * i], key, (uint32_t)j, epoch, ck);
                 f(j, cr, aj, kj);
@@ -587,25 +789,22 @@ __device__ __forceinline__ void grid_record(const ModelItem &mi, int64_t r, int 
         const double g = h * (double)next_g;
         if (!(g < t_b)) break;
         const double w = (t_b > t_a) ? (g - t_a) / (t_b - t_a) : 0.0;
-        double *out = mi.grid + ((size_t)r * (n + 1) + next_g) * T;
+        gptr<double> out = as_global(mi.grid + ((size_t)r * (n + 1) + next_g) * T);
         for (int32_t q = lane; q < T; q += GS) {
-            const double xa = (double)snap[q], xb = (double)xs[mi.track[q]];
+            const double xa = (double)snap[q], xb = (double)xs[as_global(mi.track)[q]];
             out[q] = xa + (xb - xa) * w;
         }
         next_g++;
     }
 }
 
-template <class R, int GS, bool SENS>
+template <class R, int GS, bool SENS, class P>
 __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, int32_t *xs, int32_t *dx,
-                              int32_t *snap, uint4 *head, uint64_t *critq, bool musig, double *mu, double *sig,
-                              R *a) {
-    const DevModel &m = mi.dm;
-    // this wavefront's species accumulators (NW copies of [N] each)
-    double *muw = mu + (size_t)G.wid * m.N;
-    double *sigw = sig + (size_t)G.wid * m.N;
+                              int32_t *snap, uint4 *head, uint64_t *critq, double *mu, double *sig, R *tile,
+                              const P &pa) {
+    const GModel m = global_model(mi.dm);
     const int lane = G.lane;
-    const R *c = Real<R>::rates(m);
+    const gptr<const R> c = m.template rates<R>();
     const uint2 key = rng_key(mi.seed, mi.rid_offset + r);
     const uint32_t c3 = mi.model_id << 2;
     const double tf = mi.t_final;
@@ -613,15 +812,18 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
     // Likelihood-ratio score S_j = dlog p(path)/dc_j, and D_j = dt/dc_j of the
     // current time (nonzero through tau1-limited leaps; used by the clamped
     // last leap and the truncated last direct interval).
-    double *S = SENS ? mi.score + (size_t)r * m.M : nullptr;
-    double *D = SENS ? mi.dtime + (size_t)r * m.M : nullptr;
+    gptr<double> S = nullptr, D = nullptr;
+    if (SENS) {
+        S = as_global(mi.score + (size_t)r * m.M);
+        D = as_global(mi.dtime + (size_t)r * m.M);
+    }
 
     for (int32_t i = lane; i < m.N; i += GS) {
         xs[i] = m.x0[i];
         dx[i] = 0;
     }
-    if (musig)
-        for (int32_t i = lane; i < Grp<GS>::NW * m.N; i += GS) {
+    if (GS == 64)  // scattered sums start from zero (the tiled pass assigns them)
+        for (int32_t i = lane; i < m.N; i += GS) {
             mu[i] = 0.0;
             sig[i] = 0.0;
         }
@@ -651,9 +853,12 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
         }
         const double t_before = t;
         if (grid)
-            for (int32_t q = lane; q < mi.n_track; q += GS) snap[q] = xs[mi.track[q]];
+            for (int32_t q = lane; q < mi.n_track; q += GS) snap[q] = xs[as_global(mi.track)[q]];
         double a0, a0c;
-        prop_pass<R, GS>(m, c, xs, a, critq, musig, muw, sigw, lane, a0, a0c);
+        if constexpr (GS == 64)
+            prop_pass<R, GS>(m, c, xs, pa, critq, mu, sig, lane, a0, a0c);
+        else
+            prop_tau_tiled<R, GS>(m, c, xs, pa, tile, critq, mu, sig, G, a0, a0c);
         G.sum2(a0, a0c);
         G.sync();
         if (!(a0 > 0.0)) {  // absorbing state
@@ -665,76 +870,20 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
         // tau1: per species mean / variance bound, min and argmin
         R best = (R)kFltMax;
         int bi = -1, bw = 0;
-        if (musig) {
-            // species sums were scattered by the propensity pass; wavefront
-            // copies are added in a fixed order
-            for (int32_t i = lane; i < m.N; i += GS) {
-                double smu = 0.0, ssg = 0.0;
-#pragma unroll 1
-                for (int w = 0; w < Grp<GS>::NW; ++w) {
-                    smu += mu[(size_t)w * m.N + i];
-                    ssg += sig[(size_t)w * m.N + i];
-                    mu[(size_t)w * m.N + i] = 0.0;
-                    sig[(size_t)w * m.N + i] = 0.0;
-                }
-                const R mui = (R)smu, sgi = (R)ssg;
-                if (i < m.M) {
-                    int w;
-                    const R ti = tau_species<R>(eps, xs[i], m.g_order[i], mui, sgi, w);
-                    if (ti < best) {
-                        best = ti;
-                        bi = i;
-                        bw = w;
-                    }
-                }
+        // tau_i from the species sums (zeroed after use for the next scatter)
+        for (int32_t i = lane; i < m.N; i += GS) {
+            const R mui = (R)mu[i], sgi = (R)sig[i];
+            if (GS == 64) {
+                mu[i] = 0.0;
+                sig[i] = 0.0;
             }
-        } else if (m.wave_rows) {
-            // long rows: one wavefront per species row, lanes stride its entries
-            for (int32_t i = G.wid; i < m.N; i += Grp<GS>::NW) {
-                const int32_t e0 = m.row_ptr[i], e1 = m.row_ptr[i + 1];
-                double mu = 0, sig = 0;
-                for (int32_t e = e0 + G.wl; e < e1; e += 64) {
-                    const int32_t p = m.csr[e];
-                    const int32_t j = entry_index(p);
-                    const double v = (double)entry_value(p);
-                    const double tmp = v * (double)a[j] * (double)(!crit_bit(critq, j));
-                    sig += v * tmp;
-                    mu += tmp;
-                }
-                mu = wave_total<double>(mu);
-                sig = wave_total<double>(sig);
-                if (i < m.M) {
-                    int w;
-                    const R ti = tau_species<R>(eps, xs[i], m.g_order[i], (R)mu, (R)sig, w);
-                    if (ti < best) {
-                        best = ti;
-                        bi = i;
-                        bw = w;
-                    }
-                }
-            }
-        } else {
-            // short rows: one lane per species row, in the row's channel order
-            for (int32_t i = lane; i < m.N; i += GS) {
-                double mu = 0, sig = 0;
-                const int32_t e0 = m.row_ptr[i], e1 = m.row_ptr[i + 1];
-#pragma unroll 4
-                for (int32_t e = e0; e < e1; ++e) {
-                    const int32_t p = m.csr[e];
-                    const int32_t j = entry_index(p);
-                    const double v = (double)entry_value(p);
-                    const double tmp = v * (double)a[j] * (double)(!crit_bit(critq, j));
-                    sig += v * tmp;
-                    mu += tmp;
-                }
-                if (i < m.M) {
-                    int w;
-                    const R ti = tau_species<R>(eps, xs[i], m.g_order[i], (R)mu, (R)sig, w);
-                    if (ti < best) {
-                        best = ti;
-                        bi = i;
-                        bw = w;
-                    }
+            if (i < m.M) {
+                int w;
+                const R ti = tau_species<R>(eps, xs[i], m.g_order[i], mui, sgi, w);
+                if (ti < best) {
+                    best = ti;
+                    bi = i;
+                    bw = w;
                 }
             }
         }
@@ -750,7 +899,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                 // the channels that read a changed species are recomputed
                 for (int s = 0; s < kSsaSteps; ++s) {
                     typename Grp<GS>::Scan sc;
-                    const double a0s = G.template scan_chunks<R>(a, critq, false, m.M, sc);
+                    const double a0s = G.scan_chunks(pa, critq, false, m.M, sc);
                     if (!(a0s > 0.0)) {
                         t = tf;
                         break;
@@ -771,31 +920,32 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                         if (SENS) {  // survival over rem = tf - t: d/dc_j of -a0*rem
                             const double rem = tf - t;
                             for (int32_t j = lane; j < m.M; j += GS)
-                                if (c[j] != (R)0) S[j] += a0s * D[j] - (double)a[j] * rem / (double)c[j];
+                                if (c[j] != (R)0) S[j] += a0s * D[j] - (double)pa.get(j) * rem / (double)c[j];
                         }
                         t = tf;
                         break;
                     }
-                    int32_t jf = G.template pick<R>(a, critq, false, sc, a0s * r1);
+                    int32_t jf = G.pick(pa, critq, false, sc, a0s * r1);
                     if (jf < 0) {
                         int last = -1;
                         for (int32_t j = lane; j < m.M; j += GS)
-                            if (a[j] > (R)0) last = j;
+                            if (pa.get(j) > (R)0) last = j;
                         jf = G.max_int(last);
                     }
                     if (SENS) {
                         for (int32_t j = lane; j < m.M; j += GS)
                             if (c[j] != (R)0)
-                                S[j] += ((j == jf) ? 1.0 / (double)c[j] : 0.0) - (double)a[j] * dt / (double)c[j];
+                                S[j] += ((j == jf) ? 1.0 / (double)c[j] : 0.0) - (double)pa.get(j) * dt / (double)c[j];
                     }
+                    if constexpr (SENS && !P::kStored) G.sync();
                     if (m.colq) {
-                        if (lane < 4) column_add(xs, reinterpret_cast<const int32_t *>(m.colq)[(size_t)jf * 4 + lane], 1);
+                        if (lane < 4) column_add(xs, reinterpret_cast<gptr<const int32_t>>(m.colq)[(size_t)jf * 4 + lane], 1);
                     } else {
                         const int32_t e1 = m.col_ptr[jf + 1];
                         for (int32_t e = m.col_ptr[jf] + lane; e < e1; e += GS) column_add(xs, m.csc[e], 1);
                     }
                     G.sync();
-                    prop_update<R, GS>(m, c, xs, a, jf, lane);
+                    prop_update<R, GS>(m, c, xs, pa, jf, lane);
                     t += dt;
                     nssa++;
                     G.sync();
@@ -810,7 +960,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
             int32_t jc = m.M;
             double tau2 = (double)kFltMax;
             if (a0c > 0.0) {
-                const int32_t f = G.template search<R>(a, critq, true, m.M, a0c * r1);
+                const int32_t f = G.search(pa, critq, true, m.M, a0c * r1);
                 if (f >= 0) {
                     jc = f;
                     tau2 = -log(r2) / a0c;
@@ -827,11 +977,11 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
             const int32_t jfire = fire ? jc : -1;
             const R tr = (R)tau;
             double qpart = 0.0;
-            for_each_firing<R, GS>(m, G, a, critq, head, key, epoch, c3, tr, jfire,
+            for_each_firing<R, GS>(m, G, pa, critq, head, key, epoch, c3, tr, jfire,
                                    [&](int32_t j, bool cr, R aj, int32_t kj) {
                                        if (kj) {
                                            if (m.colq) {
-                                               const int4 q = m.colq[j];
+                                               const int4 q = m.colq_at(j);
                                                column_add(dx, q.x, kj);
                                                column_add(dx, q.y, kj);
                                                column_add(dx, q.z, kj);
@@ -853,7 +1003,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
             if (neg) {
                 for (int32_t i = lane; i < m.N; i += GS) dx[i] = 0;
                 if (SENS)  // take back this attempt's score terms (same draws)
-                    for_each_firing<R, GS>(m, G, a, critq, head, key, epoch, c3, tr, jfire,
+                    for_each_firing<R, GS>(m, G, pa, critq, head, key, epoch, c3, tr, jfire,
                                            [&](int32_t j, bool cr, R aj, int32_t kj) {
                                                if (c[j] != (R)0) S[j] -= score_term<R>(cr, aj, tr, tau, kj) / (double)c[j];
                                            });
@@ -868,10 +1018,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                 continue;
             }
             epoch++;
-            for (int32_t i = lane; i < m.N; i += GS) {
-                xs[i] += dx[i];
-                dx[i] = 0;
-            }
+            // (score terms first: they read this leap's propensities)
             if (SENS && clamped) {
                 // tau = tf - t: dtau/dc_j = -D_j
                 const double Q = G.sum(qpart);
@@ -885,20 +1032,25 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                     const int32_t p = m.csr[e], j = entry_index(p);
                     if (crit_bit(critq, j)) continue;
                     const double v = (double)entry_value(p);
-                    mu += v * (double)a[j];
-                    sig += v * v * (double)a[j];
+                    mu += v * (double)pa.get(j);
+                    sig += v * v * (double)pa.get(j);
                 }
                 G.sum2(mu, sig);
                 for (int32_t e = e0 + lane; e < e1; e += GS) {
                     const int32_t p = m.csr[e], j = entry_index(p);
                     if (crit_bit(critq, j) || c[j] == (R)0) continue;
                     const double v = (double)entry_value(p);
-                    const double h = (double)a[j] / (double)c[j];
+                    const double h = (double)pa.get(j) / (double)c[j];
                     const double dtau = (which == 0) ? -tau * (mu > 0.0 ? 1.0 : -1.0) * v * h / fabs(mu)
                                                      : -tau * v * v * h / sig;
-                    atomicAdd(&S[j], (Q / tau) * dtau);
-                    atomicAdd(&D[j], dtau);
+                    atomicAdd((double *)&S[j], (Q / tau) * dtau);
+                    atomicAdd((double *)&D[j], dtau);
                 }
+            }
+            if constexpr (SENS && !P::kStored) G.sync();
+            for (int32_t i = lane; i < m.N; i += GS) {
+                xs[i] += dx[i];
+                dx[i] = 0;
             }
             const double tn = clamped ? tf : t + tau;
             if (!(tn > t)) {  // time can not advance: stop instead of spinning
@@ -917,18 +1069,18 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
     if (grid && status == kOk) {  // grid point t_final (and any left) takes the final state
         const int32_t n = mi.n_intervals, T = mi.n_track;
         for (; next_g <= n; ++next_g) {
-            double *out = mi.grid + ((size_t)r * (n + 1) + next_g) * T;
-            for (int32_t q = lane; q < T; q += GS) out[q] = (double)xs[mi.track[q]];
+            gptr<double> out = as_global(mi.grid + ((size_t)r * (n + 1) + next_g) * T);
+            for (int32_t q = lane; q < T; q += GS) out[q] = (double)xs[as_global(mi.track)[q]];
         }
     }
 
     // outputs
-    for (int32_t q = lane; q < mi.n_track; q += GS) mi.out_x[r * mi.n_track + q] = xs[mi.track[q]];
+    for (int32_t q = lane; q < mi.n_track; q += GS) as_global(mi.out_x)[r * mi.n_track + q] = xs[as_global(mi.track)[q]];
     if (lane == 0) {
-        mi.out_steps[r] = steps;
-        mi.out_ssa[r] = nssa;
-        mi.out_rej[r] = nrej;
-        mi.out_status[r] = status;
+        as_global(mi.out_steps)[r] = steps;
+        as_global(mi.out_ssa)[r] = nssa;
+        as_global(mi.out_rej)[r] = nrej;
+        as_global(mi.out_status)[r] = status;
     }
     G.sync();
 }
@@ -936,7 +1088,6 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
 template <class R, int BS, int GS, bool ALDS, bool SENS>
 __global__ __launch_bounds__(BS, (GS == 64 ? 4 : 1)) void k_persistent(PersistentLaunch L) {
     extern __shared__ __align__(16) unsigned char lds[];
-    constexpr int NG = BS / GS;
     const GroupLayout lay = group_layout(L.n_max, L.m_max, GS, ALDS, sizeof(R));
     const int tid = threadIdx.x;
     const int g = tid / GS;
@@ -953,11 +1104,7 @@ __global__ __launch_bounds__(BS, (GS == 64 ? 4 : 1)) void k_persistent(Persisten
     G.red = reinterpret_cast<double *>(base + lay.off_red);
     double *mu = reinterpret_cast<double *>(base + lay.off_mu);
     double *sig = reinterpret_cast<double *>(base + lay.off_sig);
-    R *a;
-    if constexpr (ALDS)
-        a = reinterpret_cast<R *>(base + lay.off_a);
-    else
-        a = reinterpret_cast<R *>(L.a_glob) + (size_t)((int64_t)blockIdx.x * NG + g) * L.m_max;
+    R *tile = reinterpret_cast<R *>(base + lay.off_tile);
 
     for (;;) {
         unsigned long long v = 0;
@@ -971,7 +1118,14 @@ __global__ __launch_bounds__(BS, (GS == 64 ? 4 : 1)) void k_persistent(Persisten
             else hi = mid - 1;
         }
         const ModelItem mi = ld_uniform(&L.items[uni(lo)]);
-        simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, head, critq, lay.musig, mu, sig, a);
+        if constexpr (ALDS) {
+            const PropStored<R> pa{reinterpret_cast<R *>(base + lay.off_a)};
+            simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, head, critq, mu, sig, tile, pa);
+        } else {
+            const GModel gm = global_model(mi.dm);
+            const PropRecomputed<R> pa{gm.rs, gm.template rates<R>(), xs};
+            simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, head, critq, mu, sig, tile, pa);
+        }
     }
 }
 
