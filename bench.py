@@ -1,23 +1,29 @@
 """Benchmark of the tau-leaping engine on one MI355X.
 
-Workload (one "step" = one full simulation of the batch to t_final):
-random toy network, 128 species x 256 channels (1-2 reactants with
-coefficient 1, 1-2 products, rates log-uniform in [1e-3, 1], populations
-uniform in [0, 10000], generator seed 42), 100,000 realizations, t_final 0.1,
-eps 0.03, FP64 propensities, int32 populations, simulation seed 1234.
+Workload (one "step" = one full simulation of the batch to t_final, one
+persistent-kernel launch): random toy network, 128 species x 256 channels
+(1-2 reactants with coefficient 1, 1-2 products, rates log-uniform in
+[1e-3, 1], populations uniform in [0, 10000], generator seed 42), 100,000
+realizations, t_final 0.1, eps 0.03, FP64 propensities, int32 populations,
+simulation seed 1234.
 
 value = accepted realization-steps (leaps + direct-method blocks) per second
-over the timed steps, inputs resident on the device.  The roofline entry
-prices the persistent kernel at the algorithmic bytes of the staged
-dataflow, B = M*(3*s_a + 8 + 3/8) + 16*N + 8 bytes per realization-leap
-(s_a = 8 for FP64), over its HIP-event-timed duration.  The CPU baseline is
-the C oracle (same algorithm and random streams) on the host cores over a
-bounded sample of the same workload.
+of wall time over the timed steps, model constants resident on the device.
+The roofline entry prices the persistent kernel at the algorithmic bytes of
+the staged dataflow, B = M*(3*s_a + 8 + 3/8) + 16*N + 8 bytes per
+realization-leap (s_a = 8 for FP64), over its HIP-event-timed duration (the
+events are recorded on the library's stream around the launch).  `traffic`
+is the HBM bytes per launch measured with rocprofv3 counters (FETCH_SIZE,
+WRITE_SIZE), read from the committed profile summary when it matches the
+workload.  The CPU baseline is the C oracle (same algorithm and random
+streams) on the host cores over a bounded sample of the same workload.
 
-Extra fields (--extras, on by default): the plain per-model launch loop on
-the same workload (a bounded sample of realizations), the gradient/forward
-cost ratio on the cyclic chain with score-function sensitivities, and the
-heterogeneous 64-model batch persistent vs plain.
+Extra fields (on by default, --no-extras to skip): the FP32 variant; the
+plain per-model launch loop against the persistent launch on the same
+realizations; the cyclic chain with score-function sensitivities (grad/fwd
+cost ratio); the 31,250-channel colloid model (persistent FP64 / FP32 and a
+plain sample); the heterogeneous 64-model batch in one persistent launch
+against the plain per-model loop.
 """
 import argparse
 import json
@@ -30,11 +36,29 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "leaps/s (realizations×steps) batched vs plain at fixed CI; grad/fwd cost ratio"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_persistent_toy128x256_fp64_traffic.json")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def bytes_per_leap(N, M, fp64=True, sens=False):
     sa = 8 if fp64 else 4
     return M * (3 * sa + 8 + 3 / 8) + 16 * N + 8 + (16 * M if sens else 0)
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch from the committed counter summary, if it was
+    taken on this workload."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload:
+        return None
+    return float(t["fetch_kib"]) * 1024 * float(t["fetch_scale"]) + float(t["write_kib"]) * 1024 * float(t["write_scale"])
 
 
 def cpu_baseline(net, R_sample, t_final, seed):
@@ -51,6 +75,99 @@ def cpu_baseline(net, R_sample, t_final, seed):
                        f"{steps} realization-steps in {dt:.1f} s on {out['threads']} threads")
 
 
+def rate(res):
+    return res.total_steps / (res.kernel_ms * 1e-3) if res.kernel_ms > 0 else float("nan")
+
+
+def extras_run(T, models, np, m_toy, kw_toy):
+    ex = {}
+    tf = 0.1
+    # FP32 variant of the same workload
+    log("fp32 toy")
+    r32 = m_toy.simulate(tf, kw_toy["n_realizations"], **{**kw_toy["kw"], "precision": "fp32"})
+    ex["toy_fp32"] = {"leaps_per_s": rate(r32), "kernel_ms": r32.kernel_ms, "block_size": r32.block_size}
+
+    # plain per-model launch loop vs persistent on the same realizations
+    log("plain toy sample")
+    Rp = 500
+    rp = m_toy.simulate(tf, Rp, mode="plain", **kw_toy["kw"])
+    rq = m_toy.simulate(tf, Rp, **kw_toy["kw"])
+    assert rp.total_steps > 0 and rq.total_steps > 0
+    ex["toy_plain_vs_persistent"] = {
+        "realizations": Rp, "plain_leaps_per_s": rp.total_steps / rp.sim_seconds,
+        "persistent_leaps_per_s": rq.total_steps / rq.sim_seconds,
+        "ratio": (rq.total_steps / rq.sim_seconds) / (rp.total_steps / rp.sim_seconds),
+        "note": "wall time of each call; the persistent launch is far from filling the GPU at this size"}
+
+    # cyclic chain, 1024 species, 16384 realizations, t_final 1: grad / fwd cost
+    log("cyclic grad/fwd")
+    cyc = T.Model.from_network(models.cyclic_chain(1024, seed=43))
+    track = np.array([0, 256, 512, 768], np.int32)
+    ckw = dict(seed=1234, track=track)
+    cyc.simulate(1.0, 16384, **ckw)
+    cyc.simulate(1.0, 16384, sensitivities=True, **ckw)
+    f = [cyc.simulate(1.0, 16384, **ckw) for _ in range(3)]
+    g = [cyc.simulate(1.0, 16384, sensitivities=True, **ckw) for _ in range(3)]
+    fwd = min(r.sim_seconds for r in f)
+    grd = min(r.sim_seconds for r in g)
+    ex["cyclic1024_grad"] = {
+        "realizations": 16384, "fwd_seconds": fwd, "grad_seconds": grd, "grad_fwd_ratio": grd / fwd,
+        "fwd_kernel_ms": min(r.kernel_ms for r in f), "grad_kernel_ms": min(r.kernel_ms for r in g),
+        "fwd_leaps_per_s": f[0].total_steps / fwd,
+        "dmean_dc_max_abs": float(np.abs(g[0].dmean_dc).max()),
+        "dmean_dc_median_rel_se": float(np.median(g[0].dmean_dc_se / (np.abs(g[0].dmean_dc) + 1e-300)))}
+
+    # colloid, 250 species, 31250 channels, 4096 realizations, t_final 0.05
+    log("colloid")
+    col_net = models.colloidal(250, seed=44)
+    col = T.Model.from_network(col_net)
+    ckw = dict(seed=1234, track=np.arange(4, dtype=np.int32))
+    cx = {}
+    for prec in ("fp64", "fp32"):
+        col.simulate(0.05, 256, precision=prec, **ckw)  # tunes the launch shape
+        rc = col.simulate(0.05, 4096, precision=prec, **ckw)
+        B = bytes_per_leap(250, col_net.n_channels, prec == "fp64")
+        cx[prec] = {"leaps_per_s": rate(rc), "kernel_ms": rc.kernel_ms, "block_size": rc.block_size,
+                    "bytes_per_leap": B, "algorithmic_GBs": B * rate(rc) / 1e9,
+                    "hbm_frac": B * rate(rc) / 1e9 / HBM_PEAK_GBS}
+    log("colloid plain sample")
+    rcp = col.simulate(0.05, 256, mode="plain", **ckw)
+    cx["plain_fp64_256_realizations"] = {"leaps_per_s": rcp.total_steps / rcp.sim_seconds,
+                                          "kernel_ms": rcp.kernel_ms}
+    ex["colloid250"] = cx
+
+    # heterogeneous batch: 64 toy-recipe models, FP32, sensitivities on
+    log("hetero batch")
+    nets, reals = models.hetero_batch(64)
+    ms = [T.Model.from_network(n) for n in nets]
+    kws = [dict(t_final=tf, n_realizations=r, seed=1234 + i, model_id=i, precision="fp32", sensitivities=True,
+                track=np.arange(4, dtype=np.int32)) for i, r in enumerate(reals)]
+    T.batch_simulate(ms, [dict(k, n_realizations=max(64, k["n_realizations"] // 16)) for k in kws],
+                     want_final=False)
+    t0 = time.perf_counter()
+    hb = T.batch_simulate(ms, kws, want_final=False)
+    dtb = time.perf_counter() - t0
+    steps_b = sum(r.total_steps for r in hb)
+    # plain per-model loop on a sample: every 8th model, 1/64 of its
+    # realizations (the plain form runs without sensitivities)
+    pick = list(range(0, 64, 8))
+    sub = [dict(kws[i], n_realizations=max(4, kws[i]["n_realizations"] // 64), sensitivities=False) for i in pick]
+    log("hetero plain sample")
+    t0 = time.perf_counter()
+    hp = T.batch_simulate([ms[i] for i in pick], sub, mode="plain", want_final=False)
+    dtp = time.perf_counter() - t0
+    steps_p = sum(r.total_steps for r in hp)
+    ex["hetero64"] = {
+        "models": 64, "realizations": int(sum(reals)), "species_range": [min(n.n_species for n in nets),
+                                                                       max(n.n_species for n in nets)],
+        "persistent_leaps_per_s": steps_b / dtb, "persistent_seconds": dtb,
+        "persistent_kernel_ms": hb[0].kernel_ms, "persistent_sensitivities": True,
+        "plain_models": pick, "plain_realizations": int(sum(k["n_realizations"] for k in sub)),
+        "plain_leaps_per_s": steps_p / dtp,
+        "plain_sensitivities": False, "batched_over_plain": (steps_b / dtb) / (steps_p / dtp)}
+    return ex
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,7 +177,7 @@ def main():
     ap.add_argument("--precision", default="fp64")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=1500)
+    ap.add_argument("--cpu-sample", type=int, default=4000)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (byte-counter calibration for rocprofv3)")
     args = ap.parse_args()
@@ -88,9 +205,11 @@ def main():
     fp64 = args.precision == "fp64"
     kw = dict(seed=1234, precision=args.precision, track=np.arange(4, dtype=np.int32))
 
+    log(f"warmup x{args.warmup}")
     for _ in range(args.warmup):
         m.simulate(tf, R, **kw)
     torch.cuda.synchronize()
+    log(f"timed x{args.steps}")
     t0 = time.perf_counter()
     res = []
     for _ in range(args.steps):
@@ -103,6 +222,8 @@ def main():
     B = bytes_per_leap(net.n_species, net.n_channels, fp64)
     avg_ms = float(np.mean(kernel_ms))
     achieved = B * (steps_total / len(res)) / (avg_ms * 1e-3) / 1e9
+    workload = (f"toy network 128 species x 256 channels, {R} realizations, t_final 0.1, eps 0.03, "
+                f"{args.precision}, persistent engine")
     line = {
         "metric": METRIC,
         "value": value,
@@ -116,28 +237,17 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if fp64 else "f32",
         "data": "synthetic",
-        "config": {"workload": "toy network 128 species x 256 channels, 100000 realizations, t_final 0.1, "
-                               "eps 0.03, persistent engine",
-                   "realizations": R, "species": net.n_species, "channels": net.n_channels,
+        "config": {"workload": workload, "realizations": R, "species": net.n_species, "channels": net.n_channels,
                    "t_final": tf, "leaps_per_step": steps_total // len(res),
                    "ssa_events_per_step": res[-1].total_ssa_events, "block_size": res[-1].block_size},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(workload),
                      "kernel": "k_persistent", "kernel_ms": avg_ms, "bytes_per_leap": B},
     }
     if not args.no_extras:
-        extras = {}
-        # plain per-model launch loop, bounded sample of the same workload
-        Rp = 2000
-        t0 = time.perf_counter()
-        rp = m.simulate(tf, Rp, mode="plain", **kw)
-        dtp = time.perf_counter() - t0
-        rq = m.simulate(tf, Rp, **kw)
-        extras["plain_same_workload"] = {"realizations": Rp, "leaps_per_s": rp.total_steps / dtp,
-                                         "persistent_leaps_per_s": rq.total_steps / (rq.sim_seconds or 1e-9),
-                                         "kernel_ms_plain": rp.kernel_ms, "kernel_ms_persistent": rq.kernel_ms}
-        line["extras"] = extras
+        line["extras"] = extras_run(T, models, np, m, dict(n_realizations=R, kw=kw))
     if not args.no_cpu:
+        log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(net, args.cpu_sample, tf, 1234)
     print(json.dumps(line), flush=True)
 

@@ -116,6 +116,8 @@ typedef struct tl_result {
     double *dmean_dc;     /* [n_track][M] dE[X_s(t_final)]/dc_j (needs sensitivities) */
     double *dmean_dc_se;  /* [n_track][M] standard error of dmean_dc */
     int64_t *steps;       /* [R] accepted main-loop steps per realization */
+    double *score;        /* [R][M] likelihood-ratio score d log p(path) / dc_j per realization
+                             (needs sensitivities) */
     /* filled by the library */
     int64_t total_steps;      /* sum over realizations of accepted steps (leaps + direct blocks) */
     int64_t total_ssa_events; /* direct-method events */

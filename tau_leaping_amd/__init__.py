@@ -52,7 +52,7 @@ class _Result(C.Structure):
     _fields_ = [("mean", C.POINTER(C.c_double)), ("stddev", C.POINTER(C.c_double)),
                 ("grid_times", C.POINTER(C.c_double)), ("final_x", C.POINTER(C.c_int32)),
                 ("dmean_dc", C.POINTER(C.c_double)), ("dmean_dc_se", C.POINTER(C.c_double)),
-                ("steps", C.POINTER(C.c_int64)),
+                ("steps", C.POINTER(C.c_int64)), ("score", C.POINTER(C.c_double)),
                 ("total_steps", C.c_int64), ("total_ssa_events", C.c_int64), ("total_rejections", C.c_int64),
                 ("n_failed", C.c_int64), ("first_failure", C.c_int32), ("block_size", C.c_int32),
                 ("sim_seconds", C.c_double), ("upload_seconds", C.c_double), ("kernel_ms", C.c_double)]
@@ -142,6 +142,7 @@ class SimResult:
     dmean_dc: Optional[np.ndarray]  # [n_track][M]
     dmean_dc_se: Optional[np.ndarray]
     steps: np.ndarray          # [R]
+    score: Optional[np.ndarray]     # [R][M] per-realization likelihood-ratio score (return_scores=True)
     total_steps: int
     total_ssa_events: int
     total_rejections: int
@@ -283,7 +284,7 @@ class Model:
                               mode=kw.get("mode", "persistent"))[0]
 
 
-def _result_buffers(model: Model, p: _Params, trk, want_final=True):
+def _result_buffers(model: Model, p: _Params, trk, want_final=True, want_score=False):
     T = int(p.n_track) if p.n_track > 0 else model.n_species
     G = p.n_intervals + 1 if p.n_intervals > 0 else 1
     R = int(p.n_realizations)
@@ -291,7 +292,8 @@ def _result_buffers(model: Model, p: _Params, trk, want_final=True):
                 final_x=np.zeros((R, T), np.int32) if want_final else None,
                 dmean_dc=np.zeros((T, model.n_channels)) if p.sensitivities else None,
                 dmean_dc_se=np.zeros((T, model.n_channels)) if p.sensitivities else None,
-                steps=np.zeros(R, np.int64))
+                steps=np.zeros(R, np.int64),
+                score=np.zeros((R, model.n_channels)) if (want_score and p.sensitivities) else None)
     r = _Result()
 
     def P(a, ct):
@@ -304,6 +306,7 @@ def _result_buffers(model: Model, p: _Params, trk, want_final=True):
     r.dmean_dc = P(bufs["dmean_dc"], C.c_double)
     r.dmean_dc_se = P(bufs["dmean_dc_se"], C.c_double)
     r.steps = P(bufs["steps"], C.c_int64)
+    r.score = P(bufs["score"], C.c_double)
     track = trk if trk is not None else np.arange(model.n_species, dtype=np.int32)
     return r, bufs, track
 
@@ -311,7 +314,7 @@ def _result_buffers(model: Model, p: _Params, trk, want_final=True):
 def _to_result(r: _Result, bufs, track) -> SimResult:
     return SimResult(mean=bufs["mean"], stddev=bufs["stddev"], grid_times=bufs["grid_times"], track=track,
                      final_x=bufs["final_x"], dmean_dc=bufs["dmean_dc"], dmean_dc_se=bufs["dmean_dc_se"],
-                     steps=bufs["steps"], total_steps=r.total_steps, total_ssa_events=r.total_ssa_events,
+                     steps=bufs["steps"], score=bufs["score"], total_steps=r.total_steps, total_ssa_events=r.total_ssa_events,
                      total_rejections=r.total_rejections, n_failed=r.n_failed, first_failure=r.first_failure,
                      block_size=r.block_size, sim_seconds=r.sim_seconds, upload_seconds=r.upload_seconds,
                      kernel_ms=r.kernel_ms)
@@ -328,10 +331,11 @@ def batch_simulate(model_list: Sequence[Model], kwargs_list: Sequence[dict], mod
     for i, (m, kw) in enumerate(zip(model_list, kwargs_list)):
         kw = dict(kw)
         kw.pop("mode", None)
+        want_score = bool(kw.pop("return_scores", False))
         p, trk = m.params(mode=mode, **kw)
         params[i] = p
         keep.append(trk)
-        r, bufs, track = _result_buffers(m, p, trk, want_final)
+        r, bufs, track = _result_buffers(m, p, trk, want_final, want_score)
         results[i] = r
         outs.append((bufs, track))
     handles = (C.c_void_p * n)(*[m.handle.value for m in model_list])

@@ -728,6 +728,8 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
             TL_HIP(hipMemcpyAsync(e.res->dmean_dc, e.d_dmean, (size_t)T * e.m->M * 8, hipMemcpyDeviceToHost, s));
         if (sens && e.res->dmean_dc_se)
             TL_HIP(hipMemcpyAsync(e.res->dmean_dc_se, e.d_dse, (size_t)T * e.m->M * 8, hipMemcpyDeviceToHost, s));
+        if (sens && e.res->score)
+            TL_HIP(hipMemcpyAsync(e.res->score, e.d_score, (size_t)R * e.m->M * 8, hipMemcpyDeviceToHost, s));
         std::vector<double> hg;
         const int32_t n_grid = e.p->n_intervals > 0 ? e.p->n_intervals + 1 : 1;
         if (e.p->n_intervals > 0) {

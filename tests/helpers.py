@@ -48,3 +48,34 @@ SMALL_NETS = {
     "cyclic": lambda: models.cyclic_chain(64, seed=3),
     "colloid": lambda: models.colloidal(20, seed=5),
 }
+
+
+def cyclic_generator(c):
+    """Generator of one molecule's walk on the ring S_i -> S_(i+1) (rates c)."""
+    N = len(c)
+    L = np.zeros((N, N))
+    for i in range(N):
+        L[i, i] -= c[i]
+        L[i, (i + 1) % N] += c[i]
+    return L
+
+
+def cyclic_net(N, c, x0):
+    rx = [models.Reaction({i: 1}, {(i + 1) % N: 1}, float(c[i])) for i in range(N)]
+    return models.Network(N, rx, np.asarray(x0, np.int32))
+
+
+def cyclic_mean_jacobian(c, x0, T):
+    """d E[X_k(T)] / d c_j = x0 . d exp(T L) / d c_j (Frechet derivative), [k][j]."""
+    from scipy import linalg
+
+    N = len(c)
+    Lg = cyclic_generator(c)
+    J = np.zeros((N, N))
+    for j in range(N):
+        E = np.zeros((N, N))
+        E[j, j] = -1.0
+        E[j, (j + 1) % N] = 1.0
+        _, dP = linalg.expm_frechet(T * Lg, T * E)
+        J[:, j] = np.asarray(x0, float) @ dP
+    return J

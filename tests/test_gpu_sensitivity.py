@@ -1,0 +1,79 @@
+"""Likelihood-ratio sensitivities of the HIP engine.
+
+  * per realization, the score vector S_j = d log p(path) / dc_j equals the
+    CPU oracle's (FP64) wherever the two followed the same path;
+  * dE[X_k(T)]/dc_j on the linear cyclic chain agrees with the Frechet
+    derivative of exp(T L) (the chain's mean solves dm/dt = L^T m) within
+    the estimator's standard error, and with central finite differences
+    under common random numbers;
+  * the torch.autograd.Function returns g @ dE/dc.
+"""
+import numpy as np
+import pytest
+
+import tau_leaping_amd as T
+from helpers import SMALL_NETS, cyclic_mean_jacobian, cyclic_net, pair
+from tau_leaping_amd import models
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name,tf", [("dimer", 2.0), ("toy", 0.05), ("cyclic", 0.5)])
+@pytest.mark.parametrize("bs", [64, 1024])
+def test_scores_match_oracle(cuda, name, tf, bs):
+    net = SMALL_NETS[name]()
+    m, om = pair(net)
+    R = 512
+    res = m.simulate(tf, R, seed=99, sensitivities=True, return_scores=True, block_size=bs)
+    ora = om.simulate(R, tf, seed=99, sensitivities=True)
+    assert res.n_failed == 0 and (ora["status"] == 0).all()
+    same = np.all(res.final_x == ora["x_final"], axis=1) & (res.steps == ora["steps"])
+    assert same.mean() >= 0.97, same.mean()
+    S, So = res.score[same], ora["score"][same]
+    scale = np.abs(So).max(axis=1, keepdims=True) + 1e-300
+    err = np.abs(S - So) / scale
+    # same path: the scores differ only by the rounding of FP64 sums taken
+    # in another order
+    assert np.quantile(err.max(axis=1), 0.99) < 1e-8, np.quantile(err.max(axis=1), 0.99)
+
+
+def test_dmean_matches_closed_form_and_fd(cuda):
+    N, tf = 8, 0.5
+    rng = np.random.default_rng(4)
+    c = rng.uniform(0.5, 1.5, N)
+    x0 = rng.integers(400, 1200, N)
+    net = cyclic_net(N, c, x0)
+    m = T.Model.from_network(net)
+    R = 65536
+    res = m.simulate(tf, R, seed=7, sensitivities=True)
+    lr, se = res.dmean_dc, res.dmean_dc_se
+    exact = cyclic_mean_jacobian(c, x0, tf)
+    tol = 4.5 * se + 0.03 * np.abs(exact).max()
+    assert np.all(np.abs(lr - exact) < tol), np.abs(lr - exact).max()
+    # central differences, relative step 1e-3, common random numbers
+    for j in (0, 5):
+        h = 1e-3 * c[j]
+        outs = []
+        for sgn in (+1, -1):
+            cc = c.copy()
+            cc[j] += sgn * h
+            mm = T.Model.from_network(cyclic_net(N, cc, x0))
+            outs.append(mm.simulate(tf, R, seed=7).final_x.astype(float))
+        fd = (outs[0] - outs[1]) / (2 * h)
+        fd_mean, fd_se = fd.mean(0), fd.std(0, ddof=1) / np.sqrt(R)
+        assert np.all(np.abs(lr[:, j] - fd_mean) < 4.5 * np.hypot(se[:, j], fd_se) + 0.05 * np.abs(fd_mean).max())
+
+
+def test_autograd_function(cuda):
+    import torch
+
+    net = models.cyclic_chain(16, seed=2)
+    m = T.Model.from_network(net)
+    rates = torch.tensor(m.rates, dtype=torch.float64, requires_grad=True)
+    kw = dict(t_final=0.2, n_realizations=4096, seed=5, track=[0, 3, 7])
+    y = T.mean_final_state(rates, m, **kw)
+    w = torch.tensor([1.0, -2.0, 0.5], dtype=torch.float64)
+    (y * w).sum().backward()
+    ref = m.simulate(sensitivities=True, **kw)
+    assert np.allclose(y.detach().numpy(), ref.mean[-1])
+    assert np.allclose(rates.grad.numpy(), w.numpy() @ ref.dmean_dc)

@@ -21,6 +21,8 @@ import pytest
 from scipy import linalg, stats
 
 import tl_oracle as O
+from helpers import cyclic_generator as _cyclic_generator
+from helpers import cyclic_net as _cyclic_net
 from helpers import oracle_of
 from tau_leaping_amd import models
 from tau_leaping_amd.models import Network, Reaction
@@ -272,20 +274,6 @@ def test_normal_branch_law(mean):
 
 
 # ---------------------------------------------------- cyclic chain closed form --
-def _cyclic_generator(c):
-    N = len(c)
-    L = np.zeros((N, N))
-    for i in range(N):
-        L[i, i] -= c[i]
-        L[i, (i + 1) % N] += c[i]
-    return L
-
-
-def _cyclic_net(N, c, x0):
-    rx = [Reaction({i: 1}, {(i + 1) % N: 1}, float(c[i])) for i in range(N)]
-    return Network(N, rx, np.asarray(x0, np.int32))
-
-
 def test_direct_method_cyclic_closed_form():
     """Exact SSA (the fixed DirectMethodStep) on a linear cyclic chain: each
     molecule walks the ring independently, so E[X_k(T)] = sum_i x0_i P_ik and
