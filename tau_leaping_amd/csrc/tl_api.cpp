@@ -473,10 +473,9 @@ PersistentConfig heuristic(const std::vector<PersistentConfig> &cands, int32_t m
     return c ? *c : PersistentConfig{0, 0, false};
 }
 
-tl_status launch_batch(DeviceContext &dc, const PersistentConfig &cfg, bool is64, bool sens, ModelItem *d_items,
-                       int32_t n_models, int64_t n_items, int32_t n_max, int32_t m_max, unsigned long long *d_counter,
-                       int64_t grid_cap, bool use_graph, float *ms_out) {
-    hipStream_t s = dc.stream;
+PersistentLaunch make_launch(DeviceContext &dc, const PersistentConfig &cfg, bool is64, bool sens,
+                             ModelItem *d_items, int32_t n_models, int64_t n_items, int32_t n_max, int32_t m_max,
+                             unsigned long long *d_counter, int64_t grid_cap) {
     const size_t lds = persistent_lds_bytes(cfg, is64, n_max, m_max);
     const int occ = std::max(1, persistent_occupancy(cfg, is64, sens, lds));
     const int ng = cfg.block / cfg.group;
@@ -494,6 +493,20 @@ tl_status launch_batch(DeviceContext &dc, const PersistentConfig &cfg, bool is64
     L.slots = grid * ng;
     L.grid = (int)grid;
     L.lds_bytes = lds;
+    return L;
+}
+
+// One launch per size class; with use_graph the counter resets and launches
+// of every class are captured into one hipGraph and replayed once.  The
+// classes run back to back on the library's stream (each fills the GPU with
+// its own work items).
+struct ClassLaunch {
+    PersistentConfig cfg;
+    PersistentLaunch L;
+};
+tl_status run_launches(DeviceContext &dc, bool is64, bool sens, const std::vector<ClassLaunch> &cl, bool use_graph,
+                       float *ms_out) {
+    hipStream_t s = dc.stream;
     hipEvent_t e0, e1;
     TL_HIP(hipEventCreate(&e0));
     TL_HIP(hipEventCreate(&e1));
@@ -501,8 +514,12 @@ tl_status launch_batch(DeviceContext &dc, const PersistentConfig &cfg, bool is64
         hipGraph_t gph;
         hipGraphExec_t gx;
         TL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        launch_fill_u64(d_counter, 0ull, 1, s);
-        const hipError_t le = launch_persistent(cfg, is64, sens, L, s);
+        hipError_t le = hipSuccess;
+        for (const auto &c : cl) {
+            launch_fill_u64(c.L.counter, 0ull, 1, s);
+            const hipError_t e = launch_persistent(c.cfg, is64, sens, c.L, s);
+            if (le == hipSuccess) le = e;
+        }
         TL_HIP(hipStreamEndCapture(s, &gph));
         TL_HIP(le);
         TL_HIP(hipGraphInstantiate(&gx, gph, nullptr, nullptr, 0));
@@ -513,9 +530,11 @@ tl_status launch_batch(DeviceContext &dc, const PersistentConfig &cfg, bool is64
         hipGraphExecDestroy(gx);
         hipGraphDestroy(gph);
     } else {
-        launch_fill_u64(d_counter, 0ull, 1, s);
         TL_HIP(hipEventRecord(e0, s));
-        TL_HIP(launch_persistent(cfg, is64, sens, L, s));
+        for (const auto &c : cl) {
+            launch_fill_u64(c.L.counter, 0ull, 1, s);
+            TL_HIP(launch_persistent(c.cfg, is64, sens, c.L, s));
+        }
         TL_HIP(hipEventRecord(e1, s));
         TL_HIP(hipStreamSynchronize(s));
     }
@@ -526,6 +545,15 @@ tl_status launch_batch(DeviceContext &dc, const PersistentConfig &cfg, bool is64
     hipEventDestroy(e1);
     if (ms_out) *ms_out = ms;
     return TL_OK;
+}
+
+tl_status launch_batch(DeviceContext &dc, const PersistentConfig &cfg, bool is64, bool sens, ModelItem *d_items,
+                       int32_t n_models, int64_t n_items, int32_t n_max, int32_t m_max, unsigned long long *d_counter,
+                       int64_t grid_cap, bool use_graph, float *ms_out) {
+    std::vector<ClassLaunch> cl(1);
+    cl[0].cfg = cfg;
+    cl[0].L = make_launch(dc, cfg, is64, sens, d_items, n_models, n_items, n_max, m_max, d_counter, grid_cap);
+    return run_launches(dc, is64, sens, cl, use_graph, ms_out);
 }
 
 // Time each launch shape on a bounded sample (first few steps of a subset
@@ -605,7 +633,6 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
     DevBufs B;
     std::vector<Entry> E(n_models);
     double up_total = 0.0;
-    int32_t n_max = 0, m_max = 0;
     for (int32_t i = 0; i < n_models; ++i) {
         Entry &e = E[i];
         e.m = models[i];
@@ -615,8 +642,6 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
         TL_TRY(device_model(*e.m, device, e.dm, &up));
         up_total += up;
         e.track = track_list(e.m, e.p);
-        n_max = std::max(n_max, e.m->N);
-        m_max = std::max(m_max, e.m->M);
         e.cost = (double)e.p->n_realizations * (double)(e.m->M + e.m->nnz);
         const int64_t R = e.p->n_realizations;
         const int32_t T = (int32_t)e.track.size();
@@ -641,73 +666,103 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
         }
         TL_HIP(hipMemcpyAsync(e.d_track, e.track.data(), T * 4, hipMemcpyHostToDevice, s));
     }
-    // work order: most expensive models first (better tail balance)
-    std::vector<int32_t> order(n_models);
-    for (int32_t i = 0; i < n_models; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return E[a].cost > E[b].cost; });
-    std::vector<ModelItem> items(n_models);
-    int64_t base = 0;
-    for (int32_t k = 0; k < n_models; ++k) {
-        Entry &e = E[order[k]];
-        ModelItem &mi = items[k];
-        mi.dm = e.dm;
-        mi.t_final = e.p->t_final;
-        mi.eps = e.p->eps;
-        mi.seed = e.p->seed;
-        mi.max_iter = e.p->max_iter;
-        mi.item_base = base;
-        mi.R = e.p->n_realizations;
-        mi.rid_offset = e.p->realization_offset;
-        mi.model_id = e.p->model_id;
-        mi.n_track = (int32_t)e.track.size();
-        mi.track = e.d_track;
-        mi.out_x = e.d_outx;
-        mi.out_steps = e.d_steps;
-        mi.out_ssa = e.d_ssa;
-        mi.out_rej = e.d_rej;
-        mi.out_status = e.d_status;
-        mi.score = e.d_score;
-        mi.dtime = e.d_dtime;
-        mi.n_intervals = e.p->n_intervals;
-        mi.grid = e.d_grid;
-        base += mi.R;
-    }
-    const int64_t n_items = base;
-    ModelItem *d_items;
-    unsigned long long *d_counter;
-    TL_TRY(B.alloc(d_items, n_models));
-    TL_TRY(B.alloc(d_counter, 1));
-    TL_HIP(hipMemcpyAsync(d_items, items.data(), n_models * sizeof(ModelItem), hipMemcpyHostToDevice, s));
-    TL_HIP(hipStreamSynchronize(s));
-
-    PersistentConfig cfg;
+    // Size classes: models whose one-wavefront LDS footprint is within a
+    // factor of two of each other share a launch, so a small model's groups
+    // are not laid out (and their occupancy set) for the largest model of the
+    // batch.  An explicit block size puts every model in one class.
     const int bs_req = params[0].block_size;
-    if (bs_req > 0) {
-        auto cands = candidates(*dc, is64, sens, n_max, m_max);
-        const int gs = bs_req <= 64 ? 64 : bs_req;
-        const int bs = bs_req <= 64 ? 256 : bs_req;
-        bool found = false;
-        for (auto &c : cands)
-            if (c.block == bs && c.group == gs) {
-                cfg = c;
-                found = true;
-                break;
+    std::map<int, std::vector<int32_t>> classes;
+    for (int32_t i = 0; i < n_models; ++i) {
+        int cls = 0;
+        if (bs_req <= 0) {
+            const size_t b = persistent_lds_bytes(PersistentConfig{64, 64, true}, is64, E[i].m->N, E[i].m->M);
+            size_t lim = 4096;
+            while (b > lim && cls < 12) {
+                lim *= 2;
+                ++cls;
             }
-        if (!found)
+        }
+        classes[cls].push_back(i);
+    }
+    std::vector<ClassLaunch> launches;
+    std::vector<PersistentConfig> entry_cfg(n_models);
+    for (auto &kv : classes) {
+        std::vector<int32_t> &members = kv.second;
+        // work order: most expensive models first (better tail balance)
+        std::stable_sort(members.begin(), members.end(), [&](int32_t a, int32_t b) { return E[a].cost > E[b].cost; });
+        const int32_t nm = (int32_t)members.size();
+        std::vector<ModelItem> items(nm);
+        int64_t base = 0;
+        int32_t n_max = 0, m_max = 0;
+        for (int32_t k = 0; k < nm; ++k) {
+            Entry &e = E[members[k]];
+            n_max = std::max(n_max, e.m->N);
+            m_max = std::max(m_max, e.m->M);
+            ModelItem &mi = items[k];
+            mi.dm = e.dm;
+            mi.t_final = e.p->t_final;
+            mi.eps = e.p->eps;
+            mi.seed = e.p->seed;
+            mi.max_iter = e.p->max_iter;
+            mi.item_base = base;
+            mi.R = e.p->n_realizations;
+            mi.rid_offset = e.p->realization_offset;
+            mi.model_id = e.p->model_id;
+            mi.n_track = (int32_t)e.track.size();
+            mi.track = e.d_track;
+            mi.out_x = e.d_outx;
+            mi.out_steps = e.d_steps;
+            mi.out_ssa = e.d_ssa;
+            mi.out_rej = e.d_rej;
+            mi.out_status = e.d_status;
+            mi.score = e.d_score;
+            mi.dtime = e.d_dtime;
+            mi.n_intervals = e.p->n_intervals;
+            mi.grid = e.d_grid;
+            base += mi.R;
+        }
+        const int64_t n_items = base;
+        ModelItem *d_items;
+        unsigned long long *d_counter;
+        TL_TRY(B.alloc(d_items, nm));
+        TL_TRY(B.alloc(d_counter, 1));
+        TL_HIP(hipMemcpyAsync(d_items, items.data(), nm * sizeof(ModelItem), hipMemcpyHostToDevice, s));
+        TL_HIP(hipStreamSynchronize(s));
+
+        PersistentConfig cfg;
+        if (bs_req > 0) {
+            auto cands = candidates(*dc, is64, sens, n_max, m_max);
+            const int gs = bs_req <= 64 ? 64 : bs_req;
+            const int bs = bs_req <= 64 ? 256 : bs_req;
+            bool found = false;
             for (auto &c : cands)
-                if (c.group == gs) {
+                if (c.block == bs && c.group == gs) {
                     cfg = c;
                     found = true;
                     break;
                 }
-        if (!found) return set_error(TL_ERR_INVALID_ARG, "block_size " + std::to_string(bs_req) + " does not fit this model");
-    } else {
-        TL_TRY(autotune(*dc, is64, sens, items, n_items, n_max, m_max, d_counter, nullptr, cfg));
+            if (!found)
+                for (auto &c : cands)
+                    if (c.group == gs) {
+                        cfg = c;
+                        found = true;
+                        break;
+                    }
+            if (!found)
+                return set_error(TL_ERR_INVALID_ARG, "block_size " + std::to_string(bs_req) + " does not fit this model");
+        } else {
+            TL_TRY(autotune(*dc, is64, sens, items, n_items, n_max, m_max, d_counter, nullptr, cfg));
+        }
+        ClassLaunch c;
+        c.cfg = cfg;
+        c.L = make_launch(*dc, cfg, is64, sens, d_items, nm, n_items, n_max, m_max, d_counter, 0);
+        launches.push_back(c);
+        for (int32_t k = 0; k < nm; ++k) entry_cfg[members[k]] = cfg;
     }
 
     const double t0 = now_s();
     float ms = 0.f;
-    TL_TRY(launch_batch(*dc, cfg, is64, sens, d_items, n_models, n_items, n_max, m_max, d_counter, 0, true, &ms));
+    TL_TRY(run_launches(*dc, is64, sens, launches, true, &ms));
     for (auto &e : E)
         if (sens && e.res && (e.res->dmean_dc || e.res->dmean_dc_se))
             launch_grad_reduce(e.p->n_realizations, (int32_t)e.track.size(), e.m->M, e.d_outx, e.d_score, e.d_mean,
@@ -756,7 +811,8 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
         fill_summary(res, hsteps, hssa, hrej, hst);
         res->kernel_ms = ms;
         res->upload_seconds = up_total;
-        res->block_size = cfg.group == 64 ? 64 : cfg.block;
+        const PersistentConfig &ecfg = entry_cfg[&e - &E[0]];
+        res->block_size = ecfg.group == 64 ? 64 : ecfg.block;
         res->sim_seconds = now_s() - t0;
     }
     return TL_OK;
