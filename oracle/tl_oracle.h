@@ -41,6 +41,13 @@
  *     (ref: order -1 gives an uninitialised propensity, functions.h:118-120).
  *   - tau_i for species rows i >= M is +inf (ref reads order[] out of bounds,
  *     CSRMatrixPacked.cu:353).
+ *   - the species sums mu_i, sigma^2_i of the tau bound accumulate in FP64 and
+ *     are rounded once to the propensity type (ref: FP32 sums); v * a_j is
+ *     exact in FP64, so the rounded sums do not depend on the order of the
+ *     additions to within the last FP64 bit.
+ *   - the first Poisson draw of channel j at a step is output (j & 3) of the
+ *     Philox block shared by channels 4*(j>>2)..4*(j>>2)+3; later draws come
+ *     from the channel's own counter stream (see tlo_head_draw).
  */
 #ifndef TL_ORACLE_H
 #define TL_ORACLE_H
