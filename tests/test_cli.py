@@ -1,0 +1,80 @@
+"""The command-line tools: toy_system_gen writes StochKit models the reader
+accepts, with the structure of each family (random, -w cyclic chain, -s
+colloid, --toy benchmark recipe); tau_leaping parses the reference's flags and
+fails loudly without a GPU."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import tau_leaping_amd as T
+from tau_leaping_amd import models
+
+BIN = os.path.join(os.path.dirname(T.__file__), "bin")
+GEN = os.path.join(BIN, "toy_system_gen")
+SIM = os.path.join(BIN, "tau_leaping")
+
+pytestmark = pytest.mark.skipif(not (os.path.exists(GEN) and os.path.exists(SIM)), reason="CLI tools not built")
+
+
+def gen(tmp_path, name, *args):
+    out = tmp_path / name
+    with open(out, "w") as f:
+        r = subprocess.run([GEN, *args], stdout=f, stderr=subprocess.PIPE, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    return str(out)
+
+
+def test_colloid_family_structure(tmp_path):
+    m = T.Model.from_stochkit(gen(tmp_path, "s.xml", "-r", "20", "-s", "--seed", "3"))
+    ref = T.Model.from_network(models.colloidal(20, seed=5))
+    assert (m.n_species, m.n_channels, m.nnz) == (20, 200, 580)
+    # same reactions as the generator family in models.py (rates and
+    # populations come from different random streams)
+    assert [tuple(a) for a in m.structure()[:3]] == [tuple(a) for a in ref.structure()[:3]]
+
+
+def test_cyclic_family_structure(tmp_path):
+    m = T.Model.from_stochkit(gen(tmp_path, "w.xml", "-r", "8", "-w", "--seed", "3"))
+    order, s0, s1, rp, csr = m.structure()
+    assert (m.n_species, m.n_channels) == (8, 8)
+    assert (order == 1).all() and (s0 == np.arange(8)).all()
+
+
+def test_toy_recipe(tmp_path):
+    path = gen(tmp_path, "t.xml", "-r", "16", "-c", "40", "--toy", "--seed", "5", "--log-rates", "0.001", "1")
+    m = T.Model.from_stochkit(path)
+    assert (m.n_species, m.n_channels) == (16, 40)
+    order, s0, s1, _, _ = m.structure()
+    assert set(order.tolist()) <= {1, 2}
+    r = m.rates
+    assert (r >= 1e-3 * (1 - 1e-12)).all() and (r <= 1.0).all()
+    # the same seed writes the same file
+    assert open(path).read() == open(gen(tmp_path, "t2.xml", "-r", "16", "-c", "40", "--toy", "--seed", "5",
+                                         "--log-rates", "0.001", "1")).read()
+
+
+def test_simulator_flags(tmp_path):
+    path = gen(tmp_path, "w.xml", "-r", "8", "-w", "--seed", "3")
+    bad = subprocess.run([SIM, "-f", path, "-t", "-1"], capture_output=True, text=True, timeout=60)
+    assert bad.returncode != 0 and "-t" in bad.stderr
+    nofile = subprocess.run([SIM, "-t", "1"], capture_output=True, text=True, timeout=60)
+    assert nofile.returncode != 0 and "-f" in nofile.stderr
+    unknown = subprocess.run([SIM, "-f", path, "-t", "1", "--bogus"], capture_output=True, text=True, timeout=60)
+    assert unknown.returncode != 0 and "unknown option" in unknown.stderr
+    missing = subprocess.run([SIM, "-f", str(tmp_path / "none.xml"), "-t", "1"], capture_output=True, text=True,
+                             timeout=60)
+    assert missing.returncode != 0
+
+
+def test_simulator_without_gpu_fails_loudly(tmp_path):
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    path = gen(tmp_path, "w.xml", "-r", "8", "-w", "--seed", "3")
+    r = subprocess.run([SIM, "-f", path, "-t", "1", "-n", "10"], capture_output=True, text=True, timeout=60,
+                       cwd=str(tmp_path))
+    assert r.returncode != 0
+    assert "HIP" in (r.stdout + r.stderr)
