@@ -445,13 +445,20 @@ std::map<TuneKey, PersistentConfig> g_tuned;
 std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, bool sens, int32_t n_max, int32_t m_max) {
     std::vector<PersistentConfig> out;
     const int shapes[][2] = {{64, 64}, {256, 64}, {256, 256}, {512, 512}, {1024, 1024}};
+    // one-wavefront groups come in register budgets of 5 (the default), 4
+    // and, for FP32, 6 waves per SIMD
+    const int wpe64_64[] = {5, 4}, wpe64_32[] = {5, 6, 4};
     for (auto &sh : shapes)
         for (int al = 1; al >= 0; --al) {
-            PersistentConfig c{sh[0], sh[1], al != 0};
-            const size_t lds = persistent_lds_bytes(c, is64, n_max, m_max);
-            if (lds > dc.lds_per_block) continue;
-            if (persistent_occupancy(c, is64, sens, lds) <= 0) continue;
-            out.push_back(c);
+            const int *wl = sh[1] == 64 ? (is64 ? wpe64_64 : wpe64_32) : nullptr;
+            const int nw = sh[1] == 64 ? (is64 ? 2 : 3) : 1;
+            for (int w = 0; w < nw; ++w) {
+                PersistentConfig c{sh[0], sh[1], al != 0, wl ? wl[w] : 1};
+                const size_t lds = persistent_lds_bytes(c, is64, n_max, m_max);
+                if (lds > dc.lds_per_block) continue;
+                if (persistent_occupancy(c, is64, sens, lds) <= 0) continue;
+                out.push_back(c);
+            }
         }
     return out;
 }
@@ -461,7 +468,7 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
 PersistentConfig heuristic(const std::vector<PersistentConfig> &cands, int32_t m_max) {
     auto pick = [&](int b, int g) -> const PersistentConfig * {
         for (auto &c : cands)
-            if (c.block == b && c.group == g) return &c;  // a_lds variants come first
+            if (c.block == b && c.group == g) return &c;  // a_lds and 5-wave variants come first
         return nullptr;
     };
     const PersistentConfig *c = nullptr;
@@ -470,7 +477,7 @@ PersistentConfig heuristic(const std::vector<PersistentConfig> &cands, int32_t m
     if (!c) c = pick(1024, 1024);
     if (!c) c = pick(512, 512);
     if (!c && !cands.empty()) c = &cands.front();
-    return c ? *c : PersistentConfig{0, 0, false};
+    return c ? *c : PersistentConfig{0, 0, false, 1};
 }
 
 PersistentLaunch make_launch(DeviceContext &dc, const PersistentConfig &cfg, bool is64, bool sens,
@@ -601,7 +608,8 @@ tl_status autotune(DeviceContext &dc, bool is64, bool sens, const std::vector<Mo
                 left -= take;
             }
             const double rate = ms > 0 ? (double)done / ms : 0.0;
-            if (c.block == first.block && c.group == first.group && c.a_lds == first.a_lds) first_rate = rate;
+            if (c.block == first.block && c.group == first.group && c.a_lds == first.a_lds && c.wpe == first.wpe)
+                first_rate = rate;
             if (rate > best_rate) {
                 best_rate = rate;
                 best = c;
@@ -675,7 +683,7 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
     for (int32_t i = 0; i < n_models; ++i) {
         int cls = 0;
         if (bs_req <= 0) {
-            const size_t b = persistent_lds_bytes(PersistentConfig{64, 64, true}, is64, E[i].m->N, E[i].m->M);
+            const size_t b = persistent_lds_bytes(PersistentConfig{64, 64, true, 5}, is64, E[i].m->N, E[i].m->M);
             size_t lim = 4096;
             while (b > lim && cls < 12) {
                 lim *= 2;

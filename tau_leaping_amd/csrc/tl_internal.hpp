@@ -117,6 +117,7 @@ struct PersistentConfig {
     int block;     // threads per block
     int group;     // lanes per realization (64 = one wave, == block = whole block)
     bool a_lds;    // propensities in LDS (else recomputed where they are read)
+    int wpe;       // waves per SIMD the kernel is compiled for (register budget 512 / wpe)
 };
 
 struct PersistentLaunch {

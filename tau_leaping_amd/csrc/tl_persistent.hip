@@ -1085,8 +1085,10 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
     G.sync();
 }
 
-template <class R, int BS, int GS, bool ALDS, bool SENS>
-__global__ __launch_bounds__(BS, (GS == 64 ? 4 : 1)) void k_persistent(PersistentLaunch L) {
+// WPE: waves per SIMD the register allocation targets (more waves hide more
+// latency; fewer give each wave more registers).
+template <class R, int BS, int GS, bool ALDS, bool SENS, int WPE>
+__global__ __launch_bounds__(BS, WPE) void k_persistent(PersistentLaunch L) {
     extern __shared__ __align__(16) unsigned char lds[];
     const GroupLayout lay = group_layout(L.n_max, L.m_max, GS, ALDS, sizeof(R));
     const int tid = threadIdx.x;
@@ -1131,23 +1133,32 @@ __global__ __launch_bounds__(BS, (GS == 64 ? 4 : 1)) void k_persistent(Persisten
 
 // ---------------------------------------------------------- dispatch ----
 template <class R, bool SENS>
-const void *kernel_ptr(int bs, int gs, bool alds) {
-#define TL_K(BS_, GS_)                                                                                 \
-    if (bs == BS_ && gs == GS_)                                                                        \
-        return alds ? (const void *)k_persistent<R, BS_, GS_, true, SENS>                              \
-                    : (const void *)k_persistent<R, BS_, GS_, false, SENS>;
-    TL_K(64, 64)
-    TL_K(256, 64)
-    TL_K(256, 256)
-    TL_K(512, 512)
-    TL_K(1024, 1024)
+const void *kernel_ptr(int bs, int gs, bool alds, int wpe) {
+#define TL_K(BS_, GS_, W_)                                                                             \
+    if (bs == BS_ && gs == GS_ && wpe == W_)                                                           \
+        return alds ? (const void *)k_persistent<R, BS_, GS_, true, SENS, W_>                          \
+                    : (const void *)k_persistent<R, BS_, GS_, false, SENS, W_>;
+    TL_K(64, 64, 4)
+    TL_K(64, 64, 5)
+    TL_K(256, 64, 4)
+    TL_K(256, 64, 5)
+    if constexpr (sizeof(R) == 4) {
+        TL_K(64, 64, 6)
+        TL_K(256, 64, 6)
+    }
+    TL_K(256, 256, 1)
+    TL_K(512, 512, 1)
+    TL_K(1024, 1024, 1)
 #undef TL_K
     return nullptr;
 }
 
 const void *kernel_for(const PersistentConfig &c, bool is64, bool sens) {
-    if (is64) return sens ? kernel_ptr<double, true>(c.block, c.group, c.a_lds) : kernel_ptr<double, false>(c.block, c.group, c.a_lds);
-    return sens ? kernel_ptr<float, true>(c.block, c.group, c.a_lds) : kernel_ptr<float, false>(c.block, c.group, c.a_lds);
+    if (is64)
+        return sens ? kernel_ptr<double, true>(c.block, c.group, c.a_lds, c.wpe)
+                    : kernel_ptr<double, false>(c.block, c.group, c.a_lds, c.wpe);
+    return sens ? kernel_ptr<float, true>(c.block, c.group, c.a_lds, c.wpe)
+                : kernel_ptr<float, false>(c.block, c.group, c.a_lds, c.wpe);
 }
 
 // ------------------------------------------------------- gradient reduce --
