@@ -113,7 +113,7 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     }
     const double t0 = now_s();
     // one allocation: c64 | c32 | s0 | s1 | row_ptr | csr | col_ptr | csc | x0 | order | g_order | rs |
-    // dep_ptr | dep | colq | aff_ptr | aff | tile_ptr
+    // dep_ptr | dep | colq | aff_ptr | aff | tile_ptr | aff_rs | aff_c64 | aff_c32
     const size_t M = (size_t)m.M, N = (size_t)m.N, Z = (size_t)std::max(m.nnz, 1);
     std::vector<size_t> off;
     size_t total = 0;
@@ -140,6 +140,10 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     add((size_t)std::max<size_t>(m.aff_ptr.size(), 1) * 4);  // aff_ptr
     add(std::max<size_t>(m.aff.size(), 1) * 4);      // aff
     add(m.tile_ptr.size() * 4);                       // tile_ptr
+    const size_t NA = std::max<size_t>(m.aff.size(), 1);
+    add(NA * 8);  // aff_rs
+    add(NA * 8);  // aff_c64
+    add(NA * 4);  // aff_c32
     std::vector<unsigned char> host(total, 0);
     std::vector<float> c32(M);
     for (size_t j = 0; j < M; ++j) c32[j] = (float)m.rates[j];
@@ -167,6 +171,18 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     if (!m.aff_ptr.empty()) memcpy(&host[off[15]], m.aff_ptr.data(), m.aff_ptr.size() * 4);
     if (!m.aff.empty()) memcpy(&host[off[16]], m.aff.data(), m.aff.size() * 4);
     memcpy(&host[off[17]], m.tile_ptr.data(), m.tile_ptr.size() * 4);
+    {
+        int32_t *ars = reinterpret_cast<int32_t *>(&host[off[18]]);
+        double *ac64 = reinterpret_cast<double *>(&host[off[19]]);
+        float *ac32 = reinterpret_cast<float *>(&host[off[20]]);
+        for (size_t d = 0; d < m.aff.size(); ++d) {
+            const int32_t k = m.aff[d];
+            ars[2 * d] = m.s0[k] | ((int32_t)m.order[k] << 29);
+            ars[2 * d + 1] = m.s1[k];
+            ac64[d] = m.rates[k];
+            ac32[d] = (float)m.rates[k];
+        }
+    }
 
     DeviceModelCache *c;
     if (it != m.dev.end()) {
@@ -207,6 +223,9 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     d.aff_ptr = m.aff_ptr.empty() ? nullptr : (const int32_t *)(b + off[15]);
     d.aff = m.aff_ptr.empty() ? nullptr : (const int32_t *)(b + off[16]);
     d.tile_ptr = (const int32_t *)(b + off[17]);
+    d.aff_rs = (const int2 *)(b + off[18]);
+    d.aff_c64 = (const double *)(b + off[19]);
+    d.aff_c32 = (const float *)(b + off[20]);
     c->version = m.version;
     out = d;
     if (upload_seconds) *upload_seconds = now_s() - t0;

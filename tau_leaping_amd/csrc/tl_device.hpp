@@ -57,6 +57,9 @@ struct DevModel {
     const int4 *colq;        // [M] column entries padded with v = 0, or null (a column longer than 4)
     const int32_t *aff_ptr;  // [M+1] channel -> channels whose propensity it can change, or null
     const int32_t *aff;      // [aff_ptr[M]]
+    const int2 *aff_rs;      // [aff_ptr[M]] rs of each listed channel (loads in parallel with aff)
+    const double *aff_c64;   // [aff_ptr[M]] its rate
+    const float *aff_c32;
     const int32_t *tile_ptr; // [N][n_tiles + 1] CSR offset of each species row at channel t * kTileChannels
 };
 

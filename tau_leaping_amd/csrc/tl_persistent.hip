@@ -39,6 +39,10 @@ template <class T>
 __device__ __forceinline__ gptr<T> as_global(T *p) {
     return (gptr<T>)p;
 }
+// Constant-address-space view of read-only model data: a load at a
+// wave-uniform index becomes a scalar load (scalar cache).
+template <class T>
+using cptr = const T __attribute__((address_space(4))) *;
 
 // Native vector types (the HIP vector classes do not load through
 // address-space-qualified pointers).
@@ -54,6 +58,15 @@ struct GModel {
     gptr<const int8_t> g_order;
     gptr<const v2i> rs;
     gptr<const v4i> colq;  // null when a column has more than 4 entries
+    gptr<const v2i> aff_rs;
+    gptr<const double> aff_c64;
+    gptr<const float> aff_c32;
+    // column entries of a wave-uniform channel (scalar load)
+    __device__ __forceinline__ int4 colq_uniform(int32_t j) const {
+        const v4i v = ((cptr<v4i>)colq)[j];
+        return make_int4(v.x, v.y, v.z, v.w);
+    }
+    __device__ __forceinline__ int32_t aff_ptr_uniform(int32_t j) const { return ((cptr<int32_t>)aff_ptr)[j]; }
     __device__ __forceinline__ int2 rs_at(int32_t j) const {
         const v2i v = rs[j];
         return make_int2(v.x, v.y);
@@ -90,6 +103,9 @@ __device__ __forceinline__ GModel global_model(const DevModel &d) {
     g.g_order = as_global(d.g_order);
     g.rs = as_global(reinterpret_cast<const v2i *>(d.rs));
     g.colq = as_global(reinterpret_cast<const v4i *>(d.colq));
+    g.aff_rs = as_global(reinterpret_cast<const v2i *>(d.aff_rs));
+    g.aff_c64 = as_global(d.aff_c64);
+    g.aff_c32 = as_global(d.aff_c32);
     return g;
 }
 
@@ -580,8 +596,6 @@ __device__ __forceinline__ bool prop_one(const GModel &m, int32_t j, int2 rsj, R
     return cr;
 }
 
-// Two channels per lane per round, with the constants of both loaded up
-// front (more loads in flight per wavefront).
 template <class R, int GS, class P>
 __device__ __forceinline__ void prop_pass(const GModel &m, gptr<const R> c, const int32_t *xs, const P &pa,
                                           uint64_t *critq, double *mu, double *sig, int lane, double &s,
@@ -589,31 +603,16 @@ __device__ __forceinline__ void prop_pass(const GModel &m, gptr<const R> c, cons
     s = 0.0;
     sc = 0.0;
     const bool wantq = m.colq != nullptr;
-    for (int32_t j0 = 0; j0 < m.M; j0 += 2 * GS) {
-        const int32_t ja = j0 + lane, jb = ja + GS;
-        const bool va = ja < m.M, vb = jb < m.M;
-        int2 ra = make_int2(0, 0), rb = make_int2(0, 0);
-        R ca = (R)0, cb = (R)0;
-        int4 qa = make_int4(2, 2, 2, 2), qb = make_int4(2, 2, 2, 2);
-        if (va) {
-            ra = m.rs_at(ja);
-            ca = c[ja];
-            if (wantq) qa = m.colq_at(ja);
+    for (int32_t j0 = 0; j0 < m.M; j0 += GS) {
+        const int32_t j = j0 + lane;
+        bool cr = false;
+        if (j < m.M) {
+            const int4 q = wantq ? m.colq_at(j) : make_int4(2, 2, 2, 2);
+            cr = prop_one<R>(m, j, m.rs_at(j), c[j], q, xs, pa, mu, sig, s, sc);
         }
-        if (vb) {
-            rb = m.rs_at(jb);
-            cb = c[jb];
-            if (wantq) qb = m.colq_at(jb);
-        }
-        bool cra = false, crb = false;
-        if (va) cra = prop_one<R>(m, ja, ra, ca, qa, xs, pa, mu, sig, s, sc);
-        if (vb) crb = prop_one<R>(m, jb, rb, cb, qb, xs, pa, mu, sig, s, sc);
-        const unsigned long long bala = __ballot(cra), balb = __ballot(crb);
-        const int32_t ta = j0 + (lane & ~63), tb = ta + GS;
-        if ((lane & 63) == 0) {
-            if (ta < m.M) critq[ta >> 6] = bala;
-            if (tb < m.M) critq[tb >> 6] = balb;
-        }
+        const unsigned long long bal = __ballot(cr);
+        const int32_t tile0 = j0 + (lane & ~63);
+        if ((lane & 63) == 0 && tile0 < m.M) critq[tile0 >> 6] = bal;
     }
 }
 
@@ -702,14 +701,21 @@ __device__ __forceinline__ void prop_update(const GModel &m, gptr<const R> c, co
                                             int32_t jf, int lane) {
     if constexpr (!P::kStored) return;
     if (m.aff_ptr) {
-        const int32_t d1 = m.aff_ptr[jf + 1];
-        for (int32_t d = m.aff_ptr[jf] + lane; d < d1; d += GS) {
+        // the listed channel's index, rs and rate load side by side
+        const int32_t d1 = m.aff_ptr_uniform(jf + 1);
+        for (int32_t d = m.aff_ptr_uniform(jf) + lane; d < d1; d += GS) {
             const int32_t k = m.aff[d];
+            const v2i p = m.aff_rs[d];
+            R ck;
+            if constexpr (sizeof(R) == 8)
+                ck = m.aff_c64[d];
+            else
+                ck = m.aff_c32[d];
             int order;
             int32_t s0, s1;
             bool cr;
-            chan_unpack(m.rs_at(k), order, s0, s1);
-            pa.set(k, propensity<R>(order, c[k], xs[s0], xs[s1], cr));
+            chan_unpack(make_int2(p.x, p.y), order, s0, s1);
+            pa.set(k, propensity<R>(order, ck, xs[s0], xs[s1], cr));
         }
         return;
     }
@@ -753,12 +759,14 @@ __device__ __forceinline__ void for_each_firing(const GModel &m, const Grp<GS> &
     const uint32_t ck = c3 | kKindPoisson;
     for (int32_t c0 = 0; c0 < m.M; c0 += 4 * GS) {
         if (c0 + 4 * G.lane < m.M) head[G.lane] = head_block(key, (uint32_t)((c0 >> 2) + G.lane), epoch, c3);
-        // the four propensities first (their loads overlap)
-        R a4[4];
+        // recomputed propensities: the four first (their loads overlap)
+        R a4[4] = {(R)0, (R)0, (R)0, (R)0};
+        if constexpr (!P::kStored) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int32_t j = c0 + G.lane + GS * i;
-            a4[i] = (j < m.M) ? pa.get(j) : (R)0;
+            for (int i = 0; i < 4; ++i) {
+                const int32_t j = c0 + G.lane + GS * i;
+                a4[i] = (j < m.M) ? pa.get(j) : (R)0;
+            }
         }
         G.sync();
 #pragma unroll 1
@@ -766,7 +774,11 @@ __device__ __forceinline__ void for_each_firing(const GModel &m, const Grp<GS> &
             const int32_t j = c0 + G.lane + GS * i;
             if (j < m.M) {
                 const bool cr = crit_bit(critq, j);
-                const R aj = i == 0 ? a4[0] : (i == 1 ? a4[1] : (i == 2 ? a4[2] : a4[3]));
+                R aj;
+                if constexpr (P::kStored)
+                    aj = pa.get(j);
+                else
+                    aj = i == 0 ? a4[0] : (i == 1 ? a4[1] : (i == 2 ? a4[2] : a4[3]));
                 const int32_t kj = cr ? (j == jfire ? 1 : 0)
                                       : poisson_draw<R>(aj * tr, hflat[G.lane + GS * i], key, (uint32_t)j, epoch, ck);
                 f(j, cr, aj, kj);
@@ -939,7 +951,8 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                     }
                     if constexpr (SENS && !P::kStored) G.sync();
                     if (m.colq) {
-                        if (lane < 4) column_add(xs, reinterpret_cast<gptr<const int32_t>>(m.colq)[(size_t)jf * 4 + lane], 1);
+                        const int4 q = m.colq_uniform(jf);
+                        if (lane < 4) column_add(xs, lane == 0 ? q.x : (lane == 1 ? q.y : (lane == 2 ? q.z : q.w)), 1);
                     } else {
                         const int32_t e1 = m.col_ptr[jf + 1];
                         for (int32_t e = m.col_ptr[jf] + lane; e < e1; e += GS) column_add(xs, m.csc[e], 1);
