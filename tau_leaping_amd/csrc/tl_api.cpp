@@ -491,7 +491,8 @@ PersistentConfig heuristic(const std::vector<PersistentConfig> &cands, int32_t m
         return nullptr;
     };
     const PersistentConfig *c = nullptr;
-    if (m_max <= 4096) c = pick(256, 64);
+    if (m_max <= 4096) c = pick(64, 64);
+    if (!c && m_max <= 4096) c = pick(256, 64);
     if (!c && m_max <= 16384) c = pick(256, 256);
     if (!c) c = pick(1024, 1024);
     if (!c) c = pick(512, 512);
@@ -596,9 +597,18 @@ tl_status autotune(DeviceContext &dc, bool is64, bool sens, const std::vector<Mo
             return TL_OK;
         }
     }
-    auto cands = candidates(dc, is64, sens, n_max, m_max);
-    if (cands.empty()) return set_error(TL_ERR_TOO_LARGE, "model too large for any persistent launch shape");
-    best = heuristic(cands, m_max);
+    auto all = candidates(dc, is64, sens, n_max, m_max);
+    if (all.empty()) return set_error(TL_ERR_TOO_LARGE, "model too large for any persistent launch shape");
+    best = heuristic(all, m_max);
+    // Timed candidates: the default one-wavefront shape and the block shapes.
+    // Short trials do not separate one-wavefront variants reliably (their
+    // first steps are not representative); the default among them was the
+    // fastest measured on full runs.
+    std::vector<PersistentConfig> cands;
+    for (auto &c : all)
+        if (c.group != 64 || (c.block == best.block && c.group == best.group && c.a_lds == best.a_lds &&
+                              c.wpe == best.wpe))
+            cands.push_back(c);
     const int64_t sample = std::min<int64_t>(n_items, (int64_t)dc.n_cu * 16);
     if (cands.size() > 1 && n_items >= 4 * sample) {
         std::vector<ModelItem> trial = items_in;
@@ -760,7 +770,7 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
         if (bs_req > 0) {
             auto cands = candidates(*dc, is64, sens, n_max, m_max);
             const int gs = bs_req <= 64 ? 64 : bs_req;
-            const int bs = bs_req <= 64 ? 256 : bs_req;
+            const int bs = bs_req <= 64 ? 64 : bs_req;
             bool found = false;
             for (auto &c : cands)
                 if (c.block == bs && c.group == gs) {

@@ -289,6 +289,42 @@ __device__ __forceinline__ int wave_max_int(int v) {
 
 __device__ __forceinline__ bool crit_bit(const uint64_t *critq, int32_t j) { return (critq[j >> 6] >> (j & 63)) & 1ull; }
 
+// Criticality bits of channels [lo, hi) within word w (bit b = channel 64w + b).
+__device__ __forceinline__ uint64_t crit_word(const uint64_t *critq, int32_t w, int32_t lo, int32_t hi) {
+    const int32_t b0 = max(lo - 64 * w, 0), b1 = min(hi - 64 * w, 64);
+    if (b1 <= b0) return 0ull;
+    const uint64_t upto = (b1 == 64) ? ~0ull : ((1ull << b1) - 1ull);
+    return critq[w] & upto & ~((1ull << b0) - 1ull);
+}
+
+// First channel j of [j0, j1) (critical ones only when masked) at which
+// excl + the cumulative weight passes target, else -1; `last` is the last
+// channel with positive weight.
+template <class P>
+__device__ __forceinline__ int32_t walk_chunk(const P &pa, const uint64_t *critq, bool masked, int32_t j0, int32_t j1,
+                                              double excl, double target, int32_t &last) {
+    double cum = excl;
+    if (masked) {
+        if (j0 < j1)
+            for (int32_t w = j0 >> 6; w <= (j1 - 1) >> 6; ++w)
+                for (uint64_t bits = crit_word(critq, w, j0, j1); bits; bits &= bits - 1ull) {
+                    const int32_t j = 64 * w + (int32_t)__builtin_ctzll(bits);
+                    const double v = (double)pa.get(j);
+                    if (v > 0.0) last = j;
+                    cum += v;
+                    if (cum > target) return j;
+                }
+        return -1;
+    }
+    for (int32_t j = j0; j < j1; ++j) {
+        const double v = (double)pa.get(j);
+        if (v > 0.0) last = j;
+        cum += v;
+        if (cum > target) return j;
+    }
+    return -1;
+}
+
 // ---------------------------------------------------------------- groups --
 // GS == 64: one wavefront per realization.  Only wavefront-level
 // synchronisation is used, so waves of a block never wait on each other.
@@ -426,8 +462,15 @@ struct Grp {
         sc.j0 = min(M, lane * C);
         sc.j1 = min(M, sc.j0 + C);
         double s = 0.0;
-        for (int32_t j = sc.j0; j < sc.j1; ++j)
-            if (!masked || crit_bit(critq, j)) s += (double)pa.get(j);
+        if (masked) {
+            // critical channels only: visit the set bits of the chunk
+            if (sc.j0 < sc.j1)
+                for (int32_t w = sc.j0 >> 6; w <= (sc.j1 - 1) >> 6; ++w)
+                    for (uint64_t bits = crit_word(critq, w, sc.j0, sc.j1); bits; bits &= bits - 1ull)
+                        s += (double)pa.get(64 * w + (int32_t)__builtin_ctzll(bits));
+        } else {
+            for (int32_t j = sc.j0; j < sc.j1; ++j) s += (double)pa.get(j);
+        }
         const double incl_w = wave_incl_scan(s, wl);
         double base = 0.0, total;
         if constexpr (GS != 64) {
@@ -469,18 +512,8 @@ struct Grp {
             if (hit == 0ull) return -1;
             const int L = (int)__builtin_ctzll(hit);
             if (wl == L) {
-                double cum = excl;
                 int32_t last = -1;
-                for (int32_t j = j0; j < j1; ++j) {
-                    if (masked && !crit_bit(critq, j)) continue;
-                    const double w = (double)pa.get(j);
-                    if (w > 0.0) last = j;
-                    cum += w;
-                    if (cum > target) {
-                        res = j;
-                        break;
-                    }
-                }
+                res = walk_chunk(pa, critq, masked, j0, j1, excl, target, last);
                 if (res < 0) res = last;  // rounding at the chunk end
             }
             return uni(__builtin_amdgcn_readlane(res, L));
@@ -492,18 +525,8 @@ struct Grp {
             __syncthreads();
             const int L = slot[0];
             if (lane == L) {
-                double cum = excl;
                 int32_t last = -1;
-                for (int32_t j = j0; j < j1; ++j) {
-                    if (masked && !crit_bit(critq, j)) continue;
-                    const double w = (double)pa.get(j);
-                    if (w > 0.0) last = j;
-                    cum += w;
-                    if (cum > target) {
-                        res = j;
-                        break;
-                    }
-                }
+                res = walk_chunk(pa, critq, masked, j0, j1, excl, target, last);
                 slot[1] = (res < 0) ? last : res;
             }
             __syncthreads();
