@@ -112,7 +112,7 @@ __device__ __forceinline__ GModel global_model(const DevModel &d) {
 // LDS layout of one group.  Species sums mu[N], sig[N] of the tau bound;
 // block groups without a[M] in LDS also hold one propensity tile.
 struct GroupLayout {
-    size_t off_x, off_dx, off_snap, off_head, off_crit, off_red, off_mu, off_sig, off_tile, off_a, bytes;
+    size_t off_x, off_dx, off_snap, off_crit, off_red, off_mu, off_sig, off_tile, off_a, bytes;
 };
 __host__ __device__ inline GroupLayout group_layout(int32_t n_max, int32_t m_max, int gs, bool a_lds,
                                                     size_t real_bytes) {
@@ -121,8 +121,7 @@ __host__ __device__ inline GroupLayout group_layout(int32_t n_max, int32_t m_max
     L.off_x = 0;
     L.off_dx = al16((size_t)n_max * 4);
     L.off_snap = L.off_dx + al16((size_t)n_max * 4);
-    L.off_head = L.off_snap + al16((size_t)n_max * 4);
-    L.off_crit = L.off_head + (size_t)gs * 16;
+    L.off_crit = L.off_snap + al16((size_t)n_max * 4);
     L.off_red = L.off_crit + al16((size_t)((m_max + 63) / 64) * 8);
     L.off_mu = L.off_red + al16((size_t)(4 * nw + 4) * 8);
     L.off_sig = L.off_mu + al16((size_t)n_max * 8);
@@ -770,31 +769,28 @@ __device__ __forceinline__ double score_term(bool cr, R aj, R tr, double tau, in
     return (double)kj - mj;
 }
 
-// Visit every channel of one leap attempt with its firing count k_j.  The
-// first draws of channels [c0, c0 + 4*GS) are produced one Philox block per
-// lane (4 channels per block) into the LDS staging buffer, then each lane
-// samples its channels c0 + lane + GS*i.
+// Visit every channel of one leap attempt with its firing count k_j.  Lane l
+// takes the four consecutive channels 4l..4l+3 of each span of 4*GS
+// channels, so the Philox block holding their first draws (channels 4b..4b+3
+// share block b) is its own: no staging, no barrier inside the pass.  The
+// caller synchronizes before reading what f wrote.
 template <class R, int GS, class P, class F>
 __device__ __forceinline__ void for_each_firing(const GModel &m, const Grp<GS> &G, const P &pa, const uint64_t *critq,
-                                                uint4 *head, uint2 key, uint32_t epoch, uint32_t c3, R tr,
-                                                int32_t jfire, F &&f) {
-    const uint32_t *hflat = reinterpret_cast<const uint32_t *>(head);
+                                                uint2 key, uint32_t epoch, uint32_t c3, R tr, int32_t jfire, F &&f) {
     const uint32_t ck = c3 | kKindPoisson;
     for (int32_t c0 = 0; c0 < m.M; c0 += 4 * GS) {
-        if (c0 + 4 * G.lane < m.M) head[G.lane] = head_block(key, (uint32_t)((c0 >> 2) + G.lane), epoch, c3);
+        const int32_t jb = c0 + 4 * G.lane;
+        if (jb >= m.M) continue;
+        const uint4 h = head_block(key, (uint32_t)(jb >> 2), epoch, c3);
         // recomputed propensities: the four first (their loads overlap)
         R a4[4] = {(R)0, (R)0, (R)0, (R)0};
         if constexpr (!P::kStored) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int32_t j = c0 + G.lane + GS * i;
-                a4[i] = (j < m.M) ? pa.get(j) : (R)0;
-            }
+            for (int i = 0; i < 4; ++i) a4[i] = (jb + i < m.M) ? pa.get(jb + i) : (R)0;
         }
-        G.sync();
 #pragma unroll 1
         for (int i = 0; i < 4; ++i) {
-            const int32_t j = c0 + G.lane + GS * i;
+            const int32_t j = jb + i;
             if (j < m.M) {
                 const bool cr = crit_bit(critq, j);
                 R aj;
@@ -802,12 +798,11 @@ __device__ __forceinline__ void for_each_firing(const GModel &m, const Grp<GS> &
                     aj = pa.get(j);
                 else
                     aj = i == 0 ? a4[0] : (i == 1 ? a4[1] : (i == 2 ? a4[2] : a4[3]));
-                const int32_t kj = cr ? (j == jfire ? 1 : 0)
-                                      : poisson_draw<R>(aj * tr, hflat[G.lane + GS * i], key, (uint32_t)j, epoch, ck);
+                const int32_t kj =
+                    cr ? (j == jfire ? 1 : 0) : poisson_draw<R>(aj * tr, quad_pick(h, i), key, (uint32_t)j, epoch, ck);
                 f(j, cr, aj, kj);
             }
         }
-        G.sync();
     }
 }
 
@@ -835,7 +830,7 @@ __device__ __forceinline__ void grid_record(const ModelItem &mi, int64_t r, int 
 
 template <class R, int GS, bool SENS, class P>
 __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, int32_t *xs, int32_t *dx,
-                              int32_t *snap, uint4 *head, uint64_t *critq, double *mu, double *sig, R *tile,
+                              int32_t *snap, uint64_t *critq, double *mu, double *sig, R *tile,
                               const P &pa) {
     const GModel m = global_model(mi.dm);
     const int lane = G.lane;
@@ -1013,7 +1008,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
             const int32_t jfire = fire ? jc : -1;
             const R tr = (R)tau;
             double qpart = 0.0;
-            for_each_firing<R, GS>(m, G, pa, critq, head, key, epoch, c3, tr, jfire,
+            for_each_firing<R, GS>(m, G, pa, critq, key, epoch, c3, tr, jfire,
                                    [&](int32_t j, bool cr, R aj, int32_t kj) {
                                        if (kj) {
                                            if (m.colq) {
@@ -1033,13 +1028,14 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                                            if (c[j] != (R)0) S[j] += term / (double)c[j];
                                        }
                                    });
+            G.sync();  // dx complete
             bool neg = false;
             for (int32_t i = lane; i < m.N; i += GS) neg |= (xs[i] + dx[i]) < 0;
             neg = G.any(neg);
             if (neg) {
                 for (int32_t i = lane; i < m.N; i += GS) dx[i] = 0;
                 if (SENS)  // take back this attempt's score terms (same draws)
-                    for_each_firing<R, GS>(m, G, pa, critq, head, key, epoch, c3, tr, jfire,
+                    for_each_firing<R, GS>(m, G, pa, critq, key, epoch, c3, tr, jfire,
                                            [&](int32_t j, bool cr, R aj, int32_t kj) {
                                                if (c[j] != (R)0) S[j] -= score_term<R>(cr, aj, tr, tau, kj) / (double)c[j];
                                            });
@@ -1137,7 +1133,6 @@ __global__ __launch_bounds__(BS, WPE) void k_persistent(PersistentLaunch L) {
     int32_t *xs = reinterpret_cast<int32_t *>(base + lay.off_x);
     int32_t *dx = reinterpret_cast<int32_t *>(base + lay.off_dx);
     int32_t *snap = reinterpret_cast<int32_t *>(base + lay.off_snap);
-    uint4 *head = reinterpret_cast<uint4 *>(base + lay.off_head);
     uint64_t *critq = reinterpret_cast<uint64_t *>(base + lay.off_crit);
     G.red = reinterpret_cast<double *>(base + lay.off_red);
     double *mu = reinterpret_cast<double *>(base + lay.off_mu);
@@ -1158,11 +1153,11 @@ __global__ __launch_bounds__(BS, WPE) void k_persistent(PersistentLaunch L) {
         const ModelItem mi = ld_uniform(&L.items[uni(lo)]);
         if constexpr (ALDS) {
             const PropStored<R> pa{reinterpret_cast<R *>(base + lay.off_a)};
-            simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, head, critq, mu, sig, tile, pa);
+            simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, critq, mu, sig, tile, pa);
         } else {
             const GModel gm = global_model(mi.dm);
             const PropRecomputed<R> pa{gm.rs, gm.template rates<R>(), xs};
-            simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, head, critq, mu, sig, tile, pa);
+            simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, critq, mu, sig, tile, pa);
         }
     }
 }
