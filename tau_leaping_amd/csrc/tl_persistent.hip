@@ -691,8 +691,9 @@ __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c,
             double pm = 0.0, ps = 0.0;
             if (i < m.N) {
                 const int32_t e0 = m.tile_ptr[(size_t)i * (nt + 1) + t], e1 = m.tile_ptr[(size_t)i * (nt + 1) + t + 1];
-                for (int32_t e = e0 + k; e < e1; e += L) {
-                    const int32_t p = m.csr[e];
+                // four entries per round with their loads issued together; the
+                // additions keep the lane's entry order
+                auto add = [&](int32_t p) {
                     const int32_t j = entry_index(p);
                     if (!crit_bit(critq, j)) {
                         const double v = (double)entry_value(p);
@@ -700,7 +701,16 @@ __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c,
                         pm += tmp;
                         ps += v * tmp;
                     }
+                };
+                int32_t e = e0 + k;
+                for (; e + 3 * L < e1; e += 4 * L) {
+                    const int32_t p0 = m.csr[e], p1 = m.csr[e + L], p2 = m.csr[e + 2 * L], p3 = m.csr[e + 3 * L];
+                    add(p0);
+                    add(p1);
+                    add(p2);
+                    add(p3);
                 }
+                for (; e < e1; e += L) add(m.csr[e]);
             }
             for (int o = L >> 1; o > 0; o >>= 1) {
                 pm += __shfl_xor(pm, o, 64);
