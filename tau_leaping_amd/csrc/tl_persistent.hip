@@ -669,21 +669,44 @@ __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c,
             A = tile;
             aoff = t0;
         }
-        for (int32_t j0 = t0; j0 < t1; j0 += GS) {
-            const int32_t j = j0 + lane;
-            bool cr = false;
-            if (j < t1) {
+        // two channels per lane per round, constants loaded up front
+        for (int32_t j0 = t0; j0 < t1; j0 += 2 * GS) {
+            const int32_t ja = j0 + lane, jb = ja + GS;
+            int2 ra = make_int2(0, 0), rb = make_int2(0, 0);
+            R ca = (R)0, cb = (R)0;
+            if (ja < t1) {
+                ra = m.rs_at(ja);
+                ca = c[ja];
+            }
+            if (jb < t1) {
+                rb = m.rs_at(jb);
+                cb = c[jb];
+            }
+            bool cra = false, crb = false;
+            if (ja < t1) {
                 int order;
                 int32_t s0, s1;
-                chan_unpack(m.rs_at(j), order, s0, s1);
-                const R aj = propensity<R>(order, c[j], xs[s0], xs[s1], cr);
-                A[j - aoff] = aj;
+                chan_unpack(ra, order, s0, s1);
+                const R aj = propensity<R>(order, ca, xs[s0], xs[s1], cra);
+                A[ja - aoff] = aj;
                 s += (double)aj;
-                if (cr) sc += (double)aj;
+                if (cra) sc += (double)aj;
             }
-            const unsigned long long bal = __ballot(cr);
-            const int32_t tile0 = j0 + (lane & ~63);
-            if ((lane & 63) == 0 && tile0 < t1) critq[tile0 >> 6] = bal;
+            if (jb < t1) {
+                int order;
+                int32_t s0, s1;
+                chan_unpack(rb, order, s0, s1);
+                const R aj = propensity<R>(order, cb, xs[s0], xs[s1], crb);
+                A[jb - aoff] = aj;
+                s += (double)aj;
+                if (crb) sc += (double)aj;
+            }
+            const unsigned long long bala = __ballot(cra), balb = __ballot(crb);
+            const int32_t ta = j0 + (lane & ~63), tb = ta + GS;
+            if ((lane & 63) == 0) {
+                if (ta < t1) critq[ta >> 6] = bala;
+                if (tb < t1) critq[tb >> 6] = balb;
+            }
         }
         G.sync();
         for (int32_t i0 = 0; i0 < m.N; i0 += rows_per_pass) {
