@@ -458,18 +458,25 @@ struct Grp {
     template <class P>
     __device__ double scan_chunks(const P &pa, const uint64_t *critq, bool masked, int32_t M, Scan &sc) const {
         const int32_t C = (M + GS - 1) / GS;
-        sc.j0 = min(M, lane * C);
-        sc.j1 = min(M, sc.j0 + C);
+        const int32_t j0 = min(M, lane * C), j1 = min(M, j0 + C);
         double s = 0.0;
         if (masked) {
             // critical channels only: visit the set bits of the chunk
-            if (sc.j0 < sc.j1)
-                for (int32_t w = sc.j0 >> 6; w <= (sc.j1 - 1) >> 6; ++w)
-                    for (uint64_t bits = crit_word(critq, w, sc.j0, sc.j1); bits; bits &= bits - 1ull)
+            if (j0 < j1)
+                for (int32_t w = j0 >> 6; w <= (j1 - 1) >> 6; ++w)
+                    for (uint64_t bits = crit_word(critq, w, j0, j1); bits; bits &= bits - 1ull)
                         s += (double)pa.get(64 * w + (int32_t)__builtin_ctzll(bits));
         } else {
-            for (int32_t j = sc.j0; j < sc.j1; ++j) s += (double)pa.get(j);
+            for (int32_t j = j0; j < j1; ++j) s += (double)pa.get(j);
         }
+        return scan_sums(M, s, sc);
+    }
+
+    // The group scan of scan_chunks from lane chunk sums s computed elsewhere.
+    __device__ double scan_sums(int32_t M, double s, Scan &sc) const {
+        const int32_t C = (M + GS - 1) / GS;
+        sc.j0 = min(M, lane * C);
+        sc.j1 = min(M, sc.j0 + C);
         const double incl_w = wave_incl_scan(s, wl);
         double base = 0.0, total;
         if constexpr (GS != 64) {
@@ -499,6 +506,14 @@ struct Grp {
         Scan sc;
         scan_chunks<P>(pa, critq, masked, M, sc);
         return pick<P>(pa, critq, masked, sc, target);
+    }
+    // search over the critical channels with the lane chunk sums given
+    template <class P>
+    __device__ int32_t search_critical(const P &pa, const uint64_t *critq, int32_t M, double chunk_sum,
+                                       double target) const {
+        Scan sc;
+        scan_sums(M, chunk_sum, sc);
+        return pick<P>(pa, critq, true, sc, target);
     }
 
     template <class P>
@@ -648,10 +663,16 @@ __device__ __forceinline__ void prop_pass(const GModel &m, gptr<const R> c, cons
 template <class R, int GS, class P>
 __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c, const int32_t *xs, const P &pa,
                                                R *tile, uint64_t *critq, double *mu, double *sig, const Grp<GS> &G,
-                                               double &s, double &sc) {
+                                               double &s, double &sc, double &chunk_crit) {
     const int lane = G.lane;
     s = 0.0;
     sc = 0.0;
+    // this lane's chunk of the cumulative critical search (see Grp::scan_chunks):
+    // its critical propensities are summed here, in channel order, while
+    // each tile is in LDS
+    chunk_crit = 0.0;
+    const int32_t C = (m.M + GS - 1) / GS;
+    const int32_t cj0 = min(m.M, lane * C), cj1 = min(m.M, cj0 + C);
     int L = 1;
     while (L < 64 && 2 * L * m.N <= GS) L *= 2;
     const int32_t nt = (m.M + kTileChannels - 1) / kTileChannels;
@@ -709,6 +730,13 @@ __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c,
             }
         }
         G.sync();
+        {
+            const int32_t u0 = max(cj0, t0), u1 = min(cj1, t1);
+            if (u0 < u1)
+                for (int32_t w = u0 >> 6; w <= (u1 - 1) >> 6; ++w)
+                    for (uint64_t bits = crit_word(critq, w, u0, u1); bits; bits &= bits - 1ull)
+                        chunk_crit += (double)A[64 * w + (int32_t)__builtin_ctzll(bits) - aoff];
+        }
         for (int32_t i0 = 0; i0 < m.N; i0 += rows_per_pass) {
             const int32_t i = i0 + lane / L;
             double pm = 0.0, ps = 0.0;
@@ -917,11 +945,11 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
         const double t_before = t;
         if (grid)
             for (int32_t q = lane; q < mi.n_track; q += GS) snap[q] = xs[as_global(mi.track)[q]];
-        double a0, a0c;
+        double a0, a0c, chunk_crit = 0.0;
         if constexpr (GS == 64)
             prop_pass<R, GS>(m, c, xs, pa, critq, mu, sig, lane, a0, a0c);
         else
-            prop_tau_tiled<R, GS>(m, c, xs, pa, tile, critq, mu, sig, G, a0, a0c);
+            prop_tau_tiled<R, GS>(m, c, xs, pa, tile, critq, mu, sig, G, a0, a0c, chunk_crit);
         G.sum2(a0, a0c);
         G.sync();
         if (!(a0 > 0.0)) {  // absorbing state
@@ -1024,7 +1052,8 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
             int32_t jc = m.M;
             double tau2 = (double)kFltMax;
             if (a0c > 0.0) {
-                const int32_t f = G.search(pa, critq, true, m.M, a0c * r1);
+                const int32_t f = (GS == 64) ? G.search(pa, critq, true, m.M, a0c * r1)
+                                             : G.search_critical(pa, critq, m.M, chunk_crit, a0c * r1);
                 if (f >= 0) {
                     jc = f;
                     tau2 = -log(r2) / a0c;
