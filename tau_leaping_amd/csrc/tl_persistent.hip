@@ -112,7 +112,7 @@ __device__ __forceinline__ GModel global_model(const DevModel &d) {
 // LDS layout of one group.  Species sums mu[N], sig[N] of the tau bound;
 // block groups without a[M] in LDS also hold one propensity tile.
 struct GroupLayout {
-    size_t off_x, off_dx, off_snap, off_crit, off_red, off_mu, off_sig, off_tile, off_a, bytes;
+    size_t off_x, off_dx, off_snap, off_crit, off_red, off_mu, off_sig, off_wsum, off_tile, off_a, bytes;
 };
 __host__ __device__ inline GroupLayout group_layout(int32_t n_max, int32_t m_max, int gs, bool a_lds,
                                                     size_t real_bytes) {
@@ -125,7 +125,8 @@ __host__ __device__ inline GroupLayout group_layout(int32_t n_max, int32_t m_max
     L.off_red = L.off_crit + al16((size_t)((m_max + 63) / 64) * 8);
     L.off_mu = L.off_red + al16((size_t)(4 * nw + 4) * 8);
     L.off_sig = L.off_mu + al16((size_t)n_max * 8);
-    L.off_tile = L.off_sig + al16((size_t)n_max * 8);
+    L.off_wsum = L.off_sig + al16((size_t)n_max * 8);
+    L.off_tile = L.off_wsum + (gs > 64 ? al16((size_t)((m_max + 63) / 64) * 8) : 0);
     const bool tile = gs > 64 && !a_lds;
     L.off_a = L.off_tile + (tile ? al16((size_t)(m_max < kTileChannels ? m_max : kTileChannels) * real_bytes) : 0);
     L.bytes = L.off_a + (a_lds ? al16((size_t)m_max * real_bytes) : 0);
@@ -507,13 +508,67 @@ struct Grp {
         scan_chunks<P>(pa, critq, masked, M, sc);
         return pick<P>(pa, critq, masked, sc, target);
     }
-    // search over the critical channels with the lane chunk sums given
+    // Block groups: first critical channel at which the cumulative critical
+    // propensity passes target, from the per-word sums wsum[w] (channels
+    // 64w..64w+63; filled by the propensity pass).  A group scan over the
+    // word sums finds the word, then one wavefront scans its 64 channels.
+    // -1 when the total does not pass target.
     template <class P>
-    __device__ int32_t search_critical(const P &pa, const uint64_t *critq, int32_t M, double chunk_sum,
+    __device__ int32_t search_critical(const P &pa, const uint64_t *critq, int32_t M, const double *wsum,
                                        double target) const {
+        static_assert(GS > 64, "block groups only");
+        const int32_t nwd = (M + 63) / 64;
+        const int32_t C = (nwd + GS - 1) / GS;
+        const int32_t w0 = min(nwd, lane * C), w1 = min(nwd, w0 + C);
+        double s = 0.0;
+        for (int32_t w = w0; w < w1; ++w) s += wsum[w];
         Scan sc;
-        scan_sums(M, chunk_sum, sc);
-        return pick<P>(pa, critq, true, sc, target);
+        scan_sums(nwd, s, sc);
+        int *slot = reinterpret_cast<int *>(red + 2 * NW);
+        double *sbase = red + 2 * NW + 2;  // slot[0..2] occupy the two doubles before
+        if (lane == 0) slot[0] = 1 << 30;
+        __syncthreads();
+        const unsigned long long hit = __ballot(sc.incl > target);
+        if (hit != 0ull && wl == 0) atomicMin(&slot[0], wid * 64 + (int)__builtin_ctzll(hit));
+        __syncthreads();
+        const int L = slot[0];
+        if (L >= (1 << 30)) {
+            __syncthreads();
+            return -1;
+        }
+        if (lane == L) {  // the word within this lane's word chunk
+            double cum = sc.excl;
+            int32_t W = w1 - 1;
+            for (int32_t w = w0; w < w1; ++w) {
+                if (cum + wsum[w] > target) {
+                    W = w;
+                    break;
+                }
+                cum += wsum[w];
+            }
+            slot[1] = W;
+            *sbase = cum;
+        }
+        __syncthreads();
+        if (wid == 0) {
+            const int32_t W = slot[1];
+            const double base = *sbase;
+            const int32_t j = 64 * W + wl;
+            const double v = (j < M && crit_bit(critq, j)) ? (double)pa.get(j) : 0.0;
+            const double incl = base + wave_incl_scan(v, wl);
+            const unsigned long long h = __ballot(incl > target);
+            const unsigned long long pos = __ballot(v > 0.0);
+            int32_t res;
+            if (h != 0ull)
+                res = 64 * W + (int32_t)__builtin_ctzll(h);
+            else  // rounding at the word end: the last positive channel
+                res = pos ? 64 * W + 63 - (int32_t)__builtin_clzll(pos) : -1;
+            if (wl == 0) slot[2] = res;
+        }
+        __syncthreads();
+        const int32_t out = slot[2];
+        __syncthreads();
+        return uni(out);
     }
 
     template <class P>
@@ -662,17 +717,11 @@ __device__ __forceinline__ void prop_pass(const GModel &m, gptr<const R> c, cons
 // part to mu[i], sig[i].  No atomics, and a fixed order of additions.
 template <class R, int GS, class P>
 __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c, const int32_t *xs, const P &pa,
-                                               R *tile, uint64_t *critq, double *mu, double *sig, const Grp<GS> &G,
-                                               double &s, double &sc, double &chunk_crit) {
+                                               R *tile, uint64_t *critq, double *mu, double *sig, double *wsum,
+                                               const Grp<GS> &G, double &s, double &sc) {
     const int lane = G.lane;
     s = 0.0;
     sc = 0.0;
-    // this lane's chunk of the cumulative critical search (see Grp::scan_chunks):
-    // its critical propensities are summed here, in channel order, while
-    // each tile is in LDS
-    chunk_crit = 0.0;
-    const int32_t C = (m.M + GS - 1) / GS;
-    const int32_t cj0 = min(m.M, lane * C), cj1 = min(m.M, cj0 + C);
     int L = 1;
     while (L < 64 && 2 * L * m.N <= GS) L *= 2;
     const int32_t nt = (m.M + kTileChannels - 1) / kTileChannels;
@@ -730,12 +779,13 @@ __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c,
             }
         }
         G.sync();
-        {
-            const int32_t u0 = max(cj0, t0), u1 = min(cj1, t1);
-            if (u0 < u1)
-                for (int32_t w = u0 >> 6; w <= (u1 - 1) >> 6; ++w)
-                    for (uint64_t bits = crit_word(critq, w, u0, u1); bits; bits &= bits - 1ull)
-                        chunk_crit += (double)A[64 * w + (int32_t)__builtin_ctzll(bits) - aoff];
+        // critical propensity of each 64-channel word of the tile (for the
+        // critical search), one wavefront per word
+        for (int32_t w = (t0 >> 6) + G.wid; w < (t1 + 63) >> 6; w += Grp<GS>::NW) {
+            const int32_t j = 64 * w + G.wl;
+            const double v = (j < t1 && crit_bit(critq, j)) ? (double)A[j - aoff] : 0.0;
+            const double tot = wave_sum(v);
+            if (G.wl == 0) wsum[w] = tot;
         }
         for (int32_t i0 = 0; i0 < m.N; i0 += rows_per_pass) {
             const int32_t i = i0 + lane / L;
@@ -891,7 +941,7 @@ __device__ __forceinline__ void grid_record(const ModelItem &mi, int64_t r, int 
 
 template <class R, int GS, bool SENS, class P>
 __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, int32_t *xs, int32_t *dx,
-                              int32_t *snap, uint64_t *critq, double *mu, double *sig, R *tile,
+                              int32_t *snap, uint64_t *critq, double *mu, double *sig, double *wsum, R *tile,
                               const P &pa) {
     const GModel m = global_model(mi.dm);
     const int lane = G.lane;
@@ -945,11 +995,11 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
         const double t_before = t;
         if (grid)
             for (int32_t q = lane; q < mi.n_track; q += GS) snap[q] = xs[as_global(mi.track)[q]];
-        double a0, a0c, chunk_crit = 0.0;
+        double a0, a0c;
         if constexpr (GS == 64)
             prop_pass<R, GS>(m, c, xs, pa, critq, mu, sig, lane, a0, a0c);
         else
-            prop_tau_tiled<R, GS>(m, c, xs, pa, tile, critq, mu, sig, G, a0, a0c, chunk_crit);
+            prop_tau_tiled<R, GS>(m, c, xs, pa, tile, critq, mu, sig, wsum, G, a0, a0c);
         G.sum2(a0, a0c);
         G.sync();
         if (!(a0 > 0.0)) {  // absorbing state
@@ -1052,8 +1102,11 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
             int32_t jc = m.M;
             double tau2 = (double)kFltMax;
             if (a0c > 0.0) {
-                const int32_t f = (GS == 64) ? G.search(pa, critq, true, m.M, a0c * r1)
-                                             : G.search_critical(pa, critq, m.M, chunk_crit, a0c * r1);
+                int32_t f;
+                if constexpr (GS == 64)
+                    f = G.search(pa, critq, true, m.M, a0c * r1);
+                else
+                    f = G.search_critical(pa, critq, m.M, wsum, a0c * r1);
                 if (f >= 0) {
                     jc = f;
                     tau2 = -log(r2) / a0c;
@@ -1200,6 +1253,7 @@ __global__ __launch_bounds__(BS, WPE) void k_persistent(PersistentLaunch L) {
     double *mu = reinterpret_cast<double *>(base + lay.off_mu);
     double *sig = reinterpret_cast<double *>(base + lay.off_sig);
     R *tile = reinterpret_cast<R *>(base + lay.off_tile);
+    double *wsum = reinterpret_cast<double *>(base + lay.off_wsum);
 
     for (;;) {
         unsigned long long v = 0;
@@ -1215,11 +1269,11 @@ __global__ __launch_bounds__(BS, WPE) void k_persistent(PersistentLaunch L) {
         const ModelItem mi = ld_uniform(&L.items[uni(lo)]);
         if constexpr (ALDS) {
             const PropStored<R> pa{reinterpret_cast<R *>(base + lay.off_a)};
-            simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, critq, mu, sig, tile, pa);
+            simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, critq, mu, sig, wsum, tile, pa);
         } else {
             const GModel gm = global_model(mi.dm);
             const PropRecomputed<R> pa{gm.rs, gm.template rates<R>(), xs};
-            simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, critq, mu, sig, tile, pa);
+            simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, critq, mu, sig, wsum, tile, pa);
         }
     }
 }
