@@ -773,20 +773,22 @@ __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c,
             }
             const unsigned long long bala = __ballot(cra), balb = __ballot(crb);
             const int32_t ta = j0 + (lane & ~63), tb = ta + GS;
+            // each wavefront holds whole 64-channel words: their critical
+            // propensity sums (for the critical search) by wave reduction
+            const double wa = (bala != 0ull) ? wave_sum(cra ? (double)(ja < t1 ? A[ja - aoff] : (R)0) : 0.0) : 0.0;
+            const double wb = (balb != 0ull) ? wave_sum(crb ? (double)(jb < t1 ? A[jb - aoff] : (R)0) : 0.0) : 0.0;
             if ((lane & 63) == 0) {
-                if (ta < t1) critq[ta >> 6] = bala;
-                if (tb < t1) critq[tb >> 6] = balb;
+                if (ta < t1) {
+                    critq[ta >> 6] = bala;
+                    wsum[ta >> 6] = wa;
+                }
+                if (tb < t1) {
+                    critq[tb >> 6] = balb;
+                    wsum[tb >> 6] = wb;
+                }
             }
         }
         G.sync();
-        // critical propensity of each 64-channel word of the tile (for the
-        // critical search), one wavefront per word
-        for (int32_t w = (t0 >> 6) + G.wid; w < (t1 + 63) >> 6; w += Grp<GS>::NW) {
-            const int32_t j = 64 * w + G.wl;
-            const double v = (j < t1 && crit_bit(critq, j)) ? (double)A[j - aoff] : 0.0;
-            const double tot = wave_sum(v);
-            if (G.wl == 0) wsum[w] = tot;
-        }
         for (int32_t i0 = 0; i0 < m.N; i0 += rows_per_pass) {
             const int32_t i = i0 + lane / L;
             double pm = 0.0, ps = 0.0;
