@@ -753,30 +753,33 @@ __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c,
                 cb = c[jb];
             }
             bool cra = false, crb = false;
+            R aa = (R)0, ab = (R)0;
+            // the tile buffer (recomputed propensities) keeps only what the
+            // gather needs: a_j of non-critical channels, 0 for critical ones
             if (ja < t1) {
                 int order;
                 int32_t s0, s1;
                 chan_unpack(ra, order, s0, s1);
-                const R aj = propensity<R>(order, ca, xs[s0], xs[s1], cra);
-                A[ja - aoff] = aj;
-                s += (double)aj;
-                if (cra) sc += (double)aj;
+                aa = propensity<R>(order, ca, xs[s0], xs[s1], cra);
+                A[ja - aoff] = (P::kStored || !cra) ? aa : (R)0;
+                s += (double)aa;
+                if (cra) sc += (double)aa;
             }
             if (jb < t1) {
                 int order;
                 int32_t s0, s1;
                 chan_unpack(rb, order, s0, s1);
-                const R aj = propensity<R>(order, cb, xs[s0], xs[s1], crb);
-                A[jb - aoff] = aj;
-                s += (double)aj;
-                if (crb) sc += (double)aj;
+                ab = propensity<R>(order, cb, xs[s0], xs[s1], crb);
+                A[jb - aoff] = (P::kStored || !crb) ? ab : (R)0;
+                s += (double)ab;
+                if (crb) sc += (double)ab;
             }
             const unsigned long long bala = __ballot(cra), balb = __ballot(crb);
             const int32_t ta = j0 + (lane & ~63), tb = ta + GS;
             // each wavefront holds whole 64-channel words: their critical
             // propensity sums (for the critical search) by wave reduction
-            const double wa = (bala != 0ull) ? wave_sum(cra ? (double)(ja < t1 ? A[ja - aoff] : (R)0) : 0.0) : 0.0;
-            const double wb = (balb != 0ull) ? wave_sum(crb ? (double)(jb < t1 ? A[jb - aoff] : (R)0) : 0.0) : 0.0;
+            const double wa = (bala != 0ull) ? wave_sum(cra ? (double)aa : 0.0) : 0.0;
+            const double wb = (balb != 0ull) ? wave_sum(crb ? (double)ab : 0.0) : 0.0;
             if ((lane & 63) == 0) {
                 if (ta < t1) {
                     critq[ta >> 6] = bala;
@@ -798,7 +801,7 @@ __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c,
                 // additions keep the lane's entry order
                 auto add = [&](int32_t p) {
                     const int32_t j = entry_index(p);
-                    if (!crit_bit(critq, j)) {
+                    if (P::kStored ? !crit_bit(critq, j) : true) {  // the tile holds 0 for critical channels
                         const double v = (double)entry_value(p);
                         const double tmp = v * (double)A[j - aoff];
                         pm += tmp;
