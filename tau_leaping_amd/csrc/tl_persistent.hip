@@ -797,7 +797,7 @@ __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c,
             double pm = 0.0, ps = 0.0;
             if (i < m.N) {
                 const int32_t e0 = m.tile_ptr[(size_t)i * (nt + 1) + t], e1 = m.tile_ptr[(size_t)i * (nt + 1) + t + 1];
-                // four entries per round with their loads issued together; the
+                // eight entries per round with their loads issued together; the
                 // additions keep the lane's entry order
                 auto add = [&](int32_t p) {
                     const int32_t j = entry_index(p);
@@ -809,12 +809,12 @@ __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c,
                     }
                 };
                 int32_t e = e0 + k;
-                for (; e + 3 * L < e1; e += 4 * L) {
-                    const int32_t p0 = m.csr[e], p1 = m.csr[e + L], p2 = m.csr[e + 2 * L], p3 = m.csr[e + 3 * L];
-                    add(p0);
-                    add(p1);
-                    add(p2);
-                    add(p3);
+                for (; e + 7 * L < e1; e += 8 * L) {
+                    int32_t p[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) p[u] = m.csr[e + u * L];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) add(p[u]);
                 }
                 for (; e < e1; e += L) add(m.csr[e]);
             }
