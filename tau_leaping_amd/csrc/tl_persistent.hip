@@ -890,7 +890,9 @@ __device__ __forceinline__ double score_term(bool cr, R aj, R tr, double tau, in
 // channels, so the Philox block holding their first draws (channels 4b..4b+3
 // share block b) is its own: no staging, no barrier inside the pass.  The
 // caller synchronizes before reading what f wrote.
-template <class R, int GS, class P, class F>
+// CRIT_A: f needs a_j of critical channels too (else they get 0 and a
+// recomputed propensity is not evaluated for them).
+template <class R, int GS, bool CRIT_A, class P, class F>
 __device__ __forceinline__ void for_each_firing(const GModel &m, const Grp<GS> &G, const P &pa, const uint64_t *critq,
                                                 uint2 key, uint32_t epoch, uint32_t c3, R tr, int32_t jfire, F &&f) {
     const uint32_t ck = c3 | kKindPoisson;
@@ -901,8 +903,10 @@ __device__ __forceinline__ void for_each_firing(const GModel &m, const Grp<GS> &
         // recomputed propensities: the four first (their loads overlap)
         R a4[4] = {(R)0, (R)0, (R)0, (R)0};
         if constexpr (!P::kStored) {
+            const uint32_t cw = (uint32_t)(critq[jb >> 6] >> (jb & 63)) & 15u;  // jb is a multiple of 4
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a4[i] = (jb + i < m.M) ? pa.get(jb + i) : (R)0;
+            for (int i = 0; i < 4; ++i)
+                a4[i] = (jb + i < m.M && (CRIT_A || !((cw >> i) & 1u))) ? pa.get(jb + i) : (R)0;
         }
 #pragma unroll 1
         for (int i = 0; i < 4; ++i) {
@@ -1128,7 +1132,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
             const int32_t jfire = fire ? jc : -1;
             const R tr = (R)tau;
             double qpart = 0.0;
-            for_each_firing<R, GS>(m, G, pa, critq, key, epoch, c3, tr, jfire,
+            for_each_firing<R, GS, SENS>(m, G, pa, critq, key, epoch, c3, tr, jfire,
                                    [&](int32_t j, bool cr, R aj, int32_t kj) {
                                        if (kj) {
                                            if (m.colq) {
@@ -1155,7 +1159,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
             if (neg) {
                 for (int32_t i = lane; i < m.N; i += GS) dx[i] = 0;
                 if (SENS)  // take back this attempt's score terms (same draws)
-                    for_each_firing<R, GS>(m, G, pa, critq, key, epoch, c3, tr, jfire,
+                    for_each_firing<R, GS, SENS>(m, G, pa, critq, key, epoch, c3, tr, jfire,
                                            [&](int32_t j, bool cr, R aj, int32_t kj) {
                                                if (c[j] != (R)0) S[j] -= score_term<R>(cr, aj, tr, tau, kj) / (double)c[j];
                                            });
