@@ -469,8 +469,10 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
     const int wpe64_64[] = {5, 4}, wpe64_32[] = {5, 6, 4};
     for (auto &sh : shapes)
         for (int al = 1; al >= 0; --al) {
-            const int *wl = sh[1] == 64 ? (is64 ? wpe64_64 : wpe64_32) : nullptr;
-            const int nw = sh[1] == 64 ? (is64 ? 2 : 3) : 1;
+            // 512-lane groups: also compiled for two blocks per CU
+            const int wpe512[] = {4, 1};
+            const int *wl = sh[1] == 64 ? (is64 ? wpe64_64 : wpe64_32) : (sh[1] == 512 ? wpe512 : nullptr);
+            const int nw = sh[1] == 64 ? (is64 ? 2 : 3) : (sh[1] == 512 ? 2 : 1);
             for (int w = 0; w < nw; ++w) {
                 PersistentConfig c{sh[0], sh[1], al != 0, wl ? wl[w] : 1};
                 const size_t lds = persistent_lds_bytes(c, is64, n_max, m_max);
@@ -483,9 +485,12 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
 }
 
 // Default shape without timing: one wavefront per realization up to 1024
-// channels, then a 256-lane block per realization (a 128 x 256 toy network
-// runs 3x faster on one wavefront; a 960 x 3800 one 2x faster on a block),
-// and 1024 lanes from 8192 channels; propensities in LDS when they fit.
+// channels, then 512-lane blocks two per CU, and 1024 lanes from 8192
+// channels; propensities in LDS when they fit.  Measured: a 128 x 256 toy
+// network runs 3x faster on one wavefront than on blocks; a 960 x 3800 one
+// 2.6x faster on 512-lane blocks (two per CU) than on one wavefront; the
+// 250-species colloid (31,250 channels) 1.8x faster on 1024-lane blocks than
+// on 512.
 PersistentConfig heuristic(const std::vector<PersistentConfig> &cands, int32_t m_max) {
     auto pick = [&](int b, int g) -> const PersistentConfig * {
         for (auto &c : cands)
@@ -495,6 +500,7 @@ PersistentConfig heuristic(const std::vector<PersistentConfig> &cands, int32_t m
     const PersistentConfig *c = nullptr;
     if (m_max <= 1024) c = pick(64, 64);
     if (!c && m_max <= 1024) c = pick(256, 64);
+    if (!c && m_max <= 8192) c = pick(512, 512);
     if (!c && m_max <= 8192) c = pick(256, 256);
     if (!c) c = pick(1024, 1024);
     if (!c) c = pick(512, 512);

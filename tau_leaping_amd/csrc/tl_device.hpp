@@ -24,7 +24,7 @@ constexpr int kPoissonSwitch = 80;     // Knuth below, normal approximation at o
 constexpr int kKnuthCap = 300;         // Knuth returns 300 after 299 draws
 constexpr int kMaxRetries = 32;        // rejected leaps in a row before giving up
 constexpr float kFltMax = 3.402823466e+38f;
-constexpr int32_t kTileChannels = 8192;  // channels per propensity tile of the block-group tau pass
+constexpr int32_t kTileGrain = 4096;  // channels per tile_ptr column; block-group tau tiles are 1 or 2 grains
 
 constexpr uint32_t kKindPoisson = 0u;  // ctr[3] low bits: per-channel firing stream (draws 2, 3, ...)
 constexpr uint32_t kKindLeap = 1u;     //                  leap scalars (r1, r2)
@@ -60,7 +60,7 @@ struct DevModel {
     const int2 *aff_rs;      // [aff_ptr[M]] rs of each listed channel (loads in parallel with aff)
     const double *aff_c64;   // [aff_ptr[M]] its rate
     const float *aff_c32;
-    const int32_t *tile_ptr; // [N][n_tiles + 1] CSR offset of each species row at channel t * kTileChannels
+    const int32_t *tile_ptr; // [N][n_grains + 1] CSR offset of each species row at channel g * kTileGrain
 };
 
 // ---- Philox4x32-10 ----

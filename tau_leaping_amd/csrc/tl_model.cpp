@@ -517,7 +517,7 @@ tl_status build_model(tl_model &m, std::string &msg) {
     }
     // row offsets at propensity-tile boundaries (rows are sorted by channel)
     {
-        constexpr int32_t kTile = 8192;
+        constexpr int32_t kTile = 4096;
         const int32_t nt = std::max(1, (m.M + kTile - 1) / kTile);
         m.tile_ptr.assign((size_t)m.N * (nt + 1), 0);
         for (int32_t i = 0; i < m.N; ++i) {

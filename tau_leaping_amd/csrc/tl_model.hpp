@@ -62,8 +62,8 @@ struct tl_model {
     // propensities one firing of j can change); empty when it would exceed
     // kMaxAffected entries
     std::vector<int32_t> aff_ptr, aff;
-    // [N][n_tiles + 1]: first CSR entry of row i whose channel is >= t * 8192
-    // (n_tiles = ceil(M / 8192); entry n_tiles is the row end)
+    // [N][n_grains + 1]: first CSR entry of row i whose channel is >= g * 4096
+    // (n_grains = ceil(M / 4096); entry n_grains is the row end)
     std::vector<int32_t> tile_ptr;
 
     uint64_t version = 1;  // bumped by set_rates / set_x0; invalidates device copies

@@ -109,6 +109,10 @@ __device__ __forceinline__ GModel global_model(const DevModel &d) {
     return g;
 }
 
+// Channels per propensity tile of a block group: two grains for 1024-lane
+// groups, one (a smaller LDS footprint, two blocks per CU) below.
+__host__ __device__ constexpr int32_t tile_channels(int gs) { return (gs >= 1024 ? 2 : 1) * kTileGrain; }
+
 // LDS layout of one group.  Species sums mu[N], sig[N] of the tau bound;
 // block groups without a[M] in LDS also hold one propensity tile.
 struct GroupLayout {
@@ -128,7 +132,8 @@ __host__ __device__ inline GroupLayout group_layout(int32_t n_max, int32_t m_max
     L.off_wsum = L.off_sig + al16((size_t)n_max * 8);
     L.off_tile = L.off_wsum + (gs > 64 ? al16((size_t)((m_max + 63) / 64) * 8) : 0);
     const bool tile = gs > 64 && !a_lds;
-    L.off_a = L.off_tile + (tile ? al16((size_t)(m_max < kTileChannels ? m_max : kTileChannels) * real_bytes) : 0);
+    const int32_t tile_ch = tile_channels(gs);
+    L.off_a = L.off_tile + (tile ? al16((size_t)(m_max < tile_ch ? m_max : tile_ch) * real_bytes) : 0);
     L.bytes = L.off_a + (a_lds ? al16((size_t)m_max * real_bytes) : 0);
     return L;
 }
@@ -710,7 +715,7 @@ __device__ __forceinline__ void prop_pass(const GModel &m, gptr<const R> c, cons
 
 // Block groups: propensities, criticality bits and lane partial sums of
 // a0 / a0c, then the species sums gathered along the CSR rows, one tile of
-// kTileChannels channels at a time (the tile's propensities sit in LDS: the
+// tile_channels(GS) channels at a time (the tile's propensities sit in LDS: the
 // whole a[M] when it fits, else a tile buffer).  A species row belongs to L
 // consecutive lanes (L a power of two, about GS / N) that stride its entries
 // in the tile and combine by butterfly; the row's first lane adds the tile's
@@ -724,12 +729,15 @@ __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c,
     sc = 0.0;
     int L = 1;
     while (L < 64 && 2 * L * m.N <= GS) L *= 2;
-    const int32_t nt = (m.M + kTileChannels - 1) / kTileChannels;
+    constexpr int32_t TILE = tile_channels(GS), TG = TILE / kTileGrain;
+    const int32_t ng = (m.M + kTileGrain - 1) / kTileGrain;  // tile_ptr columns - 1
+    const int32_t nt = (m.M + TILE - 1) / TILE;
     const int32_t rows_per_pass = GS / L;
     const int k = lane & (L - 1);
     for (int32_t t = 0; t < nt; ++t) {
-        const int32_t t0 = t * kTileChannels;
-        const int32_t t1 = min(m.M, t0 + kTileChannels);
+        const int32_t t0 = t * TILE;
+        const int32_t t1 = min(m.M, t0 + TILE);
+        const int32_t g0 = t * TG, g1 = min(ng, g0 + TG);
         R *A;  // A[j - aoff] holds a_j for j in the tile
         int32_t aoff;
         if constexpr (P::kStored) {
@@ -796,7 +804,7 @@ __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c,
             const int32_t i = i0 + lane / L;
             double pm = 0.0, ps = 0.0;
             if (i < m.N) {
-                const int32_t e0 = m.tile_ptr[(size_t)i * (nt + 1) + t], e1 = m.tile_ptr[(size_t)i * (nt + 1) + t + 1];
+                const int32_t e0 = m.tile_ptr[(size_t)i * (ng + 1) + g0], e1 = m.tile_ptr[(size_t)i * (ng + 1) + g1];
                 // eight entries per round with their loads issued together; the
                 // additions keep the lane's entry order
                 auto add = [&](int32_t p) {
@@ -1304,6 +1312,7 @@ const void *kernel_ptr(int bs, int gs, bool alds, int wpe) {
     }
     TL_K(256, 256, 1)
     TL_K(512, 512, 1)
+    TL_K(512, 512, 4)
     TL_K(1024, 1024, 1)
 #undef TL_K
     return nullptr;
