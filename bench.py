@@ -177,6 +177,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--realizations", type=int, default=100000)
     ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--workload", default="toy", choices=["toy", "colloid"],
+                    help="toy: 128 x 256 toy network, 100,000 realizations, t_final 0.1 (the default line); "
+                         "colloid: 250-species colloid (31,250 channels), 4,096 realizations, t_final 0.05")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=4000)
@@ -201,9 +204,13 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    net = models.toy_network(128, 256, seed=42)
+    if args.workload == "colloid":
+        net = models.colloidal(250, seed=44)
+        R, tf = (args.realizations if args.realizations != 100000 else 4096), 0.05
+    else:
+        net = models.toy_network(128, 256, seed=42)
+        R, tf = args.realizations, 0.1
     m = T.Model.from_network(net)
-    R, tf = args.realizations, 0.1
     fp64 = args.precision == "fp64"
     kw = dict(seed=1234, precision=args.precision, track=np.arange(4, dtype=np.int32))
 
@@ -224,8 +231,12 @@ def main():
     B = bytes_per_leap(net.n_species, net.n_channels, fp64)
     avg_ms = float(np.mean(kernel_ms))
     achieved = B * (steps_total / len(res)) / (avg_ms * 1e-3) / 1e9
-    workload = (f"toy network 128 species x 256 channels, {R} realizations, t_final 0.1, eps 0.03, "
-                f"{args.precision}, persistent engine")
+    if args.workload == "colloid":
+        workload = (f"colloid 250 species x {net.n_channels} channels, {R} realizations, t_final 0.05, eps 0.03, "
+                    f"{args.precision}, persistent engine")
+    else:
+        workload = (f"toy network 128 species x 256 channels, {R} realizations, t_final 0.1, eps 0.03, "
+                    f"{args.precision}, persistent engine")
     line = {
         "metric": METRIC,
         "value": value,
@@ -246,9 +257,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(workload),
                      "kernel": "k_persistent", "kernel_ms": avg_ms, "bytes_per_leap": B},
     }
-    if not args.no_extras:
+    if not args.no_extras and args.workload == "toy":
         line["extras"] = extras_run(T, models, np, m, dict(n_realizations=R, kw=kw))
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload == "toy":
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(net, args.cpu_sample, tf, 1234)
     print(json.dumps(line), flush=True)

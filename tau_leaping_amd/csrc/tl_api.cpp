@@ -485,12 +485,13 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
 }
 
 // Default shape without timing: one wavefront per realization up to 1024
-// channels, then 512-lane blocks two per CU, and 1024 lanes from 8192
-// channels; propensities in LDS when they fit.  Measured: a 128 x 256 toy
-// network runs 3x faster on one wavefront than on blocks; a 960 x 3800 one
-// 2.6x faster on 512-lane blocks (two per CU) than on one wavefront; the
-// 250-species colloid (31,250 channels) 1.8x faster on 1024-lane blocks than
-// on 512.
+// channels, then 256-lane blocks, and 1024 lanes from 8192 channels;
+// propensities in LDS when they fit.  Measured: a 128 x 256 toy network runs
+// 3x faster on one wavefront than on blocks; a 960 x 3800 one 2x faster on
+// 256-lane blocks than on one wavefront (2.6x on 512-lane blocks two per CU,
+// but those lose on batches of such models with many direct-method events);
+// the 250-species colloid (31,250 channels) 1.8x faster on 1024-lane blocks
+// than on 512.
 PersistentConfig heuristic(const std::vector<PersistentConfig> &cands, int32_t m_max) {
     auto pick = [&](int b, int g) -> const PersistentConfig * {
         for (auto &c : cands)
@@ -500,7 +501,6 @@ PersistentConfig heuristic(const std::vector<PersistentConfig> &cands, int32_t m
     const PersistentConfig *c = nullptr;
     if (m_max <= 1024) c = pick(64, 64);
     if (!c && m_max <= 1024) c = pick(256, 64);
-    if (!c && m_max <= 8192) c = pick(512, 512);
     if (!c && m_max <= 8192) c = pick(256, 256);
     if (!c) c = pick(1024, 1024);
     if (!c) c = pick(512, 512);
