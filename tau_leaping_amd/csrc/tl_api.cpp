@@ -475,7 +475,15 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
             const int nw = sh[1] == 64 ? (is64 ? 2 : 3) : (sh[1] == 512 ? 2 : 1);
             for (int w = 0; w < nw; ++w) {
                 PersistentConfig c{sh[0], sh[1], al != 0, wl ? wl[w] : 1};
-                const size_t lds = persistent_lds_bytes(c, is64, n_max, m_max);
+                // 1024-lane groups take the largest propensity tile (4, 2 or
+                // 1 grains of 4096 channels) that fits: fewer tiles, fewer
+                // barriers; smaller groups keep one grain (two blocks per CU)
+                c.tile_grains = (sh[1] >= 1024 && al == 0) ? 4 : 1;
+                size_t lds = persistent_lds_bytes(c, is64, n_max, m_max);
+                while (lds > dc.lds_per_block && c.tile_grains > 1) {
+                    c.tile_grains /= 2;
+                    lds = persistent_lds_bytes(c, is64, n_max, m_max);
+                }
                 if (lds > dc.lds_per_block) continue;
                 if (persistent_occupancy(c, is64, sens, lds) <= 0) continue;
                 out.push_back(c);
@@ -528,6 +536,7 @@ PersistentLaunch make_launch(DeviceContext &dc, const PersistentConfig &cfg, boo
     L.slots = grid * ng;
     L.grid = (int)grid;
     L.lds_bytes = lds;
+    L.tile_grains = cfg.tile_grains;
     return L;
 }
 
@@ -645,7 +654,8 @@ tl_status autotune(DeviceContext &dc, bool is64, bool sens, const std::vector<Mo
                 left -= take;
             }
             const double rate = ms > 0 ? (double)done / ms : 0.0;
-            if (c.block == first.block && c.group == first.group && c.a_lds == first.a_lds && c.wpe == first.wpe)
+            if (c.block == first.block && c.group == first.group && c.a_lds == first.a_lds && c.wpe == first.wpe &&
+                c.tile_grains == first.tile_grains)
                 first_rate = rate;
             if (rate > best_rate) {
                 best_rate = rate;

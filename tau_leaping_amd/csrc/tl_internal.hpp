@@ -118,6 +118,7 @@ struct PersistentConfig {
     int group;     // lanes per realization (64 = one wave, == block = whole block)
     bool a_lds;    // propensities in LDS (else recomputed where they are read)
     int wpe;       // waves per SIMD the kernel is compiled for (register budget 512 / wpe)
+    int tile_grains = 1;  // block groups without a in LDS: propensity tile of tile_grains * 4096 channels
 };
 
 struct PersistentLaunch {
@@ -129,6 +130,7 @@ struct PersistentLaunch {
     int64_t slots;
     int grid;
     size_t lds_bytes;  // per block
+    int tile_grains;   // see PersistentConfig
 };
 
 // LDS bytes per block for a config; 0 if it can not fit.

@@ -109,17 +109,13 @@ __device__ __forceinline__ GModel global_model(const DevModel &d) {
     return g;
 }
 
-// Channels per propensity tile of a block group: two grains for 1024-lane
-// groups, one (a smaller LDS footprint, two blocks per CU) below.
-__host__ __device__ constexpr int32_t tile_channels(int gs) { return (gs >= 1024 ? 2 : 1) * kTileGrain; }
-
 // LDS layout of one group.  Species sums mu[N], sig[N] of the tau bound;
 // block groups without a[M] in LDS also hold one propensity tile.
 struct GroupLayout {
     size_t off_x, off_dx, off_snap, off_crit, off_red, off_mu, off_sig, off_wsum, off_tile, off_a, bytes;
 };
 __host__ __device__ inline GroupLayout group_layout(int32_t n_max, int32_t m_max, int gs, bool a_lds,
-                                                    size_t real_bytes) {
+                                                    size_t real_bytes, int tile_grains) {
     GroupLayout L;
     const int nw = gs / 64;
     L.off_x = 0;
@@ -132,7 +128,7 @@ __host__ __device__ inline GroupLayout group_layout(int32_t n_max, int32_t m_max
     L.off_wsum = L.off_sig + al16((size_t)n_max * 8);
     L.off_tile = L.off_wsum + (gs > 64 ? al16((size_t)((m_max + 63) / 64) * 8) : 0);
     const bool tile = gs > 64 && !a_lds;
-    const int32_t tile_ch = tile_channels(gs);
+    const int32_t tile_ch = tile_grains * kTileGrain;
     L.off_a = L.off_tile + (tile ? al16((size_t)(m_max < tile_ch ? m_max : tile_ch) * real_bytes) : 0);
     L.bytes = L.off_a + (a_lds ? al16((size_t)m_max * real_bytes) : 0);
     return L;
@@ -715,21 +711,21 @@ __device__ __forceinline__ void prop_pass(const GModel &m, gptr<const R> c, cons
 
 // Block groups: propensities, criticality bits and lane partial sums of
 // a0 / a0c, then the species sums gathered along the CSR rows, one tile of
-// tile_channels(GS) channels at a time (the tile's propensities sit in LDS: the
+// TILE channels at a time (tile_grains grains; the tile's propensities sit in LDS: the
 // whole a[M] when it fits, else a tile buffer).  A species row belongs to L
 // consecutive lanes (L a power of two, about GS / N) that stride its entries
 // in the tile and combine by butterfly; the row's first lane adds the tile's
 // part to mu[i], sig[i].  No atomics, and a fixed order of additions.
 template <class R, int GS, class P>
 __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c, const int32_t *xs, const P &pa,
-                                               R *tile, uint64_t *critq, double *mu, double *sig, double *wsum,
-                                               const Grp<GS> &G, double &s, double &sc) {
+                                               R *tile, int tile_grains, uint64_t *critq, double *mu, double *sig,
+                                               double *wsum, const Grp<GS> &G, double &s, double &sc) {
     const int lane = G.lane;
     s = 0.0;
     sc = 0.0;
     int L = 1;
     while (L < 64 && 2 * L * m.N <= GS) L *= 2;
-    constexpr int32_t TILE = tile_channels(GS), TG = TILE / kTileGrain;
+    const int32_t TG = tile_grains, TILE = TG * kTileGrain;
     const int32_t ng = (m.M + kTileGrain - 1) / kTileGrain;  // tile_ptr columns - 1
     const int32_t nt = (m.M + TILE - 1) / TILE;
     const int32_t rows_per_pass = GS / L;
@@ -958,7 +954,7 @@ __device__ __forceinline__ void grid_record(const ModelItem &mi, int64_t r, int 
 
 template <class R, int GS, bool SENS, class P>
 __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, int32_t *xs, int32_t *dx,
-                              int32_t *snap, uint64_t *critq, double *mu, double *sig, double *wsum, R *tile,
+                              int32_t *snap, uint64_t *critq, double *mu, double *sig, double *wsum, R *tile, int tile_grains,
                               const P &pa) {
     const GModel m = global_model(mi.dm);
     const int lane = G.lane;
@@ -1016,7 +1012,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
         if constexpr (GS == 64)
             prop_pass<R, GS>(m, c, xs, pa, critq, mu, sig, lane, a0, a0c);
         else
-            prop_tau_tiled<R, GS>(m, c, xs, pa, tile, critq, mu, sig, wsum, G, a0, a0c);
+            prop_tau_tiled<R, GS>(m, c, xs, pa, tile, tile_grains, critq, mu, sig, wsum, G, a0, a0c);
         G.sum2(a0, a0c);
         G.sync();
         if (!(a0 > 0.0)) {  // absorbing state
@@ -1254,7 +1250,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
 template <class R, int BS, int GS, bool ALDS, bool SENS, int WPE>
 __global__ __launch_bounds__(BS, WPE) void k_persistent(PersistentLaunch L) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const GroupLayout lay = group_layout(L.n_max, L.m_max, GS, ALDS, sizeof(R));
+    const GroupLayout lay = group_layout(L.n_max, L.m_max, GS, ALDS, sizeof(R), L.tile_grains);
     const int tid = threadIdx.x;
     const int g = tid / GS;
     Grp<GS> G;
@@ -1286,11 +1282,11 @@ __global__ __launch_bounds__(BS, WPE) void k_persistent(PersistentLaunch L) {
         const ModelItem mi = ld_uniform(&L.items[uni(lo)]);
         if constexpr (ALDS) {
             const PropStored<R> pa{reinterpret_cast<R *>(base + lay.off_a)};
-            simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, critq, mu, sig, wsum, tile, pa);
+            simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, critq, mu, sig, wsum, tile, L.tile_grains, pa);
         } else {
             const GModel gm = global_model(mi.dm);
             const PropRecomputed<R> pa{gm.rs, gm.template rates<R>(), xs};
-            simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, critq, mu, sig, wsum, tile, pa);
+            simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, critq, mu, sig, wsum, tile, L.tile_grains, pa);
         }
     }
 }
@@ -1367,7 +1363,7 @@ __global__ __launch_bounds__(256) void k_grad(int64_t Rn, int32_t T, int32_t M, 
 }  // namespace
 
 size_t persistent_lds_bytes(const PersistentConfig &c, bool is64, int32_t n_max, int32_t m_max) {
-    const GroupLayout L = group_layout(n_max, m_max, c.group, c.a_lds, is64 ? 8 : 4);
+    const GroupLayout L = group_layout(n_max, m_max, c.group, c.a_lds, is64 ? 8 : 4, c.tile_grains);
     return L.bytes * (size_t)(c.block / c.group);
 }
 
