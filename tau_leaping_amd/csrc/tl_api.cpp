@@ -448,7 +448,7 @@ struct Entry {
     // device outputs
     int32_t *d_track, *d_outx, *d_status;
     int64_t *d_steps, *d_ssa, *d_rej;
-    double *d_score, *d_dtime, *d_grid, *d_dmean, *d_dse, *d_mean;
+    double *d_score, *d_dtime, *d_tlast, *d_grid, *d_dmean, *d_dse, *d_mean;
 };
 
 struct TuneKey {
@@ -706,10 +706,11 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
         TL_TRY(B.alloc(e.d_ssa, R));
         TL_TRY(B.alloc(e.d_rej, R));
         TL_TRY(B.alloc(e.d_status, R));
-        e.d_score = e.d_dtime = e.d_grid = e.d_dmean = e.d_dse = e.d_mean = nullptr;
+        e.d_score = e.d_dtime = e.d_tlast = e.d_grid = e.d_dmean = e.d_dse = e.d_mean = nullptr;
         if (sens) {
             TL_TRY(B.alloc(e.d_score, (size_t)R * e.m->M));
             TL_TRY(B.alloc(e.d_dtime, (size_t)R * e.m->M));
+            TL_TRY(B.alloc(e.d_tlast, (size_t)R * e.m->M));
             TL_TRY(B.alloc(e.d_dmean, (size_t)T * e.m->M));
             TL_TRY(B.alloc(e.d_dse, (size_t)T * e.m->M));
             TL_TRY(B.alloc(e.d_mean, T));
@@ -772,6 +773,7 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
             mi.out_status = e.d_status;
             mi.score = e.d_score;
             mi.dtime = e.d_dtime;
+            mi.tlast = e.d_tlast;
             mi.n_intervals = e.p->n_intervals;
             mi.grid = e.d_grid;
             base += mi.R;

@@ -109,6 +109,7 @@ struct ModelItem {
     int32_t *out_status;   // [R]
     double *score;         // [R][M] (sensitivities) or null
     double *dtime;         // [R][M] dt/dc_j scratch (sensitivities) or null
+    double *tlast;         // [R][M] time up to which S_j holds its direct-event exposure (sensitivities) or null
     int32_t n_intervals;   // > 0: record tracked species on the grid k*t_final/n_intervals
     double *grid;          // [R][n_intervals+1][n_track]
 };

@@ -966,11 +966,17 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
     // Likelihood-ratio score S_j = dlog p(path)/dc_j, and D_j = dt/dc_j of the
     // current time (nonzero through tau1-limited leaps; used by the clamped
     // last leap and the truncated last direct interval).
-    gptr<double> S = nullptr, D = nullptr;
+    gptr<double> S = nullptr, D = nullptr, TL = nullptr;
     if (SENS) {
         S = as_global(mi.score + (size_t)r * m.M);
         D = as_global(mi.dtime + (size_t)r * m.M);
+        TL = as_global(mi.tlast + (size_t)r * m.M);
     }
+    // Direct-event exposure -a_j dt / c_j is added lazily by one-wavefront
+    // groups with propensities in LDS and affected-channel lists: a channel's
+    // exposure is flushed when its propensity changes and at the end of the
+    // block (O(affected channels) per event instead of O(M)).
+    const bool lazy = SENS && GS == 64 && P::kStored && m.aff_ptr != nullptr;
 
     for (int32_t i = lane; i < m.N; i += GS) {
         xs[i] = m.x0[i];
@@ -1051,6 +1057,12 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
             if (tau1 < kSsaFactor / a0) {
                 // direct branch: up to 100 exact events; after each event only
                 // the channels that read a changed species are recomputed
+                bool flushed = false;
+                if (lazy) {
+                    for (int32_t j = lane; j < m.M; j += GS) TL[j] = t;
+                    __threadfence_block();
+                    G.sync();
+                }
                 for (int s = 0; s < kSsaSteps; ++s) {
                     typename Grp<GS>::Scan sc;
                     const double a0s = G.scan_chunks(pa, critq, false, m.M, sc);
@@ -1074,7 +1086,9 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                         if (SENS) {  // survival over rem = tf - t: d/dc_j of -a0*rem
                             const double rem = tf - t;
                             for (int32_t j = lane; j < m.M; j += GS)
-                                if (c[j] != (R)0) S[j] += a0s * D[j] - (double)pa.get(j) * rem / (double)c[j];
+                                if (c[j] != (R)0)
+                                    S[j] += a0s * D[j] - (double)pa.get(j) * (lazy ? tf - TL[j] : rem) / (double)c[j];
+                            flushed = true;
                         }
                         t = tf;
                         break;
@@ -1087,9 +1101,22 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                         jf = G.max_int(last);
                     }
                     if (SENS) {
-                        for (int32_t j = lane; j < m.M; j += GS)
-                            if (c[j] != (R)0)
-                                S[j] += ((j == jf) ? 1.0 / (double)c[j] : 0.0) - (double)pa.get(j) * dt / (double)c[j];
+                        if (lazy) {
+                            // the event's own term, then the exposure of the channels
+                            // whose propensity this event changes, up to the event time
+                            if (lane == 0 && c[jf] != (R)0) S[jf] += 1.0 / (double)c[jf];
+                            const double te = t + dt;
+                            const int32_t d1 = m.aff_ptr_uniform(jf + 1);
+                            for (int32_t d = m.aff_ptr_uniform(jf) + lane; d < d1; d += GS) {
+                                const int32_t k = m.aff[d];
+                                if (c[k] != (R)0) S[k] -= (double)pa.get(k) * (te - TL[k]) / (double)c[k];
+                                TL[k] = te;
+                            }
+                        } else {
+                            for (int32_t j = lane; j < m.M; j += GS)
+                                if (c[j] != (R)0)
+                                    S[j] += ((j == jf) ? 1.0 / (double)c[j] : 0.0) - (double)pa.get(j) * dt / (double)c[j];
+                        }
                     }
                     if constexpr (SENS && !P::kStored) G.sync();
                     if (m.colq) {
@@ -1103,6 +1130,12 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                     prop_update<R, GS>(m, c, xs, pa, jf, lane);
                     t += dt;
                     nssa++;
+                    G.sync();
+                }
+                if (lazy && !flushed) {  // exposure of every channel up to the block's end
+                    __threadfence_block();
+                    for (int32_t j = lane; j < m.M; j += GS)
+                        if (c[j] != (R)0) S[j] -= (double)pa.get(j) * (t - TL[j]) / (double)c[j];
                     G.sync();
                 }
                 steps++;
