@@ -81,6 +81,21 @@ def test_batch_equals_single_runs(cuda):
         assert np.mean(np.all(b.final_x == p.final_x, axis=1)) >= 0.97
 
 
+def test_mixed_size_batch(cuda):
+    # models of very different sizes land in different size classes (one
+    # persistent launch each, one graph); each model's results agree with its
+    # single run to the oracle bar (a class may pick another launch shape)
+    nets = [models.toy_network(12, 30, seed=201), models.toy_network(400, 1400, seed=202),
+            models.toy_network(40, 100, seed=203), models.colloidal(30, seed=204)]
+    ms = [T.Model.from_network(n) for n in nets]
+    kws = [dict(t_final=0.02, n_realizations=400, seed=77 + i, model_id=i) for i in range(len(nets))]
+    batch = T.batch_simulate(ms, kws)
+    for m, kw, b in zip(ms, kws, batch):
+        single = m.simulate(**kw)
+        assert b.n_failed == 0
+        _compare(b.final_x, single.final_x)
+
+
 def test_conservation_laws(cuda):
     # cyclic chain: sum_i x_i constant; colloid: sum_i i * x_i constant
     cyc = models.cyclic_chain(256, seed=1)
