@@ -45,6 +45,22 @@ def test_trajectories_match_oracle(cuda, name, tf, prec, mode):
     assert res.total_steps == int(res.steps.sum())
 
 
+@pytest.mark.parametrize("bs", [256, 1024])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_block_groups_match_oracle(cuda, bs, prec):
+    # one realization per block: tiled species sums, word-level critical
+    # search (the colloid keeps ~40 % of its channels critical), direct
+    # events with block barriers (the larger toy network)
+    for net, tf, R in [(models.colloidal(60, seed=8), 0.05, 512), (models.toy_network(300, 1100, seed=12), 0.004, 256)]:
+        m, om = pair(net)
+        res = m.simulate(tf, R, seed=4321, precision=prec, block_size=bs)
+        ora = om.simulate(R, tf, seed=4321, precision=prec)
+        assert res.n_failed == 0 and (ora["status"] == 0).all()
+        assert res.block_size == bs
+        same = _compare(res.final_x, ora["x_final"])
+        assert np.array_equal(res.steps[same], ora["steps"][same])
+
+
 @pytest.mark.parametrize("bs", [256, 512, 1024])
 def test_launch_shapes_agree(cuda, bs):
     # one realization per wavefront (64) against one per block: the species
@@ -79,6 +95,19 @@ def test_batch_equals_single_runs(cuda):
     plain = T.batch_simulate(ms, kws, mode="plain")
     for b, p in zip(batch, plain):
         assert np.mean(np.all(b.final_x == p.final_x, axis=1)) >= 0.97
+
+
+def test_autotuned_run_agrees(cuda):
+    # enough realizations for the autotuner to time its candidates; the
+    # chosen shape must agree with the default one-wavefront launch
+    m = T.Model.from_network(models.toy_network(32, 64, seed=31))
+    tuned = m.simulate(0.02, 20000, seed=91)
+    ref = m.simulate(0.02, 20000, seed=91, block_size=64)
+    assert tuned.n_failed == 0
+    if tuned.block_size == 64:
+        assert np.array_equal(tuned.final_x, ref.final_x)
+    else:
+        _compare(tuned.final_x, ref.final_x)
 
 
 def test_mixed_size_batch(cuda):
