@@ -335,6 +335,16 @@ struct Grp {
     int lane, wl, wid;
     double *red;
 
+    // LDS-only exchange (see simulate_item)
+    __device__ __forceinline__ void lds_sync() const {
+        if constexpr (GS == 64) {
+            __builtin_amdgcn_wave_barrier();
+            __asm__ volatile("" ::: "memory");
+        } else {
+            __syncthreads();
+        }
+    }
+
     __device__ __forceinline__ void sync() const {
         if constexpr (GS == 64) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -957,6 +967,16 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                               int32_t *snap, uint64_t *critq, double *mu, double *sig, double *wsum, R *tile, int tile_grains,
                               const P &pa) {
     const GModel m = global_model(mi.dm);
+    // Group barrier.  Without sensitivities the lanes of a one-wavefront group
+    // only exchange data through LDS, whose operations a wavefront performs in
+    // order, so a code-motion barrier suffices (no wait on outstanding global
+    // loads); scores in global memory need the full fences.
+    auto sync_ = [&]() {
+        if constexpr (SENS)
+            G.sync();
+        else
+            G.lds_sync();
+    };
     const int lane = G.lane;
     const gptr<const R> c = m.template rates<R>();
     const uint2 key = rng_key(mi.seed, mi.rid_offset + r);
@@ -992,7 +1012,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
             S[j] = 0.0;
             D[j] = 0.0;
         }
-    G.sync();
+    sync_();
 
     double t = 0.0;
     uint32_t epoch = 0;
@@ -1020,7 +1040,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
         else
             prop_tau_tiled<R, GS>(m, c, xs, pa, tile, tile_grains, critq, mu, sig, wsum, G, a0, a0c);
         G.sum2(a0, a0c);
-        G.sync();
+        sync_();
         if (!(a0 > 0.0)) {  // absorbing state
             t = tf;
             steps++;
@@ -1126,11 +1146,11 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                         const int32_t e1 = m.col_ptr[jf + 1];
                         for (int32_t e = m.col_ptr[jf] + lane; e < e1; e += GS) column_add(xs, m.csc[e], 1);
                     }
-                    G.sync();
+                    sync_();
                     prop_update<R, GS>(m, c, xs, pa, jf, lane);
                     t += dt;
                     nssa++;
-                    G.sync();
+                    sync_();
                 }
                 if (lazy && !flushed) {  // exposure of every channel up to the block's end
                     __threadfence_block();
@@ -1189,7 +1209,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                                            if (c[j] != (R)0) S[j] += term / (double)c[j];
                                        }
                                    });
-            G.sync();  // dx complete
+            sync_();  // dx complete
             bool neg = false;
             for (int32_t i = lane; i < m.N; i += GS) neg |= (xs[i] + dx[i]) < 0;
             neg = G.any(neg);
@@ -1202,7 +1222,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                                            });
                 epoch++;
                 nrej++;
-                G.sync();
+                sync_();
                 if (++retries > kMaxRetries) {
                     status = kRetryExhausted;
                     break;
@@ -1252,7 +1272,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
             }
             t = tn;
             steps++;
-            G.sync();
+            sync_();
             break;
         }
         if (status != kOk) break;
@@ -1275,7 +1295,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
         as_global(mi.out_rej)[r] = nrej;
         as_global(mi.out_status)[r] = status;
     }
-    G.sync();
+    sync_();
 }
 
 // WPE: waves per SIMD the register allocation targets (more waves hide more
