@@ -467,8 +467,12 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
     // one-wavefront groups come in register budgets of 5 (the default), 4
     // and, for FP32, 6 waves per SIMD
     const int wpe64_64[] = {5, 4}, wpe64_32[] = {5, 6, 4};
+    // order within a shape = preference: a[M] in LDS first, except for
+    // 1024-lane groups, where recomputed propensities with a large tile win
+    // (the 31,250-channel colloid)
     for (auto &sh : shapes)
-        for (int al = 1; al >= 0; --al) {
+        for (int ai = 0; ai < 2; ++ai) {
+            const int al = (sh[1] >= 1024) ? ai : 1 - ai;
             // 512-lane groups: also compiled for two blocks per CU
             const int wpe512[] = {4, 1};
             const int *wl = sh[1] == 64 ? (is64 ? wpe64_64 : wpe64_32) : (sh[1] == 512 ? wpe512 : nullptr);
@@ -503,7 +507,7 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
 PersistentConfig heuristic(const std::vector<PersistentConfig> &cands, int32_t m_max) {
     auto pick = [&](int b, int g) -> const PersistentConfig * {
         for (auto &c : cands)
-            if (c.block == b && c.group == g) return &c;  // a_lds and 5-wave variants come first
+            if (c.block == b && c.group == g) return &c;  // preferred variants come first
         return nullptr;
     };
     const PersistentConfig *c = nullptr;
