@@ -256,7 +256,7 @@ class Model:
         return order, s0, s1, rp, csr[:Z]
 
     def params(self, t_final, n_realizations, eps=0.03, seed=1234, precision="fp64", track=None, n_intervals=0,
-               sensitivities=False, max_iter=0, mode="persistent", block_size=0, device=0, model_id=0,
+               sensitivities=False, max_iter=0, mode="auto", block_size=0, device=0, model_id=0,
                realization_offset=0):
         p = _Params()
         lib().tl_sim_params_default(C.byref(p))
@@ -281,7 +281,7 @@ class Model:
 
     def simulate(self, t_final, n_realizations, **kw) -> SimResult:
         return batch_simulate([self], [dict(t_final=t_final, n_realizations=n_realizations, **kw)],
-                              mode=kw.get("mode", "persistent"))[0]
+                              mode=kw.get("mode", "auto"))[0]
 
 
 def _result_buffers(model: Model, p: _Params, trk, want_final=True, want_score=False):
@@ -320,10 +320,12 @@ def _to_result(r: _Result, bufs, track) -> SimResult:
                      kernel_ms=r.kernel_ms)
 
 
-def batch_simulate(model_list: Sequence[Model], kwargs_list: Sequence[dict], mode="persistent",
+def batch_simulate(model_list: Sequence[Model], kwargs_list: Sequence[dict], mode="auto",
                    want_final=True) -> List[SimResult]:
     """Run many models; ``mode='persistent'`` is one work-stealing launch for
-    the whole batch, ``'plain'`` the per-model stage-kernel loop."""
+    the whole batch, ``'plain'`` the per-model stage-kernel loop, ``'auto'``
+    (default) persistent except for models whose state does not fit LDS,
+    which run plain (unless sensitivities or a time grid are asked for)."""
     n = len(model_list)
     params = (_Params * n)()
     keep, outs = [], []

@@ -52,6 +52,23 @@ struct Opt {
 
 int next_int(std::mt19937_64 &g, int lo, int hi) { return std::uniform_int_distribution<int>(lo, hi)(g); }
 
+// Removals from the unused list of the channel being drawn, (position, species),
+// so a rejected channel can put them back without copying the list.
+std::vector<std::pair<int, int>> undo_log;
+
+void undo_removals(std::vector<int> &unused) {
+    for (auto it = undo_log.rbegin(); it != undo_log.rend(); ++it) {
+        const int i = it->first, s = it->second;
+        if (i == (int)unused.size()) {
+            unused.push_back(s);
+        } else {
+            unused.push_back(unused[i]);
+            unused[i] = s;
+        }
+    }
+    undo_log.clear();
+}
+
 // A species not used yet if any remain, else any species; never one of `avoid`.
 int next_species(std::mt19937_64 &g, std::vector<int> &unused, int n, const std::set<int> &avoid) {
     for (int attempt = 0; attempt < 1000; ++attempt) {
@@ -64,6 +81,7 @@ int next_species(std::mt19937_64 &g, std::vector<int> &unused, int n, const std:
                 if (!any) goto fallback;
                 continue;
             }
+            undo_log.push_back({i, s});
             unused[i] = unused.back();
             unused.pop_back();
             return s;
@@ -97,7 +115,7 @@ std::vector<Rx> gen_normal(std::mt19937_64 &g, const Opt &o) {
             int tries = 0;
             for (;; ++tries) {
                 if (tries >= 100) die("There are not enough reactants to produce unique reactions");
-                std::vector<int> keep = unused;
+                undo_log.clear();
                 Rx r;
                 r.type = type;
                 std::set<int> used;
@@ -135,7 +153,7 @@ std::vector<Rx> gen_normal(std::mt19937_64 &g, const Opt &o) {
                     out.push_back(r);
                     break;
                 }
-                unused = keep;
+                undo_removals(unused);
             }
         }
     if (!unused.empty())
@@ -143,16 +161,25 @@ std::vector<Rx> gen_normal(std::mt19937_64 &g, const Opt &o) {
     return out;
 }
 
+// 1-2 distinct reactants and 1-2 distinct products, all different species:
+// the first nr + np species of a uniform random order, drawn one at a time by
+// rejection (O(1) per channel, so 1e6 channels over 1e5 species stay fast).
 std::vector<Rx> gen_toy(std::mt19937_64 &g, const Opt &o) {
     std::vector<Rx> out;
+    out.reserve(o.channels);
     for (int n = 0; n < o.channels; ++n) {
         Rx r;
-        std::vector<int> perm(o.species);
-        for (int i = 0; i < o.species; ++i) perm[i] = i + 1;
-        std::shuffle(perm.begin(), perm.end(), g);
         int nr = next_int(g, 1, 2), np = next_int(g, 1, 2);
-        for (int i = 0; i < nr; ++i) r.reac.push_back({perm[i], 1});
-        for (int i = 0; i < np; ++i) r.prod.push_back({perm[nr + i], 1});
+        int pick[4];
+        for (int i = 0; i < nr + np; ++i) {
+            int s;
+            do {
+                s = next_int(g, 1, o.species);
+            } while (std::find(pick, pick + i, s) != pick + i);
+            pick[i] = s;
+        }
+        for (int i = 0; i < nr; ++i) r.reac.push_back({pick[i], 1});
+        for (int i = 0; i < np; ++i) r.prod.push_back({pick[nr + i], 1});
         r.type = nr == 2 ? 2 : 1;
         out.push_back(r);
     }
@@ -212,6 +239,7 @@ std::string desc(const Rx &r) {
 }  // namespace
 
 int main(int argc, char **argv) {
+    std::ios::sync_with_stdio(false);
     Opt o;
     for (int i = 1; i < argc; ++i) {
         const std::string k = argv[i];
@@ -269,7 +297,9 @@ int main(int argc, char **argv) {
     }
     const int M = (int)rx.size();
     std::cerr << "generating " << M << " reactions over " << o.species << " species" << std::endl;
-    std::ostringstream x;
+    // written straight to stdout (1e6-channel models are about half a
+    // gigabyte of text); cout is unsynchronised from stdio, so it buffers
+    std::ostream &x = std::cout;
     x << "<!-- Stochkit 2.0 text input format\n\nreference:\nM. Rathinam, L.R. Petzold, Y. Cao, and D.T. "
          "Gillespie. J Chem Phys 119, 12784 (2003).\n-->\n\n";
     x << "<!-- generated by toy_system_gen (tau_leaping_amd)";
@@ -309,7 +339,7 @@ int main(int argc, char **argv) {
           << "</Description>\n\t\t\t<InitialPopulation>" << next_int(g, o.mnp, o.mxp)
           << "</InitialPopulation>\n\t\t</Species>\n";
     x << "\t</SpeciesList>\n</Model>\n";
-    std::cout << x.str();
+    x.flush();
     std::cerr << "Finished successfully" << std::endl;
     return EXIT_SUCCESS;
 }

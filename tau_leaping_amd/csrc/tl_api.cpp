@@ -909,9 +909,26 @@ void tl_sim_params_default(tl_sim_params *p) {
     p->mode = TL_MODE_AUTO;
 }
 
+// AUTO mode: whether a model runs in the plain form because its state fits
+// no persistent launch shape (and the call needs nothing only the persistent
+// form provides).
+static tl_status auto_plain(const tl_model *m, const tl_sim_params *p, bool &plain) {
+    plain = false;
+    if (p->sensitivities || p->n_intervals > 0) return TL_OK;
+    DeviceContext *dc;
+    TL_TRY(device_context(p->device, dc));
+    plain = candidates(*dc, p->precision == TL_FP64, false, m->N, m->M).empty();
+    return TL_OK;
+}
+
 tl_status tl_simulate(const tl_model *m, const tl_sim_params *p, tl_result *r) {
     TL_TRY(check_params(m, p));
     if (p->mode == TL_MODE_PLAIN) return run_plain(m, p, r);
+    if (p->mode == TL_MODE_AUTO) {
+        bool plain;
+        TL_TRY(auto_plain(m, p, plain));
+        if (plain) return run_plain(m, p, r);
+    }
     return run_persistent(&m, 1, p, r);
 }
 
@@ -924,6 +941,35 @@ tl_status tl_batch_simulate(const tl_model *const *models, int32_t n_models, con
             TL_TRY(run_plain(models[i], &params[i], results ? &results[i] : nullptr));
         }
         return TL_OK;
+    }
+    if (mode == TL_MODE_AUTO) {
+        // models too large for LDS run one by one in the plain form, the rest
+        // in one persistent launch
+        std::vector<const tl_model *> pm;
+        std::vector<tl_sim_params> pp;
+        std::vector<int32_t> pi;
+        for (int32_t i = 0; i < n_models; ++i) {
+            TL_TRY(check_params(models[i], &params[i]));
+            bool plain;
+            TL_TRY(auto_plain(models[i], &params[i], plain));
+            if (plain) {
+                TL_TRY(run_plain(models[i], &params[i], results ? &results[i] : nullptr));
+            } else {
+                pm.push_back(models[i]);
+                pp.push_back(params[i]);
+                pi.push_back(i);
+            }
+        }
+        if (pm.empty()) return TL_OK;
+        if ((int32_t)pm.size() < n_models) {
+            std::vector<tl_result> pr(pm.size());
+            if (results)
+                for (size_t k = 0; k < pm.size(); ++k) pr[k] = results[pi[k]];
+            TL_TRY(run_persistent(pm.data(), (int32_t)pm.size(), pp.data(), results ? pr.data() : nullptr));
+            if (results)
+                for (size_t k = 0; k < pm.size(); ++k) results[pi[k]] = pr[k];
+            return TL_OK;
+        }
     }
     return run_persistent(models, n_models, params, results);
 }

@@ -1,9 +1,9 @@
-// tl_model.cpp -- StochKit XML reader, channel-major assembly and the derived
-// layouts of a model, plus the model part of the C ABI.
+// tl_model.cpp -- the derived layouts of a model (reaction orders, packed
+// species-major and channel-major maps, dependency and affected-channel
+// lists, tiles) and the model part of the C ABI.  The StochKit reader is in
+// tl_stochkit.cpp.
 //
-// Reference: StochKitReader::ReadXML (tau_leaping/StochKitReader.cpp:35-179),
-// ReadXMLToMatrixStruct (tau_leaping/CSCHostMatrixData.cpp:19-107),
-// OrderDetectorAndIndFiller (tau_leaping/CSCMatrix.cu:185-251),
+// Reference: OrderDetectorAndIndFiller (tau_leaping/CSCMatrix.cu:185-251),
 // ToCSR / CSRMatrixPacked ctor (CSCMatrix.cu:147-178, CSRMatrixPacked.cu:22-31).
 #include "tl_model.hpp"
 
@@ -19,346 +19,6 @@
 #include "tl_internal.hpp"
 
 namespace tl {
-
-// ---------------------------------------------------------------- XML ----
-// A small DOM for the StochKit subset: elements, attributes, character data,
-// comments, processing instructions and declarations (skipped).
-namespace {
-
-struct XNode {
-    std::string name;
-    std::vector<std::pair<std::string, std::string>> attrs;
-    std::string text;
-    std::vector<std::unique_ptr<XNode>> kids;
-
-    const XNode *child(const char *n) const {
-        for (auto &k : kids)
-            if (k->name == n) return k.get();
-        return nullptr;
-    }
-    const std::string *attr(const char *n) const {
-        for (auto &a : attrs)
-            if (a.first == n) return &a.second;
-        return nullptr;
-    }
-};
-
-std::string trim(const std::string &s) {
-    size_t b = 0, e = s.size();
-    while (b < e && isspace((unsigned char)s[b])) ++b;
-    while (e > b && isspace((unsigned char)s[e - 1])) --e;
-    return s.substr(b, e - b);
-}
-
-std::string decode_entities(const std::string &s) {
-    std::string out;
-    out.reserve(s.size());
-    for (size_t i = 0; i < s.size(); ++i) {
-        if (s[i] != '&') {
-            out += s[i];
-            continue;
-        }
-        size_t semi = s.find(';', i);
-        if (semi == std::string::npos) {
-            out += s[i];
-            continue;
-        }
-        std::string ent = s.substr(i + 1, semi - i - 1);
-        if (ent == "lt") out += '<';
-        else if (ent == "gt") out += '>';
-        else if (ent == "amp") out += '&';
-        else if (ent == "quot") out += '"';
-        else if (ent == "apos") out += '\'';
-        else if (!ent.empty() && ent[0] == '#') {
-            long v = (ent.size() > 1 && (ent[1] == 'x' || ent[1] == 'X')) ? strtol(ent.c_str() + 2, nullptr, 16)
-                                                                          : strtol(ent.c_str() + 1, nullptr, 10);
-            if (v > 0 && v < 128) out += (char)v;
-        } else {
-            out += s.substr(i, semi - i + 1);
-        }
-        i = semi;
-    }
-    return out;
-}
-
-class XmlParser {
-public:
-    XmlParser(const std::string &t) : s_(t) {}
-
-    bool parse(std::vector<std::unique_ptr<XNode>> &roots, std::string &err) {
-        while (true) {
-            skip_misc();
-            if (p_ >= s_.size()) return true;
-            if (s_[p_] != '<') {
-                err = "unexpected character data outside an element";
-                return false;
-            }
-            auto n = std::make_unique<XNode>();
-            if (!element(*n, err)) return false;
-            roots.push_back(std::move(n));
-        }
-    }
-
-private:
-    const std::string &s_;
-    size_t p_ = 0;
-
-    bool starts(const char *lit) const { return s_.compare(p_, strlen(lit), lit) == 0; }
-
-    // whitespace, comments, <? ?>, <! > declarations
-    void skip_misc() {
-        for (;;) {
-            while (p_ < s_.size() && isspace((unsigned char)s_[p_])) ++p_;
-            if (starts("<!--")) {
-                size_t e = s_.find("-->", p_ + 4);
-                p_ = (e == std::string::npos) ? s_.size() : e + 3;
-            } else if (starts("<?")) {
-                size_t e = s_.find("?>", p_ + 2);
-                p_ = (e == std::string::npos) ? s_.size() : e + 2;
-            } else if (starts("<!")) {
-                size_t e = s_.find('>', p_ + 2);
-                p_ = (e == std::string::npos) ? s_.size() : e + 1;
-            } else {
-                return;
-            }
-        }
-    }
-
-    bool name_tok(std::string &out) {
-        size_t b = p_;
-        while (p_ < s_.size() && !isspace((unsigned char)s_[p_]) && s_[p_] != '>' && s_[p_] != '/' && s_[p_] != '=')
-            ++p_;
-        out = s_.substr(b, p_ - b);
-        return !out.empty();
-    }
-
-    void ws() {
-        while (p_ < s_.size() && isspace((unsigned char)s_[p_])) ++p_;
-    }
-
-    bool element(XNode &n, std::string &err) {
-        ++p_;  // '<'
-        if (!name_tok(n.name)) {
-            err = "empty element name";
-            return false;
-        }
-        for (;;) {
-            ws();
-            if (p_ >= s_.size()) {
-                err = "unterminated tag <" + n.name;
-                return false;
-            }
-            if (s_[p_] == '/') {
-                if (p_ + 1 < s_.size() && s_[p_ + 1] == '>') {
-                    p_ += 2;
-                    return true;
-                }
-                err = "bad '/' in tag <" + n.name;
-                return false;
-            }
-            if (s_[p_] == '>') {
-                ++p_;
-                break;
-            }
-            std::string an;
-            if (!name_tok(an)) {
-                err = "bad attribute in <" + n.name;
-                return false;
-            }
-            ws();
-            if (p_ >= s_.size() || s_[p_] != '=') {
-                err = "attribute without value in <" + n.name;
-                return false;
-            }
-            ++p_;
-            ws();
-            if (p_ >= s_.size() || (s_[p_] != '"' && s_[p_] != '\'')) {
-                err = "unquoted attribute in <" + n.name;
-                return false;
-            }
-            char q = s_[p_++];
-            size_t e = s_.find(q, p_);
-            if (e == std::string::npos) {
-                err = "unterminated attribute in <" + n.name;
-                return false;
-            }
-            n.attrs.emplace_back(an, decode_entities(s_.substr(p_, e - p_)));
-            p_ = e + 1;
-        }
-        // content
-        std::string text;
-        for (;;) {
-            if (p_ >= s_.size()) {
-                err = "missing </" + n.name + ">";
-                return false;
-            }
-            if (starts("<!--")) {
-                size_t e = s_.find("-->", p_ + 4);
-                p_ = (e == std::string::npos) ? s_.size() : e + 3;
-                continue;
-            }
-            if (starts("<![CDATA[")) {
-                size_t e = s_.find("]]>", p_ + 9);
-                if (e == std::string::npos) {
-                    err = "unterminated CDATA";
-                    return false;
-                }
-                text += s_.substr(p_ + 9, e - p_ - 9);
-                p_ = e + 3;
-                continue;
-            }
-            if (starts("<?")) {
-                size_t e = s_.find("?>", p_ + 2);
-                p_ = (e == std::string::npos) ? s_.size() : e + 2;
-                continue;
-            }
-            if (starts("</")) {
-                p_ += 2;
-                std::string cn;
-                name_tok(cn);
-                ws();
-                if (cn != n.name || p_ >= s_.size() || s_[p_] != '>') {
-                    err = "mismatched </" + cn + "> for <" + n.name + ">";
-                    return false;
-                }
-                ++p_;
-                n.text = decode_entities(trim(text));
-                return true;
-            }
-            if (s_[p_] == '<') {
-                auto k = std::make_unique<XNode>();
-                if (!element(*k, err)) return false;
-                n.kids.push_back(std::move(k));
-                continue;
-            }
-            text += s_[p_++];
-        }
-    }
-};
-
-const XNode *need(const XNode *parent, const char *name, std::string &msg) {
-    const XNode *n = parent->child(name);
-    if (!n) msg = std::string("can not find ") + name + " node";
-    return n;
-}
-
-}  // namespace
-
-tl_status parse_stochkit(const std::string &text, StochKitDoc &doc, std::string &msg) {
-    std::vector<std::unique_ptr<XNode>> roots;
-    XmlParser xp(text);
-    if (!xp.parse(roots, msg)) return TL_ERR_PARSE;
-    const XNode *model = nullptr;
-    for (auto &r : roots)
-        if (r->name == "Model") {
-            model = r.get();
-            break;
-        }
-    if (!model) {
-        msg = "can not find Model node";
-        return TL_ERR_PARSE;
-    }
-    const XNode *n;
-    if (!(n = need(model, "NumberOfReactions", msg))) return TL_ERR_PARSE;
-    doc.n_reactions_declared = atoi(n->text.c_str());
-    if (!(n = need(model, "NumberOfSpecies", msg))) return TL_ERR_PARSE;
-    doc.n_species_declared = atoi(n->text.c_str());
-
-    // ParametersList / Parameter {Id, Expression}: numeric expressions only
-    const XNode *plist = need(model, "ParametersList", msg);
-    if (!plist) return TL_ERR_PARSE;
-    bool any = false;
-    for (auto &p : plist->kids) {
-        if (p->name != "Parameter") continue;
-        any = true;
-        const XNode *id = need(p.get(), "Id", msg), *ex = id ? need(p.get(), "Expression", msg) : nullptr;
-        if (!id || !ex) return TL_ERR_PARSE;
-        char *end = nullptr;
-        double v = strtod(ex->text.c_str(), &end);
-        if (end == ex->text.c_str()) {
-            msg = "parameter " + id->text + ": expression '" + ex->text + "' is not a number";
-            return TL_ERR_PARSE;
-        }
-        doc.params[id->text] = v;
-    }
-    if (!any) {
-        msg = "can not find Parameter node";
-        return TL_ERR_PARSE;
-    }
-
-    // ReactionsList / Reaction
-    const XNode *rlist = need(model, "ReactionsList", msg);
-    if (!rlist) return TL_ERR_PARSE;
-    any = false;
-    for (auto &r : rlist->kids) {
-        if (r->name != "Reaction") continue;
-        any = true;
-        ReactionDesc rd;
-        const XNode *rate = need(r.get(), "Rate", msg);
-        if (!rate) return TL_ERR_PARSE;
-        rd.rate_id = rate->text;
-        auto it = doc.params.find(rd.rate_id);
-        if (it == doc.params.end()) {
-            msg = "can not find the rate for the reaction " + rd.rate_id;
-            return TL_ERR_PARSE;
-        }
-        rd.rate = it->second;
-        const XNode *reac = need(r.get(), "Reactants", msg);
-        if (!reac) return TL_ERR_PARSE;
-        const XNode *prod = r->child("Products");
-        for (int side = 0; side < 2; ++side) {
-            const XNode *lst = side == 0 ? reac : prod;
-            if (!lst) continue;
-            for (auto &sr : lst->kids) {
-                if (sr->name != "SpeciesReference") continue;
-                const std::string *id = sr->attr("id"), *st = sr->attr("stoichiometry");
-                if (!id || !st) {
-                    msg = "SpeciesReference without id or stoichiometry";
-                    return TL_ERR_PARSE;
-                }
-                (side == 0 ? rd.reactants : rd.products)[*id] = atoi(st->c_str());
-            }
-        }
-        doc.reactions.push_back(std::move(rd));
-    }
-    if (!any) {
-        msg = "can not find Reaction node";
-        return TL_ERR_PARSE;
-    }
-    if ((int32_t)doc.reactions.size() != doc.n_reactions_declared) {
-        msg = "reaction count mismatch: " + std::to_string(doc.reactions.size()) + " read, " +
-              std::to_string(doc.n_reactions_declared) + " declared";
-        return TL_ERR_PARSE;
-    }
-
-    // SpeciesList / Species {Id, InitialPopulation}
-    const XNode *slist = need(model, "SpeciesList", msg);
-    if (!slist) return TL_ERR_PARSE;
-    std::unordered_map<std::string, size_t> seen;
-    for (auto &s : slist->kids) {
-        if (s->name != "Species") continue;
-        const XNode *id = need(s.get(), "Id", msg), *pop = id ? need(s.get(), "InitialPopulation", msg) : nullptr;
-        if (!id || !pop) return TL_ERR_PARSE;
-        auto f = seen.find(id->text);
-        if (f != seen.end()) {  // a repeated Id keeps its first index, takes the last population
-            doc.populations[f->second] = atoi(pop->text.c_str());
-            continue;
-        }
-        seen[id->text] = doc.species.size();
-        doc.species.push_back(id->text);
-        doc.populations.push_back(atoi(pop->text.c_str()));
-    }
-    if (doc.species.empty()) {
-        msg = "can not find Species node";
-        return TL_ERR_PARSE;
-    }
-    if ((int32_t)doc.species.size() != doc.n_species_declared) {
-        msg = "species count mismatch: " + std::to_string(doc.species.size()) + " read, " +
-              std::to_string(doc.n_species_declared) + " declared";
-        return TL_ERR_PARSE;
-    }
-    return TL_OK;
-}
 
 // ------------------------------------------------------------- layouts ----
 
@@ -538,62 +198,6 @@ tl_status build_model(tl_model &m, std::string &msg) {
     return TL_OK;
 }
 
-// Channel-major assembly (ReadXMLToMatrixStruct): entries sorted by species;
-// a species in both lists yields a reactant entry then a product entry.
-static tl_status assemble(const StochKitDoc &doc, tl_model &m, std::string &msg) {
-    std::unordered_map<std::string, int32_t> idx;
-    for (size_t i = 0; i < doc.species.size(); ++i) idx[doc.species[i]] = (int32_t)i;
-    m.N = (int32_t)doc.species.size();
-    m.M = (int32_t)doc.reactions.size();
-    m.col_ptr.assign(1, 0);
-    m.row_ind.clear();
-    m.stoich.clear();
-    m.rates.clear();
-    for (const ReactionDesc &r : doc.reactions) {
-        std::vector<int32_t> ind;
-        for (auto &kv : r.reactants) {
-            auto f = idx.find(kv.first);
-            if (f == idx.end()) {
-                msg = "reaction uses undeclared species " + kv.first;
-                return TL_ERR_PARSE;
-            }
-            ind.push_back(f->second);
-        }
-        for (auto &kv : r.products) {
-            auto f = idx.find(kv.first);
-            if (f == idx.end()) {
-                msg = "reaction uses undeclared species " + kv.first;
-                return TL_ERR_PARSE;
-            }
-            ind.push_back(f->second);
-        }
-        std::sort(ind.begin(), ind.end());
-        std::set<int32_t> found;
-        for (int32_t s : ind) {
-            const std::string &nm = doc.species[s];
-            auto rit = r.reactants.find(nm);
-            int v;
-            if (rit != r.reactants.end() && !found.count(s)) {
-                v = -rit->second;
-                found.insert(s);
-            } else {
-                v = r.products.at(nm);
-            }
-            if (v < -128 || v > 127) {
-                msg = "stoichiometry out of range";
-                return TL_ERR_STOICH_RANGE;
-            }
-            m.row_ind.push_back(s);
-            m.stoich.push_back((int8_t)v);
-        }
-        m.col_ptr.push_back((int32_t)m.row_ind.size());
-        m.rates.push_back(r.rate);
-    }
-    m.x0 = doc.populations;
-    m.names = doc.species;
-    return TL_OK;
-}
-
 }  // namespace tl
 
 tl_model::~tl_model() { tl::release_device_copies(*this); }
@@ -605,16 +209,13 @@ extern "C" {
 tl_status tl_model_from_stochkit(const char *path, tl_model **out) {
     if (!out || !path) return set_error(TL_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
-    std::ifstream f(path, std::ios::binary);
+    FILE *f = std::fopen(path, "rb");
     if (!f) return set_error(TL_ERR_IO, std::string("can not open file ") + path);
-    std::stringstream ss;
-    ss << f.rdbuf();
-    StochKitDoc doc;
-    std::string msg;
-    tl_status st = parse_stochkit(ss.str(), doc, msg);
-    if (st != TL_OK) return set_error(st, std::string(path) + ": " + msg);
     auto m = std::make_unique<tl_model>();
-    if ((st = assemble(doc, *m, msg)) != TL_OK) return set_error(st, std::string(path) + ": " + msg);
+    std::string msg;
+    tl_status st = read_stochkit(f, *m, msg);
+    std::fclose(f);
+    if (st != TL_OK) return set_error(st, std::string(path) + ": " + msg);
     if ((st = build_model(*m, msg)) != TL_OK) return set_error(st, std::string(path) + ": " + msg);
     *out = m.release();
     return TL_OK;

@@ -1,8 +1,9 @@
-// tl_model.hpp -- host-side model: StochKit reader, channel-major assembly,
-// reaction orders and the packed species-major / channel-major layouts.
+// tl_model.hpp -- host-side model: CSC entries as read, reaction orders and
+// the packed species-major / channel-major layouts derived from them.
 #pragma once
 
 #include <stdint.h>
+#include <stdio.h>
 
 #include <map>
 #include <mutex>
@@ -12,31 +13,15 @@
 
 #include "tl_api.h"
 
+struct tl_model;
+
 namespace tl {
 
 struct DeviceModelCache;  // device copies, one per HIP device (tl_engine)
 
-// One reaction as read from StochKit XML (StochKitReader.cpp:35-90):
-// species name -> stoichiometry for reactants and products.
-struct ReactionDesc {
-    std::string rate_id;
-    double rate = 0.0;
-    std::map<std::string, int> reactants;
-    std::map<std::string, int> products;
-};
-
-struct StochKitDoc {
-    int32_t n_reactions_declared = 0;
-    int32_t n_species_declared = 0;
-    std::unordered_map<std::string, double> params;
-    std::vector<ReactionDesc> reactions;
-    std::vector<std::string> species;       // SpeciesList order = species index
-    std::vector<int32_t> populations;
-};
-
-// Parse StochKit 2.0 XML text.  Returns TL_OK or TL_ERR_PARSE with msg set.
-tl_status parse_stochkit(const std::string &text, StochKitDoc &doc, std::string &msg);
-
+// Read a StochKit 2.0 file into m: N, M, CSC entries, rates, x0, names
+// (TL_OK, TL_ERR_PARSE or TL_ERR_STOICH_RANGE with msg set).
+tl_status read_stochkit(FILE *f, tl_model &m, std::string &msg);
 }  // namespace tl
 
 struct tl_model {
