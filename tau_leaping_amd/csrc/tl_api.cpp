@@ -493,6 +493,19 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
                 out.push_back(c);
             }
         }
+    if (out.empty()) {
+        // nothing fits LDS: 1024-lane groups with their state in global
+        // memory, the largest propensity tile that fits
+        PersistentConfig c{1024, 1024, false, 1};
+        c.hbm = true;
+        c.tile_grains = 4;
+        size_t lds = persistent_lds_bytes(c, is64, n_max, m_max);
+        while (lds > dc.lds_per_block && c.tile_grains > 1) {
+            c.tile_grains /= 2;
+            lds = persistent_lds_bytes(c, is64, n_max, m_max);
+        }
+        if (lds <= dc.lds_per_block && persistent_occupancy(c, is64, sens, lds) > 0) out.push_back(c);
+    }
     return out;
 }
 
@@ -541,6 +554,13 @@ PersistentLaunch make_launch(DeviceContext &dc, const PersistentConfig &cfg, boo
     L.grid = (int)grid;
     L.lds_bytes = lds;
     L.tile_grains = cfg.tile_grains;
+    L.state_stride = persistent_state_bytes(cfg, is64, n_max, m_max);
+    if (cfg.hbm) {
+        // at most 32 GB of group state (each block holds one slice)
+        const int64_t cap = std::max<int64_t>(1, (int64_t)(32e9 / (double)L.state_stride));
+        L.grid = (int)std::min<int64_t>(L.grid, cap);
+        L.slots = L.grid;
+    }
     return L;
 }
 
@@ -598,6 +618,7 @@ tl_status run_launches(DeviceContext &dc, bool is64, bool sens, const std::vecto
 tl_status launch_batch(DeviceContext &dc, const PersistentConfig &cfg, bool is64, bool sens, ModelItem *d_items,
                        int32_t n_models, int64_t n_items, int32_t n_max, int32_t m_max, unsigned long long *d_counter,
                        int64_t grid_cap, bool use_graph, float *ms_out) {
+    if (cfg.hbm) return set_error(TL_ERR_INVALID_ARG, "trial launches of global-memory state shapes are not supported");
     std::vector<ClassLaunch> cl(1);
     cl[0].cfg = cfg;
     cl[0].L = make_launch(dc, cfg, is64, sens, d_items, n_models, n_items, n_max, m_max, d_counter, grid_cap);
@@ -817,6 +838,7 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
         ClassLaunch c;
         c.cfg = cfg;
         c.L = make_launch(*dc, cfg, is64, sens, d_items, nm, n_items, n_max, m_max, d_counter, 0);
+        if (cfg.hbm) TL_TRY(B.alloc(c.L.state, (size_t)c.L.grid * c.L.state_stride));
         launches.push_back(c);
         for (int32_t k = 0; k < nm; ++k) entry_cfg[members[k]] = cfg;
     }

@@ -120,6 +120,7 @@ struct PersistentConfig {
     bool a_lds;    // propensities in LDS (else recomputed where they are read)
     int wpe;       // waves per SIMD the kernel is compiled for (register budget 512 / wpe)
     int tile_grains = 1;  // block groups without a in LDS: propensity tile of tile_grains * 4096 channels
+    bool hbm = false;     // group state in global memory (models whose state does not fit LDS)
 };
 
 struct PersistentLaunch {
@@ -132,10 +133,14 @@ struct PersistentLaunch {
     int grid;
     size_t lds_bytes;  // per block
     int tile_grains;   // see PersistentConfig
+    unsigned char *state = nullptr;  // hbm: grid slices of state_stride bytes
+    size_t state_stride = 0;
 };
 
 // LDS bytes per block for a config; 0 if it can not fit.
 size_t persistent_lds_bytes(const PersistentConfig &c, bool is64, int32_t n_max, int32_t m_max);
+// Global-memory state bytes per block (hbm configs; 0 otherwise).
+size_t persistent_state_bytes(const PersistentConfig &c, bool is64, int32_t n_max, int32_t m_max);
 // Max resident blocks per CU for the config (0 if unsupported).
 int persistent_occupancy(const PersistentConfig &c, bool is64, bool sens, size_t lds_bytes);
 hipError_t launch_persistent(const PersistentConfig &c, bool is64, bool sens, const PersistentLaunch &L, hipStream_t s);

@@ -134,6 +134,14 @@ __host__ __device__ inline GroupLayout group_layout(int32_t n_max, int32_t m_max
     return L;
 }
 
+// LDS of a group whose state is in global memory: reduction scratch, then the
+// propensity tile.
+__host__ __device__ inline size_t hbm_tile_offset(int gs) { return al16((size_t)(4 * (gs / 64) + 4) * 8); }
+__host__ __device__ inline size_t hbm_lds_bytes(int32_t m_max, int gs, size_t real_bytes, int tile_grains) {
+    const int32_t tile_ch = tile_grains * kTileGrain;
+    return hbm_tile_offset(gs) + al16((size_t)(m_max < tile_ch ? m_max : tile_ch) * real_bytes);
+}
+
 // ------------------------------------------------------ uniform values --
 // Values every lane of a wave holds identically are moved to scalar
 // registers, so model pointers, times and step sizes do not occupy VGPRs and
@@ -1300,8 +1308,15 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
 
 // WPE: waves per SIMD the register allocation targets (more waves hide more
 // latency; fewer give each wave more registers).
-template <class R, int BS, int GS, bool ALDS, bool SENS, int WPE>
+// HBM: the group's state (populations, proposal, snapshot, criticality words,
+// species sums, word sums) lives in a slice of global memory per block
+// instead of LDS, for models whose state does not fit LDS; only the
+// reduction scratch and the propensity tile stay in LDS.  Block groups only:
+// their barriers are full workgroup barriers, which order global memory
+// between the waves of a block as they order LDS.
+template <class R, int BS, int GS, bool ALDS, bool SENS, int WPE, bool HBM>
 __global__ __launch_bounds__(BS, WPE) void k_persistent(PersistentLaunch L) {
+    static_assert(!HBM || (GS == BS && GS > 64 && !ALDS), "HBM state: block groups with recomputed propensities");
     extern __shared__ __align__(16) unsigned char lds[];
     const GroupLayout lay = group_layout(L.n_max, L.m_max, GS, ALDS, sizeof(R), L.tile_grains);
     const int tid = threadIdx.x;
@@ -1310,15 +1325,15 @@ __global__ __launch_bounds__(BS, WPE) void k_persistent(PersistentLaunch L) {
     G.lane = tid % GS;
     G.wl = tid & 63;
     G.wid = G.lane >> 6;
-    unsigned char *base = lds + (size_t)g * lay.bytes;
+    unsigned char *base = HBM ? L.state + (size_t)blockIdx.x * L.state_stride : lds + (size_t)g * lay.bytes;
     int32_t *xs = reinterpret_cast<int32_t *>(base + lay.off_x);
     int32_t *dx = reinterpret_cast<int32_t *>(base + lay.off_dx);
     int32_t *snap = reinterpret_cast<int32_t *>(base + lay.off_snap);
     uint64_t *critq = reinterpret_cast<uint64_t *>(base + lay.off_crit);
-    G.red = reinterpret_cast<double *>(base + lay.off_red);
+    G.red = reinterpret_cast<double *>(HBM ? lds : base + lay.off_red);
     double *mu = reinterpret_cast<double *>(base + lay.off_mu);
     double *sig = reinterpret_cast<double *>(base + lay.off_sig);
-    R *tile = reinterpret_cast<R *>(base + lay.off_tile);
+    R *tile = reinterpret_cast<R *>(HBM ? lds + hbm_tile_offset(GS) : base + lay.off_tile);
     double *wsum = reinterpret_cast<double *>(base + lay.off_wsum);
 
     for (;;) {
@@ -1346,11 +1361,13 @@ __global__ __launch_bounds__(BS, WPE) void k_persistent(PersistentLaunch L) {
 
 // ---------------------------------------------------------- dispatch ----
 template <class R, bool SENS>
-const void *kernel_ptr(int bs, int gs, bool alds, int wpe) {
+const void *kernel_ptr(int bs, int gs, bool alds, int wpe, bool hbm) {
+    if (hbm) return (bs == 1024 && gs == 1024 && !alds && wpe == 1)
+                        ? (const void *)k_persistent<R, 1024, 1024, false, SENS, 1, true> : nullptr;
 #define TL_K(BS_, GS_, W_)                                                                             \
     if (bs == BS_ && gs == GS_ && wpe == W_)                                                           \
-        return alds ? (const void *)k_persistent<R, BS_, GS_, true, SENS, W_>                          \
-                    : (const void *)k_persistent<R, BS_, GS_, false, SENS, W_>;
+        return alds ? (const void *)k_persistent<R, BS_, GS_, true, SENS, W_, false>                   \
+                    : (const void *)k_persistent<R, BS_, GS_, false, SENS, W_, false>;
     TL_K(64, 64, 4)
     TL_K(64, 64, 5)
     TL_K(256, 64, 4)
@@ -1369,10 +1386,10 @@ const void *kernel_ptr(int bs, int gs, bool alds, int wpe) {
 
 const void *kernel_for(const PersistentConfig &c, bool is64, bool sens) {
     if (is64)
-        return sens ? kernel_ptr<double, true>(c.block, c.group, c.a_lds, c.wpe)
-                    : kernel_ptr<double, false>(c.block, c.group, c.a_lds, c.wpe);
-    return sens ? kernel_ptr<float, true>(c.block, c.group, c.a_lds, c.wpe)
-                : kernel_ptr<float, false>(c.block, c.group, c.a_lds, c.wpe);
+        return sens ? kernel_ptr<double, true>(c.block, c.group, c.a_lds, c.wpe, c.hbm)
+                    : kernel_ptr<double, false>(c.block, c.group, c.a_lds, c.wpe, c.hbm);
+    return sens ? kernel_ptr<float, true>(c.block, c.group, c.a_lds, c.wpe, c.hbm)
+                : kernel_ptr<float, false>(c.block, c.group, c.a_lds, c.wpe, c.hbm);
 }
 
 // ------------------------------------------------------- gradient reduce --
@@ -1416,8 +1433,15 @@ __global__ __launch_bounds__(256) void k_grad(int64_t Rn, int32_t T, int32_t M, 
 }  // namespace
 
 size_t persistent_lds_bytes(const PersistentConfig &c, bool is64, int32_t n_max, int32_t m_max) {
+    if (c.hbm) return hbm_lds_bytes(m_max, c.group, is64 ? 8 : 4, c.tile_grains);
     const GroupLayout L = group_layout(n_max, m_max, c.group, c.a_lds, is64 ? 8 : 4, c.tile_grains);
     return L.bytes * (size_t)(c.block / c.group);
+}
+
+size_t persistent_state_bytes(const PersistentConfig &c, bool is64, int32_t n_max, int32_t m_max) {
+    if (!c.hbm) return 0;
+    const size_t b = group_layout(n_max, m_max, c.group, false, is64 ? 8 : 4, c.tile_grains).bytes;
+    return (b + 255) & ~(size_t)255;
 }
 
 int persistent_occupancy(const PersistentConfig &c, bool is64, bool sens, size_t lds_bytes) {

@@ -125,26 +125,28 @@ def test_mixed_size_batch(cuda):
         _compare(b.final_x, single.final_x)
 
 
-def test_model_larger_than_lds(cuda):
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_model_larger_than_lds(cuda, prec):
     # 6000 species x 12000 channels: a realization's populations, snapshots
-    # and species sums (28 B per species) exceed the 160 KiB of LDS, so no
-    # persistent launch shape fits.  The persistent form refuses it; AUTO runs
-    # the plain form (state in HBM), identical to asking for it, and its
-    # trajectories follow the oracle's.
+    # and species sums (28 B per species) exceed the 160 KiB of LDS.  The
+    # persistent form runs it on 1024-lane groups whose state lives in HBM
+    # (AUTO picks that), the plain form with its state in HBM too; both follow
+    # the oracle's trajectories.
     net = models.toy_network(6000, 12000, seed=41)
     m, om = pair(net)
     R, tf = 256, 1e-4
-    with pytest.raises(T.TLError) as e:
-        m.simulate(tf, 8, seed=5, mode="persistent")
-    assert e.value.status == 11
     track = np.arange(0, 6000, 150)
-    res = m.simulate(tf, R, seed=5, track=track)
-    plain = m.simulate(tf, R, seed=5, track=track, mode="plain")
-    assert np.array_equal(res.final_x, plain.final_x) and np.array_equal(res.steps, plain.steps)
-    ora = om.simulate(R, tf, seed=5)
-    assert res.n_failed == 0 and (ora["status"] == 0).all()
-    same = _compare(res.final_x, ora["x_final"][:, track])
-    assert np.array_equal(res.steps[same], ora["steps"][same])
+    res = m.simulate(tf, R, seed=5, track=track, precision=prec)
+    pers = m.simulate(tf, R, seed=5, track=track, precision=prec, mode="persistent")
+    plain = m.simulate(tf, R, seed=5, track=track, precision=prec, mode="plain")
+    assert res.block_size == 1024
+    assert np.array_equal(res.final_x, pers.final_x) and np.array_equal(res.steps, pers.steps)
+    ora = om.simulate(R, tf, seed=5, precision=prec)
+    assert (ora["status"] == 0).all()
+    for r in (res, plain):
+        assert r.n_failed == 0
+        same = _compare(r.final_x, ora["x_final"][:, track])
+        assert np.array_equal(r.steps[same], ora["steps"][same])
 
 
 def test_conservation_laws(cuda):
