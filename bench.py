@@ -23,7 +23,8 @@ plain per-model launch loop against the persistent launch on the same
 realizations; the cyclic chain with score-function sensitivities (grad/fwd
 cost ratio); the 31,250-channel colloid model (persistent FP64 / FP32 and a
 plain sample); the heterogeneous 64-model batch in one persistent launch
-against the plain per-model loop.
+against the plain per-model loop; large-model ingestion (generator and
+reader at 1e6 channels, persistent run with the state in HBM).
 """
 import argparse
 import json
@@ -167,6 +168,32 @@ def extras_run(T, models, np, m_toy, kw_toy):
         "plain_models": pick, "plain_realizations": int(sum(k["n_realizations"] for k in sub)),
         "plain_leaps_per_s": steps_p / dtp,
         "plain_sensitivities": False, "batched_over_plain": (steps_b / dtb) / (steps_p / dtp)}
+
+    # large-model ingestion: toy_system_gen at 1e6 channels over 2e5 species
+    # (about 500 MB of StochKit XML), the streaming reader, and the
+    # persistent engine with each realization's state in HBM
+    log("large model")
+    import subprocess
+    import tempfile
+    gen = os.path.join(ROOT, "tau_leaping_amd", "bin", "toy_system_gen")
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "toy200000x1000000.xml")
+        t0 = time.perf_counter()
+        with open(path, "w") as f:
+            subprocess.run([gen, "-r", "200000", "-c", "1000000", "--toy", "--seed", "3", "--log-rates", "0.001", "1"],
+                           stdout=f, stderr=subprocess.DEVNULL, check=True)
+        t_gen = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        t0 = time.perf_counter()
+        big = T.Model.from_stochkit(path)
+        t_read = time.perf_counter() - t0
+    rb = big.simulate(1e-6, 256, seed=1, track=np.arange(4, dtype=np.int32), max_iter=3000)
+    lb = bytes_per_leap(big.n_species, big.n_channels)
+    ex["large1m"] = {
+        "species": big.n_species, "channels": big.n_channels, "nnz": big.nnz, "xml_bytes": size,
+        "generate_seconds": t_gen, "read_seconds": t_read, "realizations": 256, "t_final": 1e-6,
+        "leaps_per_s": rate(rb), "kernel_ms": rb.kernel_ms, "block_size": rb.block_size, "failed": rb.n_failed,
+        "bytes_per_leap": lb, "hbm_frac": lb * rb.total_steps / (rb.kernel_ms * 1e-3) / (HBM_PEAK_GBS * 1e9)}
     return ex
 
 

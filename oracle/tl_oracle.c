@@ -79,8 +79,10 @@ void tlo_model_destroy(tlo_model *m) {
 }
 
 /* Reaction order from the negative entries of a column, in species order
- * (OrderDetectorAndIndFiller, CSCMatrix.cu:200-231).  Patterns other than
- * S->, 2S->, Sa+Sb-> are rejected. */
+ * (OrderDetectorAndIndFiller, CSCMatrix.cu:200-231).  A column without
+ * negative entries is a source (order 0; the reference returns order -1 and
+ * leaves its propensity unset).  Patterns other than ->, S->, 2S->, Sa+Sb->
+ * are rejected. */
 static int detect_order(const tlo_model *m, int32_t j) {
     int32_t nneg = 0, first_v = 0, i0 = -1, i1 = -1, second_v = 0;
     for (int32_t e = m->col_ptr[j]; e < m->col_ptr[j + 1]; ++e) {
@@ -93,6 +95,12 @@ static int detect_order(const tlo_model *m, int32_t j) {
             i1 = m->row_ind[e];
         }
         nneg++;
+    }
+    if (nneg == 0) {
+        m->order[j] = 0;
+        m->s0[j] = 0;
+        m->s1[j] = 0;
+        return TLO_OK;
     }
     if (nneg == 1 && first_v == -2) {
         m->order[j] = 3;

@@ -37,8 +37,10 @@
  *     CSRMatrixPacked.cu:502).
  *   - RNG: Philox4x32-10 counter streams keyed per realization instead of the
  *     per-channel hybrid Tausworthe state (d_rng.cuh:27-35).
- *   - unsupported reactant patterns are rejected at model build
- *     (ref: order -1 gives an uninitialised propensity, functions.h:118-120).
+ *   - sources (no reactant, -> products) have a = c, are never critical and
+ *     take g = 1 where GetG reads their order; other unsupported reactant
+ *     patterns are rejected at model build (ref: order -1 gives an
+ *     uninitialised propensity, functions.h:118-120, CSCMatrix.cu:225-231).
  *   - tau_i for species rows i >= M is +inf (ref reads order[] out of bounds,
  *     CSRMatrixPacked.cu:353).
  *   - the species sums mu_i, sigma^2_i of the tau bound accumulate in FP64 and

@@ -39,9 +39,12 @@ static inline const REAL *FN(rates)(const tlo_model *m) {
 
 /* Propensity and criticality of one channel (functions.h:100-123, nc = 5 from
  * model.cu:225).  Operation order is the reference's: (c*x0)*x1 and
- * ((c*x0)*(x0-1))/2. */
+ * ((c*x0)*(x0-1))/2.  A source (order 0) has a = c and is never critical. */
 static inline REAL FN(prop1)(int order, REAL c, int32_t x0, int32_t x1, int *crit) {
     switch (order) {
+    case 0:
+        *crit = 0;
+        return c;
     case 1:
         *crit = x0 < TLO_NC;
         return c * (REAL)x0;
@@ -74,6 +77,7 @@ static void FN(prop_one)(const tlo_model *m, const int32_t *x, ptrdiff_t xs, REA
 /* GetG (CSRMatrixPacked.cu:294-309). */
 static inline REAL FN(getg)(int order, int32_t x) {
     switch (order) {
+    case 0: /* a source: the first-order bound */
     case 1: return (REAL)1;
     case 2: return (REAL)2;
     case 3: return (REAL)2 + (REAL)1 / ((REAL)x - (REAL)1);

@@ -60,7 +60,7 @@ struct DevModel {
     const int2 *aff_rs;      // [aff_ptr[M]] rs of each listed channel (loads in parallel with aff)
     const double *aff_c64;   // [aff_ptr[M]] its rate
     const float *aff_c32;
-    const int32_t *tile_ptr; // [N][n_grains + 1] CSR offset of each species row at channel g * kTileGrain
+    const int32_t *tile_ptr; // [n_grains + 1][N] CSR offset of each species row at channel g * kTileGrain
 };
 
 // ---- Philox4x32-10 ----
@@ -109,15 +109,16 @@ template <> struct Real<float> {
 };
 
 // Propensity of one channel; operation order (c*x0)*x1 and ((c*x0)*(x0-1))/2
-// as in IsCriticalAndPropensity.
+// as in IsCriticalAndPropensity.  Order 0 (a source with no reactant) has
+// a = c and is never critical.
 // Written without branches (channels of different orders share a wave); the
 // roundings are those of the three formulas: c*x0, (c*x0)*f, then an exact
 // halving for order 3.
 template <class R>
 __device__ __forceinline__ R propensity(int order, R c, int32_t x0, int32_t x1, bool &crit) {
-    const bool o2 = order == 2, o3 = order == 3;
-    crit = o3 ? (x0 * 2 < kNc) : ((x0 < kNc) || (o2 && x1 < kNc));
-    const R t = c * (R)x0;
+    const bool o0 = order == 0, o2 = order == 2, o3 = order == 3;
+    crit = !o0 && (o3 ? (x0 * 2 < kNc) : ((x0 < kNc) || (o2 && x1 < kNc)));
+    const R t = o0 ? c : c * (R)x0;
     const R f = (R)(o2 ? x1 : x0 - 1);
     const R t2 = (o2 || o3) ? t * f : t;
     return o3 ? t2 * (R)0.5 : t2;
@@ -125,7 +126,7 @@ __device__ __forceinline__ R propensity(int order, R c, int32_t x0, int32_t x1, 
 
 template <class R>
 __device__ __forceinline__ R getg(int order, int32_t x) {
-    if (order == 1) return (R)1;
+    if (order == 0 || order == 1) return (R)1;  // order 0: a source, the first-order bound
     if (order == 2) return (R)2;
     if (order == 3) return (R)2 + (R)1 / ((R)x - (R)1);
     return (R)0;

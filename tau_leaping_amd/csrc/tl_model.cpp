@@ -38,6 +38,11 @@ static bool channel_order(const tl_model &m, int32_t j, int8_t &order, int32_t &
         }
         ++nneg;
     }
+    if (nneg == 0) {  // a source: no reactant, a = c
+        order = 0;
+        a = b = 0;
+        return true;
+    }
     if (nneg == 1 && v0 == -1) {
         order = 1;
         a = b = i0;
@@ -127,6 +132,7 @@ tl_status build_model(tl_model &m, std::string &msg) {
     // reactant-of lists: the channels whose propensity changes with x_i
     m.dep_ptr.assign(m.N + 1, 0);
     for (int32_t j = 0; j < m.M; ++j) {
+        if (m.order[j] == 0) continue;  // sources read no species
         m.dep_ptr[m.s0[j] + 1]++;
         if (m.s1[j] != m.s0[j]) m.dep_ptr[m.s1[j] + 1]++;
     }
@@ -135,6 +141,7 @@ tl_status build_model(tl_model &m, std::string &msg) {
     {
         std::vector<int32_t> f(m.dep_ptr.begin(), m.dep_ptr.end() - 1);
         for (int32_t j = 0; j < m.M; ++j) {
+            if (m.order[j] == 0) continue;
             m.dep[f[m.s0[j]]++] = j;
             if (m.s1[j] != m.s0[j]) m.dep[f[m.s1[j]]++] = j;
         }
@@ -175,7 +182,8 @@ tl_status build_model(tl_model &m, std::string &msg) {
             m.aff.swap(list);
         }
     }
-    // row offsets at propensity-tile boundaries (rows are sorted by channel)
+    // row offsets at propensity-tile boundaries (rows are sorted by channel),
+    // grain-major so that a tile pass reads them coalesced over species
     {
         constexpr int32_t kTile = 4096;
         const int32_t nt = std::max(1, (m.M + kTile - 1) / kTile);
@@ -185,7 +193,7 @@ tl_status build_model(tl_model &m, std::string &msg) {
             for (int32_t t = 0; t <= nt; ++t) {
                 const int32_t bound = (t == nt) ? INT32_MAX : t * kTile;
                 while (e < m.row_ptr[i + 1] && (m.csr[e] >> 3) < bound) ++e;
-                m.tile_ptr[(size_t)i * (nt + 1) + t] = e;
+                m.tile_ptr[(size_t)t * m.N + i] = e;
             }
         }
     }

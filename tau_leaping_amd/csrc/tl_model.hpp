@@ -47,8 +47,8 @@ struct tl_model {
     // propensities one firing of j can change); empty when it would exceed
     // kMaxAffected entries
     std::vector<int32_t> aff_ptr, aff;
-    // [N][n_grains + 1]: first CSR entry of row i whose channel is >= g * 4096
-    // (n_grains = ceil(M / 4096); entry n_grains is the row end)
+    // [n_grains + 1][N]: first CSR entry of row i whose channel is >= g * 4096
+    // (n_grains = ceil(M / 4096); grain n_grains holds the row ends)
     std::vector<int32_t> tile_ptr;
 
     uint64_t version = 1;  // bumped by set_rates / set_x0; invalidates device copies

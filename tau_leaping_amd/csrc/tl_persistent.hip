@@ -744,7 +744,7 @@ __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c,
     int L = 1;
     while (L < 64 && 2 * L * m.N <= GS) L *= 2;
     const int32_t TG = tile_grains, TILE = TG * kTileGrain;
-    const int32_t ng = (m.M + kTileGrain - 1) / kTileGrain;  // tile_ptr columns - 1
+    const int32_t ng = (m.M + kTileGrain - 1) / kTileGrain;  // grains (tile_ptr has ng + 1 rows)
     const int32_t nt = (m.M + TILE - 1) / TILE;
     const int32_t rows_per_pass = GS / L;
     const int k = lane & (L - 1);
@@ -818,7 +818,7 @@ __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c,
             const int32_t i = i0 + lane / L;
             double pm = 0.0, ps = 0.0;
             if (i < m.N) {
-                const int32_t e0 = m.tile_ptr[(size_t)i * (ng + 1) + g0], e1 = m.tile_ptr[(size_t)i * (ng + 1) + g1];
+                const int32_t e0 = m.tile_ptr[(size_t)g0 * m.N + i], e1 = m.tile_ptr[(size_t)g1 * m.N + i];
                 // eight entries per round with their loads issued together; the
                 // additions keep the lane's entry order
                 auto add = [&](int32_t p) {

@@ -64,8 +64,10 @@ def test_validation_status_codes():
         return T.Model.from_network(Network(3, reacs, np.array(x0, np.int32)))
 
     with pytest.raises(T.TLError) as e:
-        mk([Reaction({}, {0: 1}, 1.0)])
+        mk([Reaction({0: 1, 1: 1, 2: 1}, {}, 1.0)])  # three reactants
     assert e.value.status == 3
+    order, s0, s1, _, _ = mk([Reaction({}, {0: 1}, 1.0), Reaction({0: 1}, {}, 1.0)]).structure()
+    assert list(order) == [0, 1]  # a source is order 0
     with pytest.raises(T.TLError) as e:
         mk([Reaction({0: 3}, {1: 1}, 1.0)])
     assert e.value.status == 4

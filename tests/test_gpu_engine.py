@@ -149,6 +149,30 @@ def test_model_larger_than_lds(cuda, prec):
         assert np.array_equal(r.steps[same], ora["steps"][same])
 
 
+def _source_network():
+    # sources, decay, dimerisation (2A -> B), a reversible association and a
+    # catalytic step (B + D -> B + E)
+    from tau_leaping_amd.models import Network, Reaction
+    rx = [Reaction({}, {0: 1}, 50.0), Reaction({0: 1}, {}, 0.1), Reaction({0: 2}, {1: 1}, 0.001),
+          Reaction({1: 1}, {}, 0.05), Reaction({}, {2: 1}, 20.0), Reaction({0: 1, 2: 1}, {3: 1}, 0.002),
+          Reaction({3: 1}, {0: 1, 2: 1}, 0.5), Reaction({1: 1, 3: 1}, {1: 1, 4: 1}, 0.001),
+          Reaction({4: 1}, {}, 0.2), Reaction({}, {5: 2}, 5.0)]
+    return Network(6, rx, np.array([100, 10, 50, 0, 0, 3], np.int32))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("mode,bs", [("persistent", 64), ("persistent", 256), ("plain", 0)])
+def test_sources_match_oracle(cuda, prec, mode, bs):
+    # channels without reactants (a = c, never critical) in every form
+    m, om = pair(_source_network())
+    R, tf = 2048, 5.0
+    res = m.simulate(tf, R, seed=314, precision=prec, mode=mode, block_size=bs)
+    ora = om.simulate(R, tf, seed=314, precision=prec)
+    assert res.n_failed == 0 and (ora["status"] == 0).all()
+    same = _compare(res.final_x, ora["x_final"])
+    assert np.array_equal(res.steps[same], ora["steps"][same])
+
+
 def test_conservation_laws(cuda):
     # cyclic chain: sum_i x_i constant; colloid: sum_i i * x_i constant
     cyc = models.cyclic_chain(256, seed=1)

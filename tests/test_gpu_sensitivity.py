@@ -37,6 +37,26 @@ def test_scores_match_oracle(cuda, name, tf, bs):
     assert np.quantile(err.max(axis=1), 0.99) < 1e-8, np.quantile(err.max(axis=1), 0.99)
 
 
+@pytest.mark.parametrize("bs", [64, 1024])
+def test_source_scores(cuda, bs):
+    # a source's score term is 1/k per event minus the exposure t (a = k):
+    # per-realization scores equal the oracle's, and dE/dk, dE/dg match the
+    # closed form of the source with decay
+    from tau_leaping_amd.models import Network, Reaction
+    k, g, s0, tf = 20.0, 1.0, 5, 1.5
+    net = Network(1, [Reaction({}, {0: 1}, k), Reaction({0: 1}, {}, g)], np.array([s0], np.int32))
+    m, om = pair(net)
+    R = 16384
+    res = m.simulate(tf, R, seed=12, sensitivities=True, return_scores=True, block_size=bs)
+    ora = om.simulate(R, tf, seed=12, sensitivities=True)
+    same = np.all(res.final_x == ora["x_final"], axis=1) & (res.steps == ora["steps"])
+    assert same.mean() >= 0.97
+    assert np.abs(res.score[same] - ora["score"][same]).max() < 1e-8 * np.abs(ora["score"]).max()
+    e = np.exp(-g * tf)
+    exact = np.array([(1 - e) / g, -k / g ** 2 * (1 - e) + (k / g - s0) * tf * e])
+    assert np.all(np.abs(res.dmean_dc[0] - exact) < 4.5 * res.dmean_dc_se[0]), (res.dmean_dc, exact)
+
+
 def test_dmean_matches_closed_form_and_fd(cuda):
     N, tf = 8, 0.5
     rng = np.random.default_rng(4)

@@ -98,15 +98,17 @@ def test_dimer_tau_s1_species_value():
 
 
 # ---------------------------------------------- exact rational stage checks --
-def _random_tiny(seed, with_catalyst=True):
+def _random_tiny(seed, with_catalyst=True, with_sources=True):
     rng = np.random.default_rng(seed)
     N = int(rng.integers(3, 7))
     M = int(rng.integers(4, 9))
     rx = []
     for _ in range(M):
-        t = int(rng.integers(1, 4))
+        t = int(rng.integers(0, 4)) if with_sources else int(rng.integers(1, 4))
         sp = rng.permutation(N)
-        if t == 1:
+        if t == 0:
+            reac = {}
+        elif t == 1:
             reac = {int(sp[0]): 1}
         elif t == 2:
             reac = {int(sp[0]): 1, int(sp[1]): 1}
@@ -115,7 +117,7 @@ def _random_tiny(seed, with_catalyst=True):
         prod = {}
         for s in sp[2:2 + int(rng.integers(0, 3))]:
             prod[int(s)] = int(rng.integers(1, 3))
-        if with_catalyst and rng.random() < 0.25:
+        if with_catalyst and reac and rng.random() < 0.25:
             s0 = next(iter(reac))
             prod[s0] = prod.get(s0, 0) + 1
         rx.append(Reaction(reac, prod, float(np.exp(rng.uniform(np.log(1e-3), 0)))))
@@ -125,6 +127,8 @@ def _random_tiny(seed, with_catalyst=True):
 def _exact_prop(rx, x):
     c = Fraction(rx.rate)
     items = list(rx.reactants.items())
+    if not items:  # a source
+        return c, False
     if len(items) == 2:
         (i, _), (j, _) = items
         return c * x[i] * x[j], x[i] < 5 or x[j] < 5
@@ -172,8 +176,8 @@ def test_stages_exact_rational(seed):
                 continue
             o = net.reactions[i]  # GetG is indexed by species through the channel order array
             kinds = list(o.reactants.values())
-            order = 2 if len(kinds) == 2 else (3 if kinds[0] == 2 else 1)
-            g = {1: Fraction(1), 2: Fraction(2), 3: None}[order]
+            order = 0 if not kinds else (2 if len(kinds) == 2 else (3 if kinds[0] == 2 else 1))
+            g = {0: Fraction(1), 1: Fraction(1), 2: Fraction(2), 3: None}[order]  # a source takes g = 1
             xi = xr[i]
             if order == 3:
                 g = None if xi == 1 else Fraction(2) + Fraction(1, xi - 1)
@@ -274,6 +278,53 @@ def test_normal_branch_law(mean):
 
 
 # ---------------------------------------------------- cyclic chain closed form --
+def _immigration_death(k, g, s0):
+    return Network(1, [Reaction({}, {0: 1}, k), Reaction({0: 1}, {}, g)], np.array([s0], np.int32))
+
+
+def test_source_immigration_death_law():
+    """A source 0 -> S (rate k) with decay S -> 0 (rate g).  From S(0) = s0 the
+    state at t is Binomial(s0, e^{-gt}) + Poisson(k/g (1 - e^{-gt})):
+    survivors and immigrants are independent.  At these populations tau1 is
+    below 10/a0 at every step, so the direct method runs (exact) and the whole
+    law must match: chi-square over the pmf, mean and variance."""
+    k, g, s0, t = 20.0, 1.0, 5, 3.0
+    om = oracle_of(_immigration_death(k, g, s0))
+    res = om.simulate(40000, t, seed=7)
+    assert (res["status"] == 0).all()
+    x = res["x_final"][:, 0]
+    p = np.exp(-g * t)
+    lam = k / g * (1 - p)
+    support = np.arange(120)
+    pmf = np.convolve(stats.binom.pmf(np.arange(s0 + 1), s0, p), stats.poisson.pmf(support, lam))[:support.size]
+    obs = np.bincount(x, minlength=support.size)[:support.size]
+    keep = pmf * x.size >= 5
+    exp_ = pmf[keep] * x.size
+    obs_k = obs[keep].astype(float)
+    # pool the tails into one bin so the totals agree
+    exp_ = np.append(exp_, x.size - exp_.sum())
+    obs_k = np.append(obs_k, x.size - obs_k.sum())
+    assert stats.chisquare(obs_k, exp_).pvalue > 1e-3
+    mean = s0 * p + lam
+    var = s0 * p * (1 - p) + lam
+    assert abs(x.mean() - mean) < 3.3 * np.sqrt(var / x.size)
+    assert abs(x.var() / var - 1) < 0.05
+
+
+def test_source_tau_leap_mean():
+    """The same system at k = 1e5, s0 = 0: leaps carry the run.  The exact mean
+    k/g (1 - e^{-gt}) and the tau-leap mean differ by the explicit-Euler
+    bias of the mean equation at the leap sizes eps allows (below 1 % here);
+    a source propensity other than c would be off by its whole size."""
+    k, g, t = 1e5, 1.0, 1.0
+    om = oracle_of(_immigration_death(k, g, 0))
+    res = om.simulate(256, t, seed=3)
+    assert (res["status"] == 0).all()
+    assert res["steps"].mean() < 2000  # leaps, not only direct events
+    exact = k / g * (1 - np.exp(-g * t))
+    assert abs(res["x_final"][:, 0].mean() / exact - 1) < 0.01
+
+
 def test_direct_method_cyclic_closed_form():
     """Exact SSA (the fixed DirectMethodStep) on a linear cyclic chain: each
     molecule walks the ring independently, so E[X_k(T)] = sum_i x0_i P_ik and
@@ -347,6 +398,25 @@ def test_score_function_cyclic():
         assert np.all(np.abs(lr[:, j] - fd_mean) < 4.5 * np.hypot(lr_se[:, j], fd_se) + 0.05 * np.abs(fd_mean).max())
 
 
+def test_source_score_closed_form():
+    """Likelihood-ratio dE[S(t)]/dk and dE[S(t)]/dg for the source with decay:
+    E[S(t)] = k/g (1 - e^{-gt}) + s0 e^{-gt}, so dE/dk = (1 - e^{-gt})/g and
+    dE/dg = -k/g^2 (1 - e^{-gt}) + (k/g - s0) t e^{-gt}."""
+    k, g, s0, t = 20.0, 1.0, 5, 1.5
+    om = oracle_of(_immigration_death(k, g, s0))
+    R = 40000
+    res = om.simulate(R, t, seed=11, sensitivities=True)
+    assert (res["status"] == 0).all()
+    X = res["x_final"][:, 0].astype(float)
+    S = res["score"]
+    d = X - X.mean()
+    lr = (d @ S) / R
+    se = np.sqrt(((d[:, None] * S) ** 2).mean(0) / R)
+    e = np.exp(-g * t)
+    exact = np.array([(1 - e) / g, -k / g ** 2 * (1 - e) + (k / g - s0) * t * e])
+    assert np.all(np.abs(lr - exact) < 4.5 * se), (lr, exact, se)
+
+
 # --------------------------------------------------------------- misc --
 def test_grid_endpoints():
     net = models.toy_network(10, 20, seed=1)
@@ -363,8 +433,7 @@ def test_model_validation():
         return O.OracleModel(3, len(reacs), *net.to_csc())
 
     build([Reaction({0: 1}, {1: 1}, 1.0)])
-    with pytest.raises(ValueError, match="status 3"):
-        build([Reaction({}, {1: 1}, 1.0)])  # zero-order source
+    build([Reaction({}, {1: 1}, 1.0)])  # a source (order 0) is accepted
     with pytest.raises(ValueError, match="status 3"):
         build([Reaction({0: 1, 1: 1, 2: 1}, {}, 1.0)])  # three reactants
     with pytest.raises(ValueError, match="status 3"):
