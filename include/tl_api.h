@@ -57,6 +57,11 @@ enum {
 
 enum { TL_FP64 = 0, TL_FP32 = 1 };
 
+/* Firing counts of non-critical channels with mean >= 80: the reference's
+ * truncated normal approximation |N(m, m)| (model.cu:496-520), or an exact
+ * Poisson sample (PTRS). */
+enum { TL_POISSON_NORMAL = 0, TL_POISSON_EXACT = 1 };
+
 /* Engine forms: PLAIN = per-model host loop of stage kernels over
  * structure-of-arrays state in HBM; PERSISTENT = one work-stealing kernel that
  * runs whole trajectories with each realization's state in LDS (all models of
@@ -106,6 +111,8 @@ typedef struct tl_sim_params {
     int32_t block_size;         /* persistent kernel lanes per realization (64..1024); 0: autotune */
     uint32_t model_id;          /* RNG stream id of the model within a batch */
     int64_t realization_offset; /* first realization id */
+    int32_t poisson;            /* TL_POISSON_NORMAL (the reference's approximation above mean 80) or
+                                   TL_POISSON_EXACT (PTRS, Hoermann 1993, above mean 80) */
 } tl_sim_params;
 
 void tl_sim_params_default(tl_sim_params *p);

@@ -50,6 +50,50 @@ static inline void tlo_key(uint64_t seed, int64_t rid, uint32_t key[2]) {
 
 static inline double tlo_unif64(uint32_t b) { return ((double)b + 0.5) * 2.3283064365386962890625e-10; }
 
+/* Exact Poisson sample (the sampler option that replaces the normal
+ * approximation at mean >= 80): PTRS, the transformed rejection with squeeze
+ * of W. Hoermann, "The transformed rejection method for generating Poisson
+ * random variables", Insurance: Mathematics and Economics 12 (1993) 39-45,
+ * valid for mean >= 10, in FP64 over FP64 uniforms.  Attempt i takes the
+ * channel's draws 2i+1 and 2i+2 (draw 1 = head, draws n >= 2 as in the
+ * Poisson stream): U = u - 1/2 from the first, V from the second.  After
+ * TLO_PTRS_CAP rejected attempts (probability below 1e-60) it returns
+ * floor(mean), so the loop ends even for a non-finite mean. */
+#define TLO_PTRS_CAP 64
+static int32_t tlo_ptrs(double mean, uint32_t head, const uint32_t key[2], uint32_t channel, uint32_t epoch,
+                        uint32_t ctr3) {
+    const double smu = sqrt(mean);
+    const double b = 0.931 + 2.53 * smu;
+    const double a = -0.059 + 0.02483 * b;
+    const double lia = log(1.1239 + 1.1328 / (b - 3.4));
+    const double vr = 0.9277 - 3.6224 / (b - 2.0);
+    const double lmu = log(mean);
+    uint32_t ctr[4] = {0u, channel, epoch, ctr3}, blk[4];
+    int pos = 4;
+#define PTRS_NEXT(dst)                            \
+    do {                                          \
+        if (pos == 4) {                           \
+            tlo_philox4x32_10(ctr, key, blk);     \
+            ctr[0]++;                             \
+            pos = 0;                              \
+        }                                         \
+        (dst) = blk[pos++];                       \
+    } while (0)
+    uint32_t ub = head, vb;
+    for (int attempt = 0; attempt < TLO_PTRS_CAP; ++attempt) {
+        if (attempt > 0) PTRS_NEXT(ub);
+        PTRS_NEXT(vb);
+        const double U = tlo_unif64(ub) - 0.5, V = tlo_unif64(vb);
+        const double us = 0.5 - fabs(U);
+        const double k = floor((2.0 * a / us + b) * U + mean + 0.43);
+        if (us >= 0.07 && V <= vr) return (int32_t)k;
+        if (k < 0.0 || (us < 0.013 && V > us)) continue;
+        if (log(V) + lia - log(a / (us * us) + b) <= -mean + k * lmu - lgamma(k + 1.0)) return (int32_t)k;
+    }
+#undef PTRS_NEXT
+    return (int32_t)mean;
+}
+
 /* First draw of channel j at an epoch: output (j & 3) of the Philox block
  * shared by channels 4b..4b+3, b = j >> 2, at counter (b, 0, epoch,
  * ctr3 | TLO_KIND_HEAD). */
@@ -206,6 +250,19 @@ int tlo_model_create(int32_t N, int32_t M, const int32_t *col_ptr, const int32_t
 #undef REAL
 #undef SUF
 #undef TLO_REAL_IS_DOUBLE
+
+int32_t tlo_poisson_ptrs(double mean, uint64_t seed, uint32_t model_id, int64_t rid, uint32_t epoch,
+                         int32_t channel) {
+    uint32_t key[2];
+    tlo_key(seed, rid, key);
+    const uint32_t head = tlo_head_draw(key, model_id << 2, epoch, channel);
+    return tlo_ptrs(mean, head, key, (uint32_t)channel, epoch, (model_id << 2) | TLO_KIND_POISSON);
+}
+
+void tlo_poisson_ptrs_many(double mean, uint64_t seed, uint32_t model_id, int64_t rid0, int64_t n,
+                           uint32_t epoch, int32_t channel, int32_t *out) {
+    for (int64_t i = 0; i < n; ++i) out[i] = tlo_poisson_ptrs(mean, seed, model_id, rid0 + i, epoch, channel);
+}
 
 void tlo_update(const tlo_model *m, int64_t R, const int32_t *x, const int32_t *k, int32_t *x_out,
                 uint8_t *neg) {

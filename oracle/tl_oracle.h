@@ -47,6 +47,8 @@
  *     are rounded once to the propensity type (ref: FP32 sums); v * a_j is
  *     exact in FP64, so the rounded sums do not depend on the order of the
  *     additions to within the last FP64 bit.
+ *   - option: an exact Poisson sampler (PTRS, Hoermann 1993) in place of the
+ *     normal approximation at mean >= 80 (tlo_params.poisson_exact).
  *   - the first Poisson draw of channel j at a step is output (j & 3) of the
  *     Philox block shared by channels 4*(j>>2)..4*(j>>2)+3; later draws come
  *     from the channel's own counter stream (see tlo_head_draw).
@@ -159,6 +161,14 @@ void tlo_ssa_f32(const tlo_model *m, int64_t R, uint64_t seed, uint32_t model_id
                  int32_t *x, double *t, uint32_t *epoch, double t_final, int32_t n_events,
                  int32_t *events);
 
+/* Exact sampler (PTRS, mean >= 10) on the draws of channel j at an epoch
+ * (test hook). */
+int32_t tlo_poisson_ptrs(double mean, uint64_t seed, uint32_t model_id, int64_t rid, uint32_t epoch,
+                         int32_t channel);
+/* The same for realizations rid0 .. rid0 + n - 1. */
+void tlo_poisson_ptrs_many(double mean, uint64_t seed, uint32_t model_id, int64_t rid0, int64_t n,
+                           uint32_t epoch, int32_t channel, int32_t *out);
+
 /* Single Poisson draw from the channel stream (test hook for the sampler). */
 int32_t tlo_poisson_f64(double mean, uint64_t seed, uint32_t model_id, int64_t rid,
                         uint32_t epoch, int32_t channel);
@@ -178,6 +188,7 @@ typedef struct tlo_params {
     int32_t n_track;
     const int32_t *track;  /* [n_track] species indices */
     int32_t n_threads;     /* 0: all online cores */
+    int32_t poisson_exact; /* 1: PTRS instead of the normal approximation at mean >= 80 */
 } tlo_params;
 
 typedef struct tlo_outputs {

@@ -38,12 +38,21 @@ class _Model(C.Structure):
     ]
 
 
+def _poisson(p):
+    """Sampler above mean 80: 'normal' (the reference's approximation) or 'exact' (PTRS)."""
+    if p in ("normal", 0):
+        return 0
+    if p in ("exact", "ptrs", 1):
+        return 1
+    raise ValueError(f"unknown Poisson sampler {p!r}")
+
+
 class _Params(C.Structure):
     _fields_ = [
         ("t_final", C.c_double), ("eps", C.c_double), ("seed", C.c_uint64),
         ("model_id", C.c_uint32), ("precision", C.c_int32), ("max_iter", C.c_int64),
         ("sensitivities", C.c_int32), ("n_intervals", C.c_int32), ("n_track", C.c_int32),
-        ("track", C.POINTER(C.c_int32)), ("n_threads", C.c_int32),
+        ("track", C.POINTER(C.c_int32)), ("n_threads", C.c_int32), ("poisson_exact", C.c_int32),
     ]
 
 
@@ -84,6 +93,10 @@ def lib():
         L.tlo_poisson_f64.restype = C.c_int32
         L.tlo_poisson_f32.argtypes = [C.c_float, C.c_uint64, C.c_uint32, C.c_int64, C.c_uint32, C.c_int32]
         L.tlo_poisson_f32.restype = C.c_int32
+        L.tlo_poisson_ptrs.argtypes = [C.c_double, C.c_uint64, C.c_uint32, C.c_int64, C.c_uint32, C.c_int32]
+        L.tlo_poisson_ptrs.restype = C.c_int32
+        L.tlo_poisson_ptrs_many.argtypes = [C.c_double, C.c_uint64, C.c_uint32, C.c_int64, C.c_int64, C.c_uint32,
+                                            C.c_int32, I32P]
         L.tlo_update.argtypes = [C.POINTER(_Model), C.c_int64, I32P, I32P, I32P, U8P]
         L.tlo_simulate.argtypes = [C.POINTER(_Model), C.POINTER(_Params), C.c_int64, C.c_int64, C.POINTER(_Outputs)]
         L.tlo_simulate.restype = C.c_int
@@ -195,12 +208,12 @@ class OracleModel:
 
     # ---- full simulation ----
     def simulate(self, R, t_final, eps=0.03, seed=1234, precision="fp64", max_iter=0, sensitivities=False,
-                 n_intervals=0, track=None, n_threads=0, model_id=0, rbase=0):
+                 n_intervals=0, track=None, n_threads=0, model_id=0, rbase=0, poisson="normal"):
         R = int(R)
         track = np.arange(self.N, dtype=np.int32) if track is None else np.ascontiguousarray(track, np.int32)
         p = _Params(float(t_final), float(eps), int(seed), int(model_id), 0 if _rtype(precision) == np.float64 else 1,
                     int(max_iter), int(bool(sensitivities)), int(n_intervals), int(track.size),
-                    track.ctypes.data_as(C.POINTER(C.c_int32)), int(n_threads))
+                    track.ctypes.data_as(C.POINTER(C.c_int32)), int(n_threads), _poisson(poisson))
         xf = np.zeros((R, self.N), np.int32)
         score = np.zeros((R, self.M)) if sensitivities else None
         grid = np.zeros((R, n_intervals + 1, track.size)) if n_intervals > 0 else None
@@ -217,6 +230,16 @@ class OracleModel:
         nt = lib().tlo_simulate(self._p, C.byref(p), int(rbase), R, C.byref(o))
         return dict(x_final=xf, score=score, grid=grid, steps=steps, ssa_events=ssa, rejections=rej,
                     status=status, threads=nt)
+
+    def poisson_ptrs(self, mean, seed, rid, epoch, channel, model_id=0, n=None):
+        """PTRS draw of channel `channel` at `epoch` for realization rid, or an
+        array over realizations rid .. rid + n - 1."""
+        if n is None:
+            return lib().tlo_poisson_ptrs(float(mean), int(seed), int(model_id), int(rid), int(epoch), int(channel))
+        out = np.zeros(int(n), np.int32)
+        lib().tlo_poisson_ptrs_many(float(mean), int(seed), int(model_id), int(rid), int(n), int(epoch), int(channel),
+                                    out)
+        return out
 
     def poisson(self, mean, seed, rid, epoch, channel, precision="fp64", model_id=0):
         f = lib().tlo_poisson_f64 if _rtype(precision) == np.float64 else lib().tlo_poisson_f32

@@ -126,12 +126,13 @@ static double FN(tau_one)(const tlo_model *m, REAL eps, const int32_t *x, ptrdif
 }
 
 /* Knuth product (mean < 80, cap 300) or |polar Box-Muller| truncated
- * (model.cu:477-520, functions.h:37-54).  Draw 1 of the channel is `head`
+ * (model.cu:477-520, functions.h:37-54); with `exact`, PTRS (tlo_ptrs)
+ * instead of the normal approximation at mean >= 80.  Draw 1 of the channel is `head`
  * (see head_draw); draws 2, 3, ... come from the channel's own stream, output
  * ((n-2) & 3) of Philox block (n-2) >> 2 at counter (block, channel, epoch,
  * ctr3). */
 static int32_t FN(poisson_stream)(REAL mean, uint32_t head, const uint32_t key[2], uint32_t channel,
-                                  uint32_t epoch, uint32_t ctr3) {
+                                  uint32_t epoch, uint32_t ctr3, int exact) {
     uint32_t blk[4];
     uint32_t ctr[4] = {0u, channel, epoch, ctr3};
     int have = 0, pos = 0, first = 1;
@@ -160,6 +161,8 @@ static int32_t FN(poisson_stream)(REAL mean, uint32_t head, const uint32_t key[2
             if (val < lam) return n - 1;
         }
         return TLO_KNUTH_CAP;
+    } else if (exact) {
+        return tlo_ptrs((double)mean, head, key, channel, epoch, ctr3);
     } else {
         REAL s, u, v;
         do {
@@ -182,7 +185,7 @@ int32_t FN(tlo_poisson)(REAL mean, uint64_t seed, uint32_t model_id, int64_t rid
     uint32_t key[2];
     tlo_key(seed, rid, key);
     uint32_t head = tlo_head_draw(key, model_id << 2, epoch, channel);
-    return FN(poisson_stream)(mean, head, key, (uint32_t)channel, epoch, (model_id << 2) | TLO_KIND_POISSON);
+    return FN(poisson_stream)(mean, head, key, (uint32_t)channel, epoch, (model_id << 2) | TLO_KIND_POISSON, 0);
 }
 
 /* Critical pick (GetReaction2Fire, model.cu:555-598): upper_bound over the
@@ -224,7 +227,7 @@ static void FN(select_one)(const tlo_model *m, const uint32_t key[2], uint32_t c
 /* k_j for all channels (NumberOfFiringFn, model.cu:470-474). */
 static void FN(fire_one)(const tlo_model *m, const uint32_t key[2], uint32_t ctr3, uint32_t epoch,
                          const REAL *a, const uint8_t *crit, ptrdiff_t as, double tau, int32_t jc,
-                         int32_t *k, ptrdiff_t ks) {
+                         int32_t *k, ptrdiff_t ks, int exact) {
     REAL tr = (REAL)tau;
     for (int32_t j = 0; j < m->M; ++j) {
         int32_t kj;
@@ -232,7 +235,7 @@ static void FN(fire_one)(const tlo_model *m, const uint32_t key[2], uint32_t ctr
             kj = (j == jc) ? 1 : 0;
         else
             kj = FN(poisson_stream)(a[(ptrdiff_t)j * as] * tr, tlo_head_draw(key, ctr3, epoch, j), key, (uint32_t)j,
-                                    epoch, ctr3 | TLO_KIND_POISSON);
+                                    epoch, ctr3 | TLO_KIND_POISSON, exact);
         k[(ptrdiff_t)j * ks] = kj;
     }
 }
@@ -319,7 +322,7 @@ void FN(tlo_fire)(const tlo_model *m, int64_t R, uint64_t seed, uint32_t model_i
     for (int64_t r = 0; r < R; ++r) {
         uint32_t key[2];
         tlo_key(seed, rbase + r, key);
-        FN(fire_one)(m, key, model_id << 2, epoch[r], a + r, crit + r, R, tau[r], jc[r], k + r, R);
+        FN(fire_one)(m, key, model_id << 2, epoch[r], a + r, crit + r, R, tau[r], jc[r], k + r, R, 0);
     }
 }
 
@@ -347,8 +350,9 @@ void FN(tlo_ssa)(const tlo_model *m, int64_t R, uint64_t seed, uint32_t model_id
 }
 
 /* ---- likelihood-ratio score of one accepted leap ----
- * term_j = d_j * m_j with d_j = dlog p(k_j)/dm_j:  Poisson / critical:
- * k - m;  normal branch (m >= 80): y - m + ((y-m)^2/m - 1)/2 with y = k + 1/2.
+ * term_j = d_j * m_j with d_j = dlog p(k_j)/dm_j:  Poisson / critical (and
+ * every channel with the exact sampler): k - m;  normal branch (m >= 80):
+ * y - m + ((y-m)^2/m - 1)/2 with y = k + 1/2.
  * tau depends on c in two cases, each adding Q * dtau/dc_j with
  * Q = sum_l term_l / tau:
  *   - tau = tau1: through the argmin species' mean / variance bound; the same
@@ -358,14 +362,14 @@ void FN(tlo_ssa)(const tlo_model *m, int64_t R, uint64_t seed, uint32_t model_id
  * of the random variables). */
 static void FN(score_leap)(const tlo_model *m, const REAL *a, const uint8_t *crit, const int32_t *k,
                            double tau, int tau_from_tau1, int clamped, int32_t istar, int32_t which,
-                           double *score, double *dtime) {
+                           double *score, double *dtime, int exact) {
     const REAL *c = FN(rates)(m);
     double qsum = 0.0;
     REAL tr = (REAL)tau;
     for (int32_t j = 0; j < m->M; ++j) {
         double mj = (double)a[j] * tau;
         double term;
-        if (!crit[j] && (a[j] * tr) >= (REAL)TLO_POISSON_SWITCH) {
+        if (!exact && !crit[j] && (a[j] * tr) >= (REAL)TLO_POISSON_SWITCH) {
             double y = (double)k[j] + 0.5, d = y - mj;
             term = d + 0.5 * (d * d / mj - 1.0);
         } else {
@@ -483,7 +487,7 @@ static void FN(simulate_one)(const tlo_model *m, const tlo_params *p, int64_t ri
             int32_t jc;
             uint8_t clamped;
             FN(select_one)(m, key, ctr3, epoch, w->a, w->crit, 1, a0c, tau1, tf - t, &tau, &jc, &clamped);
-            FN(fire_one)(m, key, ctr3, epoch, w->a, w->crit, 1, tau, jc, w->k, 1);
+            FN(fire_one)(m, key, ctr3, epoch, w->a, w->crit, 1, tau, jc, w->k, 1, p->poisson_exact);
             epoch++;
             int32_t *xn = w->xn;
             memcpy(xn, x, sizeof(int32_t) * (size_t)N);
@@ -505,7 +509,7 @@ static void FN(simulate_one)(const tlo_model *m, const tlo_params *p, int64_t ri
             }
             if (score)
                 FN(score_leap)(m, w->a, w->crit, w->k, tau, (jc < 0 && !clamped), clamped, istar, which, score,
-                               w->dtime);
+                               w->dtime, p->poisson_exact);
             /* swap: x <- xn */
             double tn = clamped ? tf : t + tau;
             if (!(tn > t)) {

@@ -24,10 +24,12 @@ from . import models  # noqa: F401  (reaction networks and generators)
 
 __all__ = ["Model", "SimResult", "simulate", "batch_simulate", "mean_final_state", "lib", "TLError",
            "stage_propensities", "stage_tau", "stage_select", "stage_fire", "stage_update", "stage_ssa",
-           "FP64", "FP32", "MODE_AUTO", "MODE_PLAIN", "MODE_PERSISTENT", "models"]
+           "FP64", "FP32", "MODE_AUTO", "MODE_PLAIN", "MODE_PERSISTENT", "POISSON_NORMAL", "POISSON_EXACT",
+           "models"]
 
 FP64, FP32 = 0, 1
 MODE_AUTO, MODE_PLAIN, MODE_PERSISTENT = 0, 1, 2
+POISSON_NORMAL, POISSON_EXACT = 0, 1
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtau_leaping_amd.so")
@@ -45,7 +47,8 @@ class _Params(C.Structure):
                 ("seed", C.c_uint64), ("max_iter", C.c_int64), ("precision", C.c_int32),
                 ("n_track", C.c_int32), ("track", C.POINTER(C.c_int32)), ("n_intervals", C.c_int32),
                 ("sensitivities", C.c_int32), ("device", C.c_int32), ("mode", C.c_int32),
-                ("block_size", C.c_int32), ("model_id", C.c_uint32), ("realization_offset", C.c_int64)]
+                ("block_size", C.c_int32), ("model_id", C.c_uint32), ("realization_offset", C.c_int64),
+                ("poisson", C.c_int32)]
 
 
 class _Result(C.Structure):
@@ -126,6 +129,15 @@ def _prec(p):
     if p in (FP32, "fp32", "f32", "float", np.float32):
         return FP32
     raise ValueError(f"unknown precision {p!r}")
+
+
+def _poisson(p):
+    """Firing counts above mean 80: 'normal' (the reference's |N(m, m)|) or
+    'exact' (PTRS)."""
+    v = {"normal": POISSON_NORMAL, "exact": POISSON_EXACT}.get(p, p)
+    if v not in (POISSON_NORMAL, POISSON_EXACT):
+        raise ValueError(f"unknown Poisson sampler {p!r}")
+    return v
 
 
 def _mode(m):
@@ -257,7 +269,7 @@ class Model:
 
     def params(self, t_final, n_realizations, eps=0.03, seed=1234, precision="fp64", track=None, n_intervals=0,
                sensitivities=False, max_iter=0, mode="auto", block_size=0, device=0, model_id=0,
-               realization_offset=0):
+               realization_offset=0, poisson="normal"):
         p = _Params()
         lib().tl_sim_params_default(C.byref(p))
         p.t_final = float(t_final)
@@ -277,6 +289,7 @@ class Model:
         p.block_size = int(block_size)
         p.model_id = int(model_id)
         p.realization_offset = int(realization_offset)
+        p.poisson = _poisson(poisson)
         return p, trk
 
     def simulate(self, t_final, n_realizations, **kw) -> SimResult:

@@ -7,7 +7,9 @@
 //   -i <logging intervals, 1>
 // and writes mean_<log>.txt / stddev_<log>.txt with lines "t v1 v2 ..."
 // (StatToFile, statistics.cpp:89-111).  Extensions: --precision fp64|fp32,
-// --seed N, --mode persistent|plain, --block-size N, --sensitivities (writes
+// --seed N, --poisson normal|exact (exact: PTRS instead of the normal
+// approximation above mean 80), --mode auto|persistent|plain (auto: persistent, plain for a model
+// whose state fits no persistent launch shape), --block-size N, --sensitivities (writes
 // dmean_<log>.txt: one line per tracked species, dE[X(t_final)]/dc_j),
 // --json (one JSON record of the run on stdout).
 //
@@ -46,7 +48,8 @@ struct Args {
     int precision = TL_FP64;
     unsigned long long seed = 0;
     bool seed_set = false;
-    int mode = TL_MODE_PERSISTENT;
+    int mode = TL_MODE_AUTO;
+    int poisson = TL_POISSON_NORMAL;
     int block_size = 0;
     bool sens = false;
     bool json = false;
@@ -56,8 +59,8 @@ struct Args {
     if (msg) std::cerr << msg << "\n";
     std::cerr << "usage: tau_leaping -f model.xml -t t_end [-n realizations] [-r names...|all] [-e eps]\n"
                  "                   [-a max_iterations] [-d device] [-b log_name] [-i intervals]\n"
-                 "                   [--precision fp64|fp32] [--seed N] [--mode persistent|plain]\n"
-                 "                   [--block-size N] [--sensitivities] [--json]\n";
+                 "                   [--precision fp64|fp32] [--seed N] [--mode auto|persistent|plain]\n"
+                 "                   [--poisson normal|exact] [--block-size N] [--sensitivities] [--json]\n";
     std::exit(EXIT_FAILURE);
 }
 
@@ -100,9 +103,15 @@ Args parse(int argc, char **argv) {
             A.seed_set = true;
         } else if (k == "--mode") {
             const std::string v = need(argc, argv, a);
-            if (v == "persistent") A.mode = TL_MODE_PERSISTENT;
+            if (v == "auto") A.mode = TL_MODE_AUTO;
+            else if (v == "persistent") A.mode = TL_MODE_PERSISTENT;
             else if (v == "plain") A.mode = TL_MODE_PLAIN;
-            else usage("mode must be persistent or plain");
+            else usage("mode must be auto, persistent or plain");
+        } else if (k == "--poisson") {
+            const std::string v = need(argc, argv, a);
+            if (v == "normal") A.poisson = TL_POISSON_NORMAL;
+            else if (v == "exact") A.poisson = TL_POISSON_EXACT;
+            else usage("poisson must be normal or exact");
         } else if (k == "--block-size") A.block_size = atoi(need(argc, argv, a));
         else if (k == "--sensitivities") A.sens = true;
         else if (k == "--json") A.json = true;
@@ -176,6 +185,7 @@ int main(int argc, char **argv) {
     p.device = A.device;
     p.mode = A.mode;
     p.block_size = A.block_size;
+    p.poisson = A.poisson;
 
     const int T = p.n_track;
     const int G = p.n_intervals > 0 ? p.n_intervals + 1 : 1;
@@ -241,7 +251,8 @@ int main(int argc, char **argv) {
         std::cout << "{\"file\": \"" << A.file << "\", \"species\": " << N << ", \"channels\": " << M
                   << ", \"realizations\": " << A.n << ", \"t_final\": " << A.t_end
                   << ", \"precision\": \"" << (A.precision == TL_FP64 ? "fp64" : "fp32") << "\", \"mode\": \""
-                  << (A.mode == TL_MODE_PLAIN ? "plain" : "persistent") << "\", \"seed\": " << p.seed
+                  << (A.mode == TL_MODE_PLAIN ? "plain" : (A.mode == TL_MODE_AUTO ? "auto" : "persistent"))
+                  << "\", \"seed\": " << p.seed
                   << ", \"total_steps\": " << r.total_steps << ", \"ssa_events\": " << r.total_ssa_events
                   << ", \"rejections\": " << r.total_rejections << ", \"failed\": " << r.n_failed
                   << ", \"kernel_ms\": " << r.kernel_ms << ", \"wall_s\": " << wall

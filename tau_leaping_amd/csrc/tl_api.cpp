@@ -273,6 +273,8 @@ tl_status check_params(const tl_model *m, const tl_sim_params *p) {
         if (p->track[q] < 0 || p->track[q] >= m->N) return set_error(TL_ERR_INVALID_ARG, "tracked species out of range");
     if (p->n_intervals < 0) return set_error(TL_ERR_INVALID_ARG, "n_intervals must be >= 0");
     if (p->model_id >= (1u << 30)) return set_error(TL_ERR_INVALID_ARG, "model_id must be < 2^30");
+    if (p->poisson != TL_POISSON_NORMAL && p->poisson != TL_POISSON_EXACT)
+        return set_error(TL_ERR_INVALID_ARG, "poisson must be TL_POISSON_NORMAL or TL_POISSON_EXACT");
     return TL_OK;
 }
 
@@ -400,7 +402,7 @@ tl_status run_plain(const tl_model *m, const tl_sim_params *p, tl_result *res) {
         launch_plain_select(is64, dm, R, seed, p->model_id, p->realization_offset, p->t_final, p->max_iter, x, a, crit,
                             a0, a0c, tau1, st, s);
         launch_fire(is64, dm, R, seed, p->model_id, p->realization_offset, st.epoch, a, crit, st.tau, st.jc, k, st.mode,
-                    s);
+                    p->poisson == TL_POISSON_EXACT, s);
         launch_update(dm, R, x, k, xn, st.neg, st.mode, s);
         launch_plain_commit(dm, R, x, xn, st, p->t_final, s);
         if ((it & 15) == 15) {
@@ -801,6 +803,7 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
             mi.tlast = e.d_tlast;
             mi.n_intervals = e.p->n_intervals;
             mi.grid = e.d_grid;
+            mi.poisson_exact = e.p->poisson == TL_POISSON_EXACT ? 1 : 0;
             base += mi.R;
         }
         const int64_t n_items = base;
@@ -1059,7 +1062,7 @@ tl_status tl_stage_fire(const tl_model *m, int32_t precision, int64_t R, uint64_
     DevModel dm;
     TL_TRY(stage_setup(m, precision, R, dc, dm));
     if (!epoch || !a || !crit || !tau || !jc || !k) return set_error(TL_ERR_INVALID_ARG, "null buffer");
-    launch_fire(precision == TL_FP64, dm, R, seed, model_id, realization_offset, epoch, a, crit, tau, jc, k, nullptr,
+    launch_fire(precision == TL_FP64, dm, R, seed, model_id, realization_offset, epoch, a, crit, tau, jc, k, nullptr, false,
                 dc->stream);
     return stage_finish(dc);
 }

@@ -154,8 +154,48 @@ __device__ __forceinline__ uint32_t quad_pick(uint4 o, int s) {
     return s == 0 ? o.x : (s == 1 ? o.y : (s == 2 ? o.z : o.w));
 }
 
+// Exact Poisson sample for mean >= 10: PTRS, the transformed rejection with
+// squeeze of W. Hoermann, "The transformed rejection method for generating
+// Poisson random variables", Insurance: Mathematics and Economics 12 (1993)
+// 39-45, in FP64 over FP64 uniforms of the channel's draws.  Attempt i takes
+// draws 2i+1 and 2i+2 (draw 1 = head; draw n >= 2 as in poisson_draw):
+// U = u - 1/2 from the first, V from the second.  After 64 rejected attempts
+// (probability below 1e-60) it returns floor(mean), so the loop always ends.
+__device__ __forceinline__ int32_t poisson_ptrs(double mean, uint32_t head, uint2 key, uint32_t channel,
+                                                uint32_t epoch, uint32_t ctr3) {
+    const double smu = sqrt(mean);
+    const double b = 0.931 + 2.53 * smu;
+    const double a = -0.059 + 0.02483 * b;
+    const double lia = log(1.1239 + 1.1328 / (b - 3.4));
+    const double vr = 0.9277 - 3.6224 / (b - 2.0);
+    const double lmu = log(mean);
+    uint32_t blk = 0;
+    uint4 o = make_uint4(0u, 0u, 0u, 0u);
+    int pos = 4;
+    auto next = [&]() -> uint32_t {
+        if (pos == 4) {
+            o = philox4x32_10(make_uint4(blk++, channel, epoch, ctr3), key);
+            pos = 0;
+        }
+        return quad_pick(o, pos++);
+    };
+    uint32_t ub = head;
+    for (int attempt = 0; attempt < 64; ++attempt) {
+        if (attempt > 0) ub = next();
+        const uint32_t vb = next();
+        const double U = unif64(ub) - 0.5, V = unif64(vb);
+        const double us = 0.5 - fabs(U);
+        const double k = floor((2.0 * a / us + b) * U + mean + 0.43);
+        if (us >= 0.07 && V <= vr) return (int32_t)k;
+        if (k < 0.0 || (us < 0.013 && V > us)) continue;
+        if (log(V) + lia - log(a / (us * us) + b) <= -mean + k * lmu - lgamma(k + 1.0)) return (int32_t)k;
+    }
+    return (int32_t)mean;
+}
+
 // Firing count for a non-critical channel: Knuth product below 80 (capped at
-// 300), else |mean + sqrt(mean) * z| truncated, z from the polar method.
+// 300), else |mean + sqrt(mean) * z| truncated, z from the polar method, or
+// with `exact` the PTRS sample above.
 // Draw 1 of the channel is `head`; draw n >= 2 is output ((n-2) & 3) of the
 // channel's own Philox block (n-2) >> 2 at counter (block, channel, epoch,
 // ctr3).  The Knuth loop returns 0 on its first draw without evaluating exp
@@ -163,7 +203,7 @@ __device__ __forceinline__ uint32_t quad_pick(uint4 o, int s) {
 // result is the one the full loop returns).
 template <class R>
 __device__ __forceinline__ int32_t poisson_draw(R mean, uint32_t head, uint2 key, uint32_t channel,
-                                                uint32_t epoch, uint32_t ctr3) {
+                                                uint32_t epoch, uint32_t ctr3, bool exact) {
     uint32_t blk = 0;
     const R u1 = Real<R>::unif(head);
     if (mean < (R)kPoissonSwitch) {
@@ -188,6 +228,8 @@ __device__ __forceinline__ int32_t poisson_draw(R mean, uint32_t head, uint2 key
             if (val < lam) return n - 1;
             if (++n == kKnuthCap) return kKnuthCap;
         }
+    } else if (exact) {
+        return poisson_ptrs((double)mean, head, key, channel, epoch, ctr3);
     } else {
         // pairs (head, t0), (t1, t2), (t3, t4), ... of the channel's draws
         R u = 2 * u1 - 1;

@@ -898,9 +898,9 @@ __device__ __forceinline__ void prop_update(const GModel &m, gptr<const R> c, co
 // Score term d_j * m_j of one channel (see the likelihood-ratio notes in
 // simulate_item).
 template <class R>
-__device__ __forceinline__ double score_term(bool cr, R aj, R tr, double tau, int32_t kj) {
+__device__ __forceinline__ double score_term(bool cr, R aj, R tr, double tau, int32_t kj, bool exact) {
     const double mj = (double)aj * tau;
-    if (!cr && aj * tr >= (R)kPoissonSwitch) {
+    if (!exact && !cr && aj * tr >= (R)kPoissonSwitch) {
         const double d = ((double)kj + 0.5) - mj;
         return d + 0.5 * (d * d / mj - 1.0);
     }
@@ -916,7 +916,8 @@ __device__ __forceinline__ double score_term(bool cr, R aj, R tr, double tau, in
 // recomputed propensity is not evaluated for them).
 template <class R, int GS, bool CRIT_A, class P, class F>
 __device__ __forceinline__ void for_each_firing(const GModel &m, const Grp<GS> &G, const P &pa, const uint64_t *critq,
-                                                uint2 key, uint32_t epoch, uint32_t c3, R tr, int32_t jfire, F &&f) {
+                                                uint2 key, uint32_t epoch, uint32_t c3, R tr, int32_t jfire, bool exact,
+                                                F &&f) {
     const uint32_t ck = c3 | kKindPoisson;
     for (int32_t c0 = 0; c0 < m.M; c0 += 4 * GS) {
         const int32_t jb = c0 + 4 * G.lane;
@@ -941,7 +942,8 @@ __device__ __forceinline__ void for_each_firing(const GModel &m, const Grp<GS> &
                 else
                     aj = i == 0 ? a4[0] : (i == 1 ? a4[1] : (i == 2 ? a4[2] : a4[3]));
                 const int32_t kj =
-                    cr ? (j == jfire ? 1 : 0) : poisson_draw<R>(aj * tr, quad_pick(h, i), key, (uint32_t)j, epoch, ck);
+                    cr ? (j == jfire ? 1 : 0)
+                       : poisson_draw<R>(aj * tr, quad_pick(h, i), key, (uint32_t)j, epoch, ck, exact);
                 f(j, cr, aj, kj);
             }
         }
@@ -991,6 +993,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
     const uint32_t c3 = mi.model_id << 2;
     const double tf = mi.t_final;
     const R eps = (R)mi.eps;
+    const bool exact = mi.poisson_exact != 0;
     // Likelihood-ratio score S_j = dlog p(path)/dc_j, and D_j = dt/dc_j of the
     // current time (nonzero through tau1-limited leaps; used by the clamped
     // last leap and the truncated last direct interval).
@@ -1197,7 +1200,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
             const int32_t jfire = fire ? jc : -1;
             const R tr = (R)tau;
             double qpart = 0.0;
-            for_each_firing<R, GS, SENS>(m, G, pa, critq, key, epoch, c3, tr, jfire,
+            for_each_firing<R, GS, SENS>(m, G, pa, critq, key, epoch, c3, tr, jfire, exact,
                                    [&](int32_t j, bool cr, R aj, int32_t kj) {
                                        if (kj) {
                                            if (m.colq) {
@@ -1212,7 +1215,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                                            }
                                        }
                                        if (SENS) {
-                                           const double term = score_term<R>(cr, aj, tr, tau, kj);
+                                           const double term = score_term<R>(cr, aj, tr, tau, kj, exact);
                                            qpart += term;
                                            if (c[j] != (R)0) S[j] += term / (double)c[j];
                                        }
@@ -1224,9 +1227,9 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
             if (neg) {
                 for (int32_t i = lane; i < m.N; i += GS) dx[i] = 0;
                 if (SENS)  // take back this attempt's score terms (same draws)
-                    for_each_firing<R, GS, SENS>(m, G, pa, critq, key, epoch, c3, tr, jfire,
+                    for_each_firing<R, GS, SENS>(m, G, pa, critq, key, epoch, c3, tr, jfire, exact,
                                            [&](int32_t j, bool cr, R aj, int32_t kj) {
-                                               if (c[j] != (R)0) S[j] -= score_term<R>(cr, aj, tr, tau, kj) / (double)c[j];
+                                               if (c[j] != (R)0) S[j] -= score_term<R>(cr, aj, tr, tau, kj, exact) / (double)c[j];
                                            });
                 epoch++;
                 nrej++;

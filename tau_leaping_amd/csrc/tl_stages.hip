@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kT) void k_fire(DevModel m, int64_t Rn, uint64_t se
                                              const uint32_t *__restrict__ epoch, const R *__restrict__ a,
                                              const uint8_t *__restrict__ crit, const double *__restrict__ tau,
                                              const int32_t *__restrict__ jc, int32_t *__restrict__ k,
-                                             const uint8_t *__restrict__ mode) {
+                                             const uint8_t *__restrict__ mode, int exact) {
     const int64_t rt = (Rn + kT - 1) / kT;
     const int32_t j = (int32_t)(blockIdx.x / rt);
     const int64_t r = (int64_t)(blockIdx.x % rt) * kT + threadIdx.x;
@@ -187,7 +187,8 @@ __global__ __launch_bounds__(kT) void k_fire(DevModel m, int64_t Rn, uint64_t se
     } else {
         const uint2 key = rng_key(seed, rbase + r);
         const uint32_t head = quad_pick(head_block(key, (uint32_t)j >> 2, epoch[r], model_id << 2), j & 3);
-        kj = poisson_draw<R>(a[o] * (R)tau[r], head, key, (uint32_t)j, epoch[r], (model_id << 2) | kKindPoisson);
+        kj = poisson_draw<R>(a[o] * (R)tau[r], head, key, (uint32_t)j, epoch[r], (model_id << 2) | kKindPoisson,
+                             exact != 0);
     }
     k[o] = kj;
 }
@@ -480,14 +481,14 @@ void launch_select(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint3
 
 void launch_fire(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase,
                  const uint32_t *epoch, const void *a, const uint8_t *crit, const double *tau, const int32_t *jc,
-                 int32_t *k, const uint8_t *mode, hipStream_t s) {
+                 int32_t *k, const uint8_t *mode, bool exact, hipStream_t s) {
     const int64_t nb = (int64_t)blocks_for(R) * m.M;
     if (is64)
         k_fire<double><<<(unsigned)nb, kT, 0, s>>>(m, R, seed, model_id, rbase, epoch, (const double *)a, crit, tau, jc,
-                                                   k, mode);
+                                                   k, mode, exact ? 1 : 0);
     else
         k_fire<float><<<(unsigned)nb, kT, 0, s>>>(m, R, seed, model_id, rbase, epoch, (const float *)a, crit, tau, jc,
-                                                  k, mode);
+                                                  k, mode, exact ? 1 : 0);
 }
 
 void launch_update(const DevModel &m, int64_t R, const int32_t *x, const int32_t *k, int32_t *x_out, uint8_t *neg,

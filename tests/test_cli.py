@@ -78,3 +78,36 @@ def test_simulator_without_gpu_fails_loudly(tmp_path):
                        cwd=str(tmp_path))
     assert r.returncode != 0
     assert "HIP" in (r.stdout + r.stderr)
+
+
+@pytest.mark.gpu
+def test_simulator_end_to_end(tmp_path, cuda):
+    """tau_leaping -f -t -n -r -i -b on the dimerisation-decay model
+    (S1 = 100,000): the mean_/stddev_ files hold the grid moments the Python
+    binding returns for the same seed, and the moments at t_final agree with
+    the oracle's within their standard errors."""
+    import json
+
+    from helpers import oracle_of
+
+    net = models.dimer_decay(100000)
+    path = tmp_path / "dimer.xml"
+    path.write_text(net.to_stochkit_xml())
+    R, tf = 4000, 10.0
+    r = subprocess.run([SIM, "-f", str(path), "-t", str(tf), "-n", str(R), "-r", "S1", "S3", "-i", "5",
+                        "--seed", "1234", "-b", "run", "--json"], capture_output=True, text=True, timeout=300,
+                       cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    rec = json.loads(r.stdout.strip().splitlines()[-1])
+    assert rec["failed"] == 0 and rec["realizations"] == R and rec["mode"] == "auto"
+    mean = np.loadtxt(tmp_path / "mean_run.txt")
+    sd = np.loadtxt(tmp_path / "stddev_run.txt")
+    assert mean.shape == (6, 3) and sd.shape == (6, 3)
+    np.testing.assert_allclose(mean[:, 0], np.linspace(0, tf, 6))
+    m = T.Model.from_stochkit(str(path))
+    res = m.simulate(tf, R, seed=1234, track=[0, 2], n_intervals=5)
+    np.testing.assert_allclose(mean[:, 1:], res.mean, rtol=1e-5, atol=1e-3)  # six significant digits in the files
+    np.testing.assert_allclose(sd[:, 1:], res.stddev, rtol=1e-5, atol=1e-3)
+    ora = oracle_of(net).simulate(R, tf, seed=1234)["x_final"][:, [0, 2]].astype(float)
+    se = np.sqrt(sd[-1, 1:] ** 2 / R + ora.var(0, ddof=1) / R)
+    assert np.all(np.abs(mean[-1, 1:] - ora.mean(0)) < 3.3 * se)

@@ -173,6 +173,44 @@ def test_sources_match_oracle(cuda, prec, mode, bs):
     assert np.array_equal(res.steps[same], ora["steps"][same])
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("mode,bs", [("persistent", 64), ("persistent", 1024), ("plain", 0)])
+def test_exact_sampler_matches_oracle(cuda, prec, mode, bs):
+    # dimerisation-decay from S1 = 100,000: firing means far above 80, so
+    # every leap draws through PTRS in the exact option
+    m, om = pair(models.dimer_decay(100000))
+    R, tf = 1024, 2.0
+    res = m.simulate(tf, R, seed=77, precision=prec, mode=mode, block_size=bs, poisson="exact")
+    ora = om.simulate(R, tf, seed=77, precision=prec, poisson="exact")
+    assert res.n_failed == 0 and (ora["status"] == 0).all()
+    same = _compare(res.final_x, ora["x_final"])
+    assert np.array_equal(res.steps[same], ora["steps"][same])
+    normal = m.simulate(tf, R, seed=77, precision=prec, mode=mode, block_size=bs)
+    assert not np.array_equal(normal.final_x, res.final_x)  # the option changes the draws
+
+
+def test_exact_sampler_law(cuda):
+    # a source at rate 1e4 from x0 = 1e6: tau1 = eps*x0/c = 3 > t_final, so the
+    # run is one leap clamped to t_final and x - x0 is one firing count of
+    # mean 500.  Exact option: Poisson(500) by chi-square; the default keeps
+    # the reference's floor(|N(500, 500)|).
+    from scipy import stats
+
+    from tau_leaping_amd.models import Network, Reaction
+    m = T.Model.from_network(Network(1, [Reaction({}, {0: 1}, 1e4)], np.array([10 ** 6], np.int32)))
+    R = 40000
+    res = m.simulate(0.05, R, seed=5, poisson="exact")
+    assert (res.steps == 1).all()
+    k = res.final_x[:, 0].astype(np.int64) - 10 ** 6
+    support = np.arange(380, 630)
+    pmf = stats.poisson.pmf(support, 500.0)
+    obs = np.array([(k == v).sum() for v in support], float)
+    exp_ = np.append(pmf * R, R - (pmf * R).sum())
+    obs = np.append(obs, R - obs.sum())
+    assert stats.chisquare(obs, exp_).pvalue > 1e-3
+    assert abs(k.mean() - 500) < 4 * np.sqrt(500 / R)
+
+
 def test_conservation_laws(cuda):
     # cyclic chain: sum_i x_i constant; colloid: sum_i i * x_i constant
     cyc = models.cyclic_chain(256, seed=1)

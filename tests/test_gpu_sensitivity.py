@@ -57,6 +57,22 @@ def test_source_scores(cuda, bs):
     assert np.all(np.abs(res.dmean_dc[0] - exact) < 4.5 * res.dmean_dc_se[0]), (res.dmean_dc, exact)
 
 
+def test_exact_sampler_scores(cuda):
+    # one clamped leap of a source (rate c, mean m = c*t_final = 500): with
+    # the exact sampler the score is (k - m)/c, so dE[X]/dc = Var(k)/c =
+    # t_final exactly; scores equal the oracle's realization by realization
+    from tau_leaping_amd.models import Network, Reaction
+    net = Network(1, [Reaction({}, {0: 1}, 1e4)], np.array([10 ** 6], np.int32))
+    m, om = pair(net)
+    R = 16384
+    res = m.simulate(0.05, R, seed=6, poisson="exact", sensitivities=True, return_scores=True)
+    ora = om.simulate(R, 0.05, seed=6, poisson="exact", sensitivities=True)
+    same = np.all(res.final_x == ora["x_final"], axis=1)
+    assert same.mean() >= 0.97
+    assert np.abs(res.score[same] - ora["score"][same]).max() < 1e-12
+    assert abs(res.dmean_dc[0, 0] - 0.05) < 4.5 * res.dmean_dc_se[0, 0]
+
+
 def test_dmean_matches_closed_form_and_fd(cuda):
     N, tf = 8, 0.5
     rng = np.random.default_rng(4)

@@ -278,6 +278,34 @@ def test_normal_branch_law(mean):
 
 
 # ---------------------------------------------------- cyclic chain closed form --
+def _chi2_poisson(x, mean):
+    """Chi-square p-value of integer samples against Poisson(mean), bins with
+    at least 5 expected counts and the two tails pooled."""
+    lo, hi = int(stats.poisson.ppf(1e-9, mean)), int(stats.poisson.isf(1e-9, mean)) + 1
+    support = np.arange(lo, hi)
+    pmf = stats.poisson.pmf(support, mean)
+    obs = np.array([(x == v).sum() for v in support], float) if support.size < 4000 else \
+        np.bincount(x - lo, minlength=support.size)[:support.size].astype(float)
+    keep = pmf * x.size >= 5
+    exp_ = np.append(pmf[keep] * x.size, x.size - (pmf[keep] * x.size).sum())
+    obs_k = np.append(obs[keep], x.size - obs[keep].sum())
+    return stats.chisquare(obs_k, exp_).pvalue
+
+
+@pytest.mark.parametrize("mean", [10.0, 37.5, 80.0, 500.0, 1e4, 1e6])
+def test_ptrs_is_poisson(mean):
+    """The exact sampler option (PTRS, Hoermann 1993) against the Poisson law
+    itself: chi-square over the pmf and the first two moments, on the draws of
+    400,000 realizations' streams (an off-by-one on the squeeze-accepted
+    path, which moves the mean at 10 by 0.45, fails the mean test)."""
+    om = oracle_of(Network(1, [Reaction({0: 1}, {}, 1.0)], np.array([1], np.int32)))
+    x = om.poisson_ptrs(mean, 99, 0, 3, 5, n=400000)
+    assert x.min() >= 0
+    assert _chi2_poisson(x, mean) > 1e-3
+    assert abs(x.mean() - mean) < 4 * np.sqrt(mean / x.size)
+    assert abs(x.var() / mean - 1) < 0.015
+
+
 def _immigration_death(k, g, s0):
     return Network(1, [Reaction({}, {0: 1}, k), Reaction({0: 1}, {}, g)], np.array([s0], np.int32))
 
