@@ -497,7 +497,12 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
         }
     if (out.empty()) {
         // nothing fits LDS: 1024-lane groups with their state in global
-        // memory, the largest propensity tile that fits
+        // memory, either with recomputed propensities and the largest LDS
+        // tile that fits (one pass over the species rows per tile), or with
+        // a[M] stored in global memory too (one pass over the rows).  Measured
+        // (2e4 x 1e5 and 2e5 x 1e6 toy networks): recomputed wins at 7 tiles
+        // (2.8e5 vs 1.8e5 leaps/s), stored at 62 (8.7e3 vs 5.0e3); the
+        // preferred one comes first.
         PersistentConfig c{1024, 1024, false, 1};
         c.hbm = true;
         c.tile_grains = 4;
@@ -506,7 +511,17 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
             c.tile_grains /= 2;
             lds = persistent_lds_bytes(c, is64, n_max, m_max);
         }
-        if (lds <= dc.lds_per_block && persistent_occupancy(c, is64, sens, lds) > 0) out.push_back(c);
+        const bool c_ok = lds <= dc.lds_per_block && persistent_occupancy(c, is64, sens, lds) > 0;
+        PersistentConfig st{1024, 1024, true, 1};
+        st.hbm = true;
+        st.tile_grains = std::max(1, (m_max + kTileGrain - 1) / kTileGrain);
+        lds = persistent_lds_bytes(st, is64, n_max, m_max);
+        const bool st_ok = lds <= dc.lds_per_block && persistent_occupancy(st, is64, sens, lds) > 0;
+        const int64_t tiles = ((int64_t)m_max + (int64_t)c.tile_grains * kTileGrain - 1) / ((int64_t)c.tile_grains * kTileGrain);
+        const bool stored_first = tiles > 16;
+        if (st_ok && stored_first) out.push_back(st);
+        if (c_ok) out.push_back(c);
+        if (st_ok && !stored_first) out.push_back(st);
     }
     return out;
 }
@@ -644,6 +659,11 @@ tl_status autotune(DeviceContext &dc, bool is64, bool sens, const std::vector<Mo
     auto all = candidates(dc, is64, sens, n_max, m_max);
     if (all.empty()) return set_error(TL_ERR_TOO_LARGE, "model too large for any persistent launch shape");
     best = heuristic(all, m_max);
+    if (best.hbm) {  // global-memory state shapes are not timed (no trial launches)
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tuned[key] = best;
+        return TL_OK;
+    }
     // Timed candidates: the default one-wavefront shape and the block shapes.
     // Short trials do not separate one-wavefront variants reliably (their
     // first steps are not representative); the default among them was the

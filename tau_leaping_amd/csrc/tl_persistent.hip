@@ -137,7 +137,9 @@ __host__ __device__ inline GroupLayout group_layout(int32_t n_max, int32_t m_max
 // LDS of a group whose state is in global memory: reduction scratch, then the
 // propensity tile.
 __host__ __device__ inline size_t hbm_tile_offset(int gs) { return al16((size_t)(4 * (gs / 64) + 4) * 8); }
-__host__ __device__ inline size_t hbm_lds_bytes(int32_t m_max, int gs, size_t real_bytes, int tile_grains) {
+__host__ __device__ inline size_t hbm_lds_bytes(int32_t m_max, int gs, bool a_stored, size_t real_bytes,
+                                                int tile_grains) {
+    if (a_stored) return hbm_tile_offset(gs);  // a[M] is in the global slice: no tile
     const int32_t tile_ch = tile_grains * kTileGrain;
     return hbm_tile_offset(gs) + al16((size_t)(m_max < tile_ch ? m_max : tile_ch) * real_bytes);
 }
@@ -1312,14 +1314,15 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
 // WPE: waves per SIMD the register allocation targets (more waves hide more
 // latency; fewer give each wave more registers).
 // HBM: the group's state (populations, proposal, snapshot, criticality words,
-// species sums, word sums) lives in a slice of global memory per block
-// instead of LDS, for models whose state does not fit LDS; only the
-// reduction scratch and the propensity tile stay in LDS.  Block groups only:
+// species sums, word sums, and a[M] with ALDS) lives in a slice of global
+// memory per block instead of LDS, for models whose state does not fit LDS;
+// only the reduction scratch (and without ALDS the propensity tile) stays in
+// LDS.  Block groups only:
 // their barriers are full workgroup barriers, which order global memory
 // between the waves of a block as they order LDS.
 template <class R, int BS, int GS, bool ALDS, bool SENS, int WPE, bool HBM>
 __global__ __launch_bounds__(BS, WPE) void k_persistent(PersistentLaunch L) {
-    static_assert(!HBM || (GS == BS && GS > 64 && !ALDS), "HBM state: block groups with recomputed propensities");
+    static_assert(!HBM || (GS == BS && GS > 64), "HBM state: block groups");
     extern __shared__ __align__(16) unsigned char lds[];
     const GroupLayout lay = group_layout(L.n_max, L.m_max, GS, ALDS, sizeof(R), L.tile_grains);
     const int tid = threadIdx.x;
@@ -1365,8 +1368,11 @@ __global__ __launch_bounds__(BS, WPE) void k_persistent(PersistentLaunch L) {
 // ---------------------------------------------------------- dispatch ----
 template <class R, bool SENS>
 const void *kernel_ptr(int bs, int gs, bool alds, int wpe, bool hbm) {
-    if (hbm) return (bs == 1024 && gs == 1024 && !alds && wpe == 1)
-                        ? (const void *)k_persistent<R, 1024, 1024, false, SENS, 1, true> : nullptr;
+    if (hbm) {
+        if (bs != 1024 || gs != 1024 || wpe != 1) return nullptr;
+        return alds ? (const void *)k_persistent<R, 1024, 1024, true, SENS, 1, true>
+                    : (const void *)k_persistent<R, 1024, 1024, false, SENS, 1, true>;
+    }
 #define TL_K(BS_, GS_, W_)                                                                             \
     if (bs == BS_ && gs == GS_ && wpe == W_)                                                           \
         return alds ? (const void *)k_persistent<R, BS_, GS_, true, SENS, W_, false>                   \
@@ -1436,14 +1442,14 @@ __global__ __launch_bounds__(256) void k_grad(int64_t Rn, int32_t T, int32_t M, 
 }  // namespace
 
 size_t persistent_lds_bytes(const PersistentConfig &c, bool is64, int32_t n_max, int32_t m_max) {
-    if (c.hbm) return hbm_lds_bytes(m_max, c.group, is64 ? 8 : 4, c.tile_grains);
+    if (c.hbm) return hbm_lds_bytes(m_max, c.group, c.a_lds, is64 ? 8 : 4, c.tile_grains);
     const GroupLayout L = group_layout(n_max, m_max, c.group, c.a_lds, is64 ? 8 : 4, c.tile_grains);
     return L.bytes * (size_t)(c.block / c.group);
 }
 
 size_t persistent_state_bytes(const PersistentConfig &c, bool is64, int32_t n_max, int32_t m_max) {
     if (!c.hbm) return 0;
-    const size_t b = group_layout(n_max, m_max, c.group, false, is64 ? 8 : 4, c.tile_grains).bytes;
+    const size_t b = group_layout(n_max, m_max, c.group, c.a_lds, is64 ? 8 : 4, c.tile_grains).bytes;
     return (b + 255) & ~(size_t)255;
 }
 

@@ -79,3 +79,38 @@ def cyclic_mean_jacobian(c, x0, T):
         _, dP = linalg.expm_frechet(T * Lg, T * E)
         J[:, j] = np.asarray(x0, float) @ dP
     return J
+
+
+def toy_csc(N, M, seed):
+    """CSC arrays of a random toy network (the benchmark recipe: 1-2 distinct
+    reactants with coefficient 1, 1-2 distinct products, all different
+    species; rates log-uniform in [1e-3, 1]; populations uniform in
+    [0, 10000]), built with numpy so that 1e5-1e6 channels take a second."""
+    rng = np.random.default_rng(seed)
+    nr = rng.integers(1, 3, M)
+    npd = rng.integers(1, 3, M)
+    pick = rng.integers(0, N, (M, 4))
+    while True:  # redraw rows whose four species are not distinct
+        srt = np.sort(pick, axis=1)
+        bad = np.any(srt[:, 1:] == srt[:, :-1], axis=1)
+        if not bad.any():
+            break
+        pick[bad] = rng.integers(0, N, (int(bad.sum()), 4))
+    col_ptr = np.zeros(M + 1, np.int64)
+    col_ptr[1:] = np.cumsum(nr + npd)
+    rows, vals = [], []
+    for k in range(4):  # slot k: reactant if k < nr, product if nr <= k < nr + np
+        use = k < nr + npd
+        rows.append(np.where(use, pick[:, k], -1))
+        vals.append(np.where(k < nr, -1, 1))
+    rows = np.stack(rows, 1)
+    vals = np.stack(vals, 1)
+    order = np.argsort(np.where(rows < 0, N, rows), axis=1, kind="stable")  # species order within a column
+    rows = np.take_along_axis(rows, order, 1)
+    vals = np.take_along_axis(vals, order, 1)
+    keep = rows >= 0
+    row_ind = rows[keep].astype(np.int32)
+    stoich = vals[keep].astype(np.int8)
+    rates = np.exp(rng.uniform(np.log(1e-3), 0.0, M))
+    x0 = rng.integers(0, 10001, N).astype(np.int32)
+    return col_ptr.astype(np.int32), row_ind, stoich, rates, x0

@@ -149,6 +149,25 @@ def test_model_larger_than_lds(cuda, prec):
         assert np.array_equal(r.steps[same], ora["steps"][same])
 
 
+def test_model_with_many_tiles_stores_a_in_hbm(cuda):
+    # 6000 species x 300,000 channels: 19 propensity tiles, so the HBM-state
+    # kernel keeps a[M] in global memory and sums the species rows in one
+    # pass; trajectories follow the oracle's
+    import tl_oracle as O
+    from helpers import toy_csc
+
+    csc = toy_csc(6000, 300000, 7)
+    m = T.Model.from_csc(6000, *csc)
+    om = O.OracleModel(6000, 300000, *csc)
+    R, tf = 32, 1e-6
+    track = np.arange(0, 6000, 100)
+    res = m.simulate(tf, R, seed=5, track=track)
+    ora = om.simulate(R, tf, seed=5)
+    assert res.block_size == 1024 and res.n_failed == 0 and (ora["status"] == 0).all()
+    same = _compare(res.final_x, ora["x_final"][:, track], min_same=0.9)
+    assert np.array_equal(res.steps[same], ora["steps"][same])
+
+
 def _source_network():
     # sources, decay, dimerisation (2A -> B), a reversible association and a
     # catalytic step (B + D -> B + E)
