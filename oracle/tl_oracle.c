@@ -60,6 +60,20 @@ static inline double tlo_unif64(uint32_t b) { return ((double)b + 0.5) * 2.32830
  * TLO_PTRS_CAP rejected attempts (probability below 1e-60) it returns
  * floor(mean), so the loop ends even for a non-finite mean. */
 #define TLO_PTRS_CAP 64
+/* ln(k!) for integer k >= 0: the exact product below 10 (k! <= 362880 is
+ * exact in FP64), else Stirling's series for ln Gamma(k + 1) to x^-7
+ * (absolute error below 4e-13 from k = 10). */
+static double tlo_lnfact(double k) {
+    if (k < 10.0) {
+        double f = 1.0;
+        for (double i = 2.0; i <= k; i += 1.0) f *= i;
+        return log(f);
+    }
+    const double x = k + 1.0, r = 1.0 / x, r2 = r * r;
+    return (x - 0.5) * log(x) - x + 0.91893853320467274178 +
+           r * (1.0 / 12.0 - r2 * (1.0 / 360.0 - r2 * (1.0 / 1260.0 - r2 / 1680.0)));
+}
+
 static int32_t tlo_ptrs(double mean, uint32_t head, const uint32_t key[2], uint32_t channel, uint32_t epoch,
                         uint32_t ctr3) {
     const double smu = sqrt(mean);
@@ -88,7 +102,7 @@ static int32_t tlo_ptrs(double mean, uint32_t head, const uint32_t key[2], uint3
         const double k = floor((2.0 * a / us + b) * U + mean + 0.43);
         if (us >= 0.07 && V <= vr) return (int32_t)k;
         if (k < 0.0 || (us < 0.013 && V > us)) continue;
-        if (log(V) + lia - log(a / (us * us) + b) <= -mean + k * lmu - lgamma(k + 1.0)) return (int32_t)k;
+        if (log(V) + lia - log(a / (us * us) + b) <= -mean + k * lmu - tlo_lnfact(k)) return (int32_t)k;
     }
 #undef PTRS_NEXT
     return (int32_t)mean;
@@ -116,6 +130,7 @@ void tlo_model_destroy(tlo_model *m) {
     free(m->order);
     free(m->s0);
     free(m->s1);
+    free(m->s2);
     free(m->row_ptr);
     free(m->csr_packed);
     free(m->g_order);
@@ -125,43 +140,55 @@ void tlo_model_destroy(tlo_model *m) {
 /* Reaction order from the negative entries of a column, in species order
  * (OrderDetectorAndIndFiller, CSCMatrix.cu:200-231).  A column without
  * negative entries is a source (order 0; the reference returns order -1 and
- * leaves its propensity unset).  Patterns other than ->, S->, 2S->, Sa+Sb->
+ * leaves its propensity unset); Sa+Sb+Sc (4) and 2Sa+Sb (5) have three
+ * reactant molecules (the reference treats them as order 2).  Other patterns
  * are rejected. */
 static int detect_order(const tlo_model *m, int32_t j) {
-    int32_t nneg = 0, first_v = 0, i0 = -1, i1 = -1, second_v = 0;
+    int32_t nneg = 0, v[3] = {0, 0, 0}, id[3] = {-1, -1, -1};
     for (int32_t e = m->col_ptr[j]; e < m->col_ptr[j + 1]; ++e) {
         if (m->stoich[e] >= 0) continue;
-        if (nneg == 0) {
-            first_v = m->stoich[e];
-            i0 = m->row_ind[e];
-        } else {
-            second_v = m->stoich[e];
-            i1 = m->row_ind[e];
+        if (nneg < 3) {
+            v[nneg] = m->stoich[e];
+            id[nneg] = m->row_ind[e];
         }
         nneg++;
     }
+    m->s2[j] = 0;
     if (nneg == 0) {
         m->order[j] = 0;
         m->s0[j] = 0;
         m->s1[j] = 0;
         return TLO_OK;
     }
-    if (nneg == 1 && first_v == -2) {
+    if (nneg == 1 && v[0] == -2) {
         m->order[j] = 3;
-        m->s0[j] = i0;
-        m->s1[j] = i0;
+        m->s0[j] = id[0];
+        m->s1[j] = id[0];
         return TLO_OK;
     }
-    if (nneg == 2 && first_v == -1 && second_v == -1 && i0 != i1) {
+    if (nneg == 2 && v[0] == -1 && v[1] == -1 && id[0] != id[1]) {
         m->order[j] = 2;
-        m->s0[j] = i0;
-        m->s1[j] = i1;
+        m->s0[j] = id[0];
+        m->s1[j] = id[1];
         return TLO_OK;
     }
-    if (nneg == 1 && first_v == -1) {
+    if (nneg == 1 && v[0] == -1) {
         m->order[j] = 1;
-        m->s0[j] = i0;
-        m->s1[j] = i0;
+        m->s0[j] = id[0];
+        m->s1[j] = id[0];
+        return TLO_OK;
+    }
+    if (nneg == 2 && id[0] != id[1] && v[0] + v[1] == -3 && (v[0] == -2 || v[1] == -2)) {
+        m->order[j] = 5; /* 2Sa + Sb, s0 = the species taken twice */
+        m->s0[j] = v[0] == -2 ? id[0] : id[1];
+        m->s1[j] = v[0] == -2 ? id[1] : id[0];
+        return TLO_OK;
+    }
+    if (nneg == 3 && v[0] == -1 && v[1] == -1 && v[2] == -1 && id[0] != id[1] && id[1] != id[2] && id[0] != id[2]) {
+        m->order[j] = 4; /* Sa + Sb + Sc */
+        m->s0[j] = id[0];
+        m->s1[j] = id[1];
+        m->s2[j] = id[2];
         return TLO_OK;
     }
     return TLO_ERR_UNSUPPORTED_ORDER;
@@ -193,6 +220,7 @@ int tlo_model_create(int32_t N, int32_t M, const int32_t *col_ptr, const int32_t
     m->order = (int8_t *)malloc((size_t)M);
     m->s0 = (int32_t *)malloc(sizeof(int32_t) * (size_t)M);
     m->s1 = (int32_t *)malloc(sizeof(int32_t) * (size_t)M);
+    m->s2 = (int32_t *)calloc((size_t)M, sizeof(int32_t));
     m->row_ptr = (int32_t *)calloc((size_t)(N + 1), sizeof(int32_t));
     m->csr_packed = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1));
     m->g_order = (int8_t *)malloc((size_t)N);

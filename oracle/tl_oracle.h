@@ -38,9 +38,13 @@
  *   - RNG: Philox4x32-10 counter streams keyed per realization instead of the
  *     per-channel hybrid Tausworthe state (d_rng.cuh:27-35).
  *   - sources (no reactant, -> products) have a = c, are never critical and
- *     take g = 1 where GetG reads their order; other unsupported reactant
+ *     take g = 1 where GetG reads their order; channels with three reactant
+ *     molecules, Sa+Sb+Sc (order 4: ((c*xa)*xb)*xc, g = 3) and 2Sa+Sb (order
+ *     5: (((c*xa)*(xa-1))*0.5)*xb, g = 1.5*(2 + 1/(x-1))), are critical when a
+ *     reactant count is below nc times its coefficient; other reactant
  *     patterns are rejected at model build (ref: order -1 gives an
- *     uninitialised propensity, functions.h:118-120, CSCMatrix.cu:225-231).
+ *     uninitialised propensity, functions.h:118-120, CSCMatrix.cu:225-231,
+ *     and a third reactant overwrites the second).
  *   - tau_i for species rows i >= M is +inf (ref reads order[] out of bounds,
  *     CSRMatrixPacked.cu:353).
  *   - the species sums mu_i, sigma^2_i of the tau bound accumulate in FP64 and
@@ -96,11 +100,12 @@ typedef struct tlo_model {
     double *rates;       /* [M] */
     float *rates_f;      /* [M]    rates rounded once to float */
     int32_t *x0;         /* [N] */
-    int8_t *order;       /* [M]    1: S->, 2: Sa+Sb->, 3: 2S-> */
+    int8_t *order;       /* [M]    0: ->, 1: S->, 2: Sa+Sb->, 3: 2S->, 4: Sa+Sb+Sc->, 5: 2Sa+Sb-> (s0 = a) */
     int32_t *s0, *s1;    /* [M]    reactant species (s1 == s0 for orders 1, 3) */
     int32_t *row_ptr;    /* [N+1]  species-major (CSR) */
     int32_t *csr_packed; /* [nnz]  (channel << 3) | ((v + 2) & 7) */
     int8_t *g_order;     /* [N]    order[i] for i < M, else 0 */
+    int32_t *s2;         /* [M]    third reactant of order-4 channels (else 0) */
 } tlo_model;
 
 /* Philox4x32-10 (Salmon et al., SC'11). */

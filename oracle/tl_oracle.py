@@ -35,6 +35,7 @@ class _Model(C.Structure):
         ("order", C.POINTER(C.c_int8)), ("s0", C.POINTER(C.c_int32)),
         ("s1", C.POINTER(C.c_int32)), ("row_ptr", C.POINTER(C.c_int32)),
         ("csr_packed", C.POINTER(C.c_int32)), ("g_order", C.POINTER(C.c_int8)),
+        ("s2", C.POINTER(C.c_int32)),
     ]
 
 

@@ -40,11 +40,17 @@ static inline const REAL *FN(rates)(const tlo_model *m) {
 /* Propensity and criticality of one channel (functions.h:100-123, nc = 5 from
  * model.cu:225).  Operation order is the reference's: (c*x0)*x1 and
  * ((c*x0)*(x0-1))/2.  A source (order 0) has a = c and is never critical. */
-static inline REAL FN(prop1)(int order, REAL c, int32_t x0, int32_t x1, int *crit) {
+static inline REAL FN(prop1)(int order, REAL c, int32_t x0, int32_t x1, int32_t x2, int *crit) {
     switch (order) {
     case 0:
         *crit = 0;
         return c;
+    case 4: /* Sa + Sb + Sc */
+        *crit = (x0 < TLO_NC) || (x1 < TLO_NC) || (x2 < TLO_NC);
+        return ((c * (REAL)x0) * (REAL)x1) * (REAL)x2;
+    case 5: /* 2Sa + Sb */
+        *crit = (x0 * 2 < TLO_NC) || (x1 < TLO_NC);
+        return (((c * (REAL)x0) * (REAL)(x0 - 1)) * (REAL)0.5) * (REAL)x1;
     case 1:
         *crit = x0 < TLO_NC;
         return c * (REAL)x0;
@@ -64,7 +70,8 @@ static void FN(prop_one)(const tlo_model *m, const int32_t *x, ptrdiff_t xs, REA
     double s = 0.0, sc = 0.0;
     for (int32_t j = 0; j < m->M; ++j) {
         int cr;
-        REAL aj = FN(prop1)(m->order[j], c[j], x[(ptrdiff_t)m->s0[j] * xs], x[(ptrdiff_t)m->s1[j] * xs], &cr);
+        REAL aj = FN(prop1)(m->order[j], c[j], x[(ptrdiff_t)m->s0[j] * xs], x[(ptrdiff_t)m->s1[j] * xs],
+                            x[(ptrdiff_t)m->s2[j] * xs], &cr);
         a[(ptrdiff_t)j * as] = aj;
         if (crit) crit[(ptrdiff_t)j * as] = (uint8_t)cr;
         s += (double)aj;
@@ -81,6 +88,8 @@ static inline REAL FN(getg)(int order, int32_t x) {
     case 1: return (REAL)1;
     case 2: return (REAL)2;
     case 3: return (REAL)2 + (REAL)1 / ((REAL)x - (REAL)1);
+    case 4: return (REAL)3;
+    case 5: return (REAL)1.5 * ((REAL)2 + (REAL)1 / ((REAL)x - (REAL)1));
     default: return (REAL)0;
     }
 }

@@ -113,7 +113,7 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     }
     const double t0 = now_s();
     // one allocation: c64 | c32 | s0 | s1 | row_ptr | csr | col_ptr | csc | x0 | order | g_order | rs |
-    // dep_ptr | dep | colq | aff_ptr | aff | tile_ptr | aff_rs | aff_c64 | aff_c32
+    // dep_ptr | dep | colq | aff_ptr | aff | tile_ptr | aff_rs | aff_c64 | aff_c32 | s2
     const size_t M = (size_t)m.M, N = (size_t)m.N, Z = (size_t)std::max(m.nnz, 1);
     std::vector<size_t> off;
     size_t total = 0;
@@ -144,6 +144,7 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     add(NA * 8);  // aff_rs
     add(NA * 8);  // aff_c64
     add(NA * 4);  // aff_c32
+    add(std::max<size_t>(m.s2.size(), 1) * 4);  // s2 (three-reactant channels)
     std::vector<unsigned char> host(total, 0);
     std::vector<float> c32(M);
     for (size_t j = 0; j < M; ++j) c32[j] = (float)m.rates[j];
@@ -192,6 +193,7 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
         c->device = device;
         m.dev[device] = c;
     }
+    if (!m.s2.empty()) memcpy(&host[off[21]], m.s2.data(), M * 4);
     if (c->bytes < total) {
         if (c->buf) TL_HIP(hipFree(c->buf));
         c->buf = nullptr;
@@ -226,6 +228,7 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     d.aff_rs = (const int2 *)(b + off[18]);
     d.aff_c64 = (const double *)(b + off[19]);
     d.aff_c32 = (const float *)(b + off[20]);
+    d.s2 = m.s2.empty() ? nullptr : (const int32_t *)(b + off[21]);
     c->version = m.version;
     out = d;
     if (upload_seconds) *upload_seconds = now_s() - t0;
@@ -276,6 +279,12 @@ tl_status check_params(const tl_model *m, const tl_sim_params *p) {
     if (p->poisson != TL_POISSON_NORMAL && p->poisson != TL_POISSON_EXACT)
         return set_error(TL_ERR_INVALID_ARG, "poisson must be TL_POISSON_NORMAL or TL_POISSON_EXACT");
     return TL_OK;
+}
+
+// Whether a run needs the kernels with three-reactant channels and the
+// exact sampler compiled in.
+bool needs_ext(const tl_model *m, const tl_sim_params *p) {
+    return !m->s2.empty() || p->poisson == TL_POISSON_EXACT;
 }
 
 std::vector<int32_t> track_list(const tl_model *m, const tl_sim_params *p) {
@@ -454,16 +463,18 @@ struct Entry {
 };
 
 struct TuneKey {
-    bool is64, sens;
+    bool is64, sens, ext;
     int32_t n_max, m_max;
     bool operator<(const TuneKey &o) const {
-        return std::tie(is64, sens, n_max, m_max) < std::tie(o.is64, o.sens, o.n_max, o.m_max);
+        return std::tie(is64, sens, ext, n_max, m_max) < std::tie(o.is64, o.sens, o.ext, o.n_max, o.m_max);
     }
 };
 std::mutex g_tune_mu;
 std::map<TuneKey, PersistentConfig> g_tuned;
 
-std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, bool sens, int32_t n_max, int32_t m_max) {
+// ext: the kernels with three-reactant channels and the exact sampler.
+std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, bool sens, int32_t n_max, int32_t m_max,
+                                         bool ext) {
     std::vector<PersistentConfig> out;
     const int shapes[][2] = {{64, 64}, {256, 64}, {256, 256}, {512, 512}, {1024, 1024}};
     // one-wavefront groups come in register budgets of 5 (the default), 4
@@ -481,6 +492,7 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
             const int nw = sh[1] == 64 ? (is64 ? 2 : 3) : (sh[1] == 512 ? 2 : 1);
             for (int w = 0; w < nw; ++w) {
                 PersistentConfig c{sh[0], sh[1], al != 0, wl ? wl[w] : 1};
+                c.ext = ext;
                 // 1024-lane groups take the largest propensity tile (4, 2 or
                 // 1 grains of 4096 channels) that fits: fewer tiles, fewer
                 // barriers; smaller groups keep one grain (two blocks per CU)
@@ -505,6 +517,7 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
         // preferred one comes first.
         PersistentConfig c{1024, 1024, false, 1};
         c.hbm = true;
+        c.ext = ext;
         c.tile_grains = 4;
         size_t lds = persistent_lds_bytes(c, is64, n_max, m_max);
         while (lds > dc.lds_per_block && c.tile_grains > 1) {
@@ -514,6 +527,7 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
         const bool c_ok = lds <= dc.lds_per_block && persistent_occupancy(c, is64, sens, lds) > 0;
         PersistentConfig st{1024, 1024, true, 1};
         st.hbm = true;
+        st.ext = ext;
         st.tile_grains = std::max(1, (m_max + kTileGrain - 1) / kTileGrain);
         lds = persistent_lds_bytes(st, is64, n_max, m_max);
         const bool st_ok = lds <= dc.lds_per_block && persistent_occupancy(st, is64, sens, lds) > 0;
@@ -644,10 +658,10 @@ tl_status launch_batch(DeviceContext &dc, const PersistentConfig &cfg, bool is64
 
 // Time each launch shape on a bounded sample (first few steps of a subset
 // of the work items) and keep the fastest in realization-steps per second.
-tl_status autotune(DeviceContext &dc, bool is64, bool sens, const std::vector<ModelItem> &items_in, int64_t n_items,
-                   int32_t n_max, int32_t m_max, unsigned long long *d_counter, int64_t *d_steps_all,
+tl_status autotune(DeviceContext &dc, bool is64, bool sens, bool ext, const std::vector<ModelItem> &items_in,
+                   int64_t n_items, int32_t n_max, int32_t m_max, unsigned long long *d_counter, int64_t *d_steps_all,
                    PersistentConfig &best) {
-    TuneKey key{is64, sens, n_max, m_max};
+    TuneKey key{is64, sens, ext, n_max, m_max};
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
         auto it = g_tuned.find(key);
@@ -656,7 +670,7 @@ tl_status autotune(DeviceContext &dc, bool is64, bool sens, const std::vector<Mo
             return TL_OK;
         }
     }
-    auto all = candidates(dc, is64, sens, n_max, m_max);
+    auto all = candidates(dc, is64, sens, n_max, m_max, ext);
     if (all.empty()) return set_error(TL_ERR_TOO_LARGE, "model too large for any persistent launch shape");
     best = heuristic(all, m_max);
     if (best.hbm) {  // global-memory state shapes are not timed (no trial launches)
@@ -797,10 +811,12 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
         std::vector<ModelItem> items(nm);
         int64_t base = 0;
         int32_t n_max = 0, m_max = 0;
+        bool ext = false;
         for (int32_t k = 0; k < nm; ++k) {
             Entry &e = E[members[k]];
             n_max = std::max(n_max, e.m->N);
             m_max = std::max(m_max, e.m->M);
+            ext |= needs_ext(e.m, e.p);
             ModelItem &mi = items[k];
             mi.dm = e.dm;
             mi.t_final = e.p->t_final;
@@ -836,7 +852,7 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
 
         PersistentConfig cfg;
         if (bs_req > 0) {
-            auto cands = candidates(*dc, is64, sens, n_max, m_max);
+            auto cands = candidates(*dc, is64, sens, n_max, m_max, ext);
             const int gs = bs_req <= 64 ? 64 : bs_req;
             const int bs = bs_req <= 64 ? 64 : bs_req;
             bool found = false;
@@ -856,7 +872,7 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
             if (!found)
                 return set_error(TL_ERR_INVALID_ARG, "block_size " + std::to_string(bs_req) + " does not fit this model");
         } else {
-            TL_TRY(autotune(*dc, is64, sens, items, n_items, n_max, m_max, d_counter, nullptr, cfg));
+            TL_TRY(autotune(*dc, is64, sens, ext, items, n_items, n_max, m_max, d_counter, nullptr, cfg));
         }
         ClassLaunch c;
         c.cfg = cfg;
@@ -962,7 +978,7 @@ static tl_status auto_plain(const tl_model *m, const tl_sim_params *p, bool &pla
     if (p->sensitivities || p->n_intervals > 0) return TL_OK;
     DeviceContext *dc;
     TL_TRY(device_context(p->device, dc));
-    plain = candidates(*dc, p->precision == TL_FP64, false, m->N, m->M).empty();
+    plain = candidates(*dc, p->precision == TL_FP64, false, m->N, m->M, needs_ext(m, p)).empty();
     return TL_OK;
 }
 

@@ -45,6 +45,7 @@ struct DevModel {
     const int8_t *order;     // [M] 1, 2, 3
     const int32_t *s0;       // [M]
     const int32_t *s1;       // [M]
+    const int32_t *s2;       // [M] third reactant of Sa+Sb+Sc channels; null unless some order >= 4
     const int32_t *row_ptr;  // [N+1] species-major
     const int32_t *csr;      // [nnz] (channel << 3) | ((v + 2) & 7)
     const int32_t *col_ptr;  // [M+1] channel-major
@@ -124,11 +125,33 @@ __device__ __forceinline__ R propensity(int order, R c, int32_t x0, int32_t x1, 
     return o3 ? t2 * (R)0.5 : t2;
 }
 
+// Channels with three reactant molecules (models with m.s2 != null):
+// order 4 Sa+Sb+Sc ((c*x0)*x1)*x2, order 5 2Sa+Sb (((c*x0)*(x0-1))*0.5)*x1
+// (x0 the species taken twice); critical when a reactant count is below
+// nc times its coefficient.  Other orders as propensity().
+template <class R>
+__device__ __forceinline__ R propensity_ext(int order, R c, int32_t x0, int32_t x1, int32_t x2, bool &crit) {
+    if (order == 4) {
+        crit = (x0 < kNc) || (x1 < kNc) || (x2 < kNc);
+        return ((c * (R)x0) * (R)x1) * (R)x2;
+    }
+    if (order == 5) {
+        crit = (x0 * 2 < kNc) || (x1 < kNc);
+        return (((c * (R)x0) * (R)(x0 - 1)) * (R)0.5) * (R)x1;
+    }
+    return propensity<R>(order, c, x0, x1, crit);
+}
+
+// g of the tau bound for the order GetG reads (the channel-indexed order):
+// Cao-Gillespie-Petzold's highest-order forms, with 3 for Sa+Sb+Sc and
+// (3/2)(2 + 1/(x-1)) for 2Sa+Sb.
 template <class R>
 __device__ __forceinline__ R getg(int order, int32_t x) {
     if (order == 0 || order == 1) return (R)1;  // order 0: a source, the first-order bound
     if (order == 2) return (R)2;
     if (order == 3) return (R)2 + (R)1 / ((R)x - (R)1);
+    if (order == 4) return (R)3;
+    if (order == 5) return (R)1.5 * ((R)2 + (R)1 / ((R)x - (R)1));
     return (R)0;
 }
 
@@ -161,34 +184,49 @@ __device__ __forceinline__ uint32_t quad_pick(uint4 o, int s) {
 // draws 2i+1 and 2i+2 (draw 1 = head; draw n >= 2 as in poisson_draw):
 // U = u - 1/2 from the first, V from the second.  After 64 rejected attempts
 // (probability below 1e-60) it returns floor(mean), so the loop always ends.
+// ln(k!) for integer k >= 0: the exact product below 10, else Stirling's
+// series for ln Gamma(k + 1) to x^-7 (absolute error below 4e-13 from k = 10).
+__device__ __forceinline__ double ln_factorial(double k) {
+    if (k < 10.0) {
+        double f = 1.0;
+        for (double i = 2.0; i <= k; i += 1.0) f *= i;
+        return log(f);
+    }
+    const double x = k + 1.0, r = 1.0 / x, r2 = r * r;
+    return (x - 0.5) * log(x) - x + 0.91893853320467274178 +
+           r * (1.0 / 12.0 - r2 * (1.0 / 360.0 - r2 * (1.0 / 1260.0 - r2 / 1680.0)));
+}
+
 __device__ __forceinline__ int32_t poisson_ptrs(double mean, uint32_t head, uint2 key, uint32_t channel,
                                                 uint32_t epoch, uint32_t ctr3) {
-    const double smu = sqrt(mean);
-    const double b = 0.931 + 2.53 * smu;
-    const double a = -0.059 + 0.02483 * b;
-    const double lia = log(1.1239 + 1.1328 / (b - 3.4));
-    const double vr = 0.9277 - 3.6224 / (b - 2.0);
-    const double lmu = log(mean);
+    // the constants are recomputed per attempt (cheap on this rare path) so
+    // that little stays live across the loop inside the firing pass
     uint32_t blk = 0;
     uint4 o = make_uint4(0u, 0u, 0u, 0u);
     int pos = 4;
-    auto next = [&]() -> uint32_t {
+    uint32_t ub = head;
+    for (int attempt = 0; attempt < 64; ++attempt) {
+        if (attempt > 0) {
+            if (pos == 4) {
+                o = philox4x32_10(make_uint4(blk++, channel, epoch, ctr3), key);
+                pos = 0;
+            }
+            ub = quad_pick(o, pos++);
+        }
         if (pos == 4) {
             o = philox4x32_10(make_uint4(blk++, channel, epoch, ctr3), key);
             pos = 0;
         }
-        return quad_pick(o, pos++);
-    };
-    uint32_t ub = head;
-    for (int attempt = 0; attempt < 64; ++attempt) {
-        if (attempt > 0) ub = next();
-        const uint32_t vb = next();
+        const uint32_t vb = quad_pick(o, pos++);
+        const double b = 0.931 + 2.53 * sqrt(mean);
+        const double a = -0.059 + 0.02483 * b;
         const double U = unif64(ub) - 0.5, V = unif64(vb);
         const double us = 0.5 - fabs(U);
         const double k = floor((2.0 * a / us + b) * U + mean + 0.43);
-        if (us >= 0.07 && V <= vr) return (int32_t)k;
+        if (us >= 0.07 && V <= 0.9277 - 3.6224 / (b - 2.0)) return (int32_t)k;
         if (k < 0.0 || (us < 0.013 && V > us)) continue;
-        if (log(V) + lia - log(a / (us * us) + b) <= -mean + k * lmu - lgamma(k + 1.0)) return (int32_t)k;
+        const double lia = log(1.1239 + 1.1328 / (b - 3.4));
+        if (log(V) + lia - log(a / (us * us) + b) <= -mean + k * log(mean) - ln_factorial(k)) return (int32_t)k;
     }
     return (int32_t)mean;
 }

@@ -122,6 +122,7 @@ struct PersistentConfig {
     int wpe;       // waves per SIMD the kernel is compiled for (register budget 512 / wpe)
     int tile_grains = 1;  // block groups without a in LDS: propensity tile of tile_grains * 4096 channels
     bool hbm = false;     // group state in global memory (models whose state does not fit LDS)
+    bool ext = false;     // kernels with three-reactant channels and the exact sampler compiled in
 };
 
 struct PersistentLaunch {

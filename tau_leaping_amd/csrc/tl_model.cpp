@@ -23,40 +23,53 @@ namespace tl {
 // ------------------------------------------------------------- layouts ----
 
 // Reaction order and reactant indices from the negative entries of column j in
-// species order.  Accepted patterns: S-> (1), Sa+Sb-> (2), 2S-> (3).
-static bool channel_order(const tl_model &m, int32_t j, int8_t &order, int32_t &a, int32_t &b) {
-    int nneg = 0, v0 = 0, v1 = 0;
-    int32_t i0 = -1, i1 = -1;
+// species order.  Accepted patterns: -> (0, a source), S-> (1), Sa+Sb-> (2),
+// 2S-> (3), and three reactant molecules: Sa+Sb+Sc-> (4, third index in c),
+// 2Sa+Sb-> (5, a the species taken twice).
+static bool channel_order(const tl_model &m, int32_t j, int8_t &order, int32_t &a, int32_t &b, int32_t &c) {
+    int nneg = 0;
+    int v[3] = {0, 0, 0};
+    int32_t id[3] = {-1, -1, -1};
     for (int32_t e = m.col_ptr[j]; e < m.col_ptr[j + 1]; ++e) {
         if (m.stoich[e] >= 0) continue;
-        if (nneg == 0) {
-            v0 = m.stoich[e];
-            i0 = m.row_ind[e];
-        } else {
-            v1 = m.stoich[e];
-            i1 = m.row_ind[e];
+        if (nneg < 3) {
+            v[nneg] = m.stoich[e];
+            id[nneg] = m.row_ind[e];
         }
         ++nneg;
     }
+    a = b = c = 0;
     if (nneg == 0) {  // a source: no reactant, a = c
         order = 0;
-        a = b = 0;
         return true;
     }
-    if (nneg == 1 && v0 == -1) {
+    if (nneg == 1 && v[0] == -1) {
         order = 1;
-        a = b = i0;
+        a = b = id[0];
         return true;
     }
-    if (nneg == 1 && v0 == -2) {
+    if (nneg == 1 && v[0] == -2) {
         order = 3;
-        a = b = i0;
+        a = b = id[0];
         return true;
     }
-    if (nneg == 2 && v0 == -1 && v1 == -1 && i0 != i1) {
+    if (nneg == 2 && v[0] == -1 && v[1] == -1 && id[0] != id[1]) {
         order = 2;
-        a = i0;
-        b = i1;
+        a = id[0];
+        b = id[1];
+        return true;
+    }
+    if (nneg == 2 && id[0] != id[1] && (v[0] + v[1] == -3) && (v[0] == -2 || v[1] == -2)) {
+        order = 5;  // 2Sa + Sb: a is the species taken twice
+        a = v[0] == -2 ? id[0] : id[1];
+        b = v[0] == -2 ? id[1] : id[0];
+        return true;
+    }
+    if (nneg == 3 && v[0] == -1 && v[1] == -1 && v[2] == -1 && id[0] != id[1] && id[1] != id[2] && id[0] != id[2]) {
+        order = 4;  // Sa + Sb + Sc
+        a = id[0];
+        b = id[1];
+        c = id[2];
         return true;
     }
     return false;
@@ -109,11 +122,20 @@ tl_status build_model(tl_model &m, std::string &msg) {
     m.order.assign(m.M, 0);
     m.s0.assign(m.M, 0);
     m.s1.assign(m.M, 0);
-    for (int32_t j = 0; j < m.M; ++j)
-        if (!channel_order(m, j, m.order[j], m.s0[j], m.s1[j])) {
-            msg = "channel " + std::to_string(j) + ": unsupported reactant pattern (need S->, 2S-> or Sa+Sb->)";
+    std::vector<int32_t> third(m.M, 0);
+    bool ext = false;
+    for (int32_t j = 0; j < m.M; ++j) {
+        if (!channel_order(m, j, m.order[j], m.s0[j], m.s1[j], third[j])) {
+            msg = "channel " + std::to_string(j) +
+                  ": unsupported reactant pattern (need ->, S->, 2S->, Sa+Sb->, Sa+Sb+Sc-> or 2Sa+Sb->)";
             return TL_ERR_UNSUPPORTED_ORDER;
         }
+        ext |= m.order[j] >= 4;
+    }
+    // third reactant of Sa+Sb+Sc channels; empty unless some channel has
+    // three reactant molecules (the kernels then take the extended formulas)
+    if (ext) m.s2.swap(third);
+    else m.s2.clear();
     // species-major transpose, rows in channel order
     m.row_ptr.assign(m.N + 1, 0);
     for (int32_t e = 0; e < m.nnz; ++e) m.row_ptr[m.row_ind[e] + 1]++;
@@ -135,6 +157,7 @@ tl_status build_model(tl_model &m, std::string &msg) {
         if (m.order[j] == 0) continue;  // sources read no species
         m.dep_ptr[m.s0[j] + 1]++;
         if (m.s1[j] != m.s0[j]) m.dep_ptr[m.s1[j] + 1]++;
+        if (m.order[j] == 4) m.dep_ptr[m.s2[j] + 1]++;
     }
     for (int32_t i = 0; i < m.N; ++i) m.dep_ptr[i + 1] += m.dep_ptr[i];
     m.dep.assign(m.dep_ptr[m.N], 0);
@@ -144,6 +167,7 @@ tl_status build_model(tl_model &m, std::string &msg) {
             if (m.order[j] == 0) continue;
             m.dep[f[m.s0[j]]++] = j;
             if (m.s1[j] != m.s0[j]) m.dep[f[m.s1[j]]++] = j;
+            if (m.order[j] == 4) m.dep[f[m.s2[j]]++] = j;
         }
     }
     // fixed-width columns

@@ -22,6 +22,16 @@
 // leap rejection on negative populations, clamp to t_final).
 #include "tl_internal.hpp"
 
+// This file is compiled twice: as is (TL_PERSISTENT_EXT 0, the kernels for
+// the reference's reaction forms and sampler, plus the host side), and from
+// tl_persistent_ext.hip (TL_PERSISTENT_EXT 1: the same kernels with channels
+// of three reactant molecules and the exact Poisson sampler).  Keeping those
+// paths out of the default instances keeps their registers out of its hot
+// loops.
+#ifndef TL_PERSISTENT_EXT
+#define TL_PERSISTENT_EXT 0
+#endif
+
 namespace tl {
 
 namespace {
@@ -61,6 +71,7 @@ struct GModel {
     gptr<const v2i> aff_rs;
     gptr<const double> aff_c64;
     gptr<const float> aff_c32;
+    gptr<const int32_t> s2;  // null unless some channel has three reactant molecules
     // column entries of a wave-uniform channel (scalar load)
     __device__ __forceinline__ int4 colq_uniform(int32_t j) const {
         const v4i v = ((cptr<v4i>)colq)[j];
@@ -105,6 +116,7 @@ __device__ __forceinline__ GModel global_model(const DevModel &d) {
     g.colq = as_global(reinterpret_cast<const v4i *>(d.colq));
     g.aff_rs = as_global(reinterpret_cast<const v2i *>(d.aff_rs));
     g.aff_c64 = as_global(d.aff_c64);
+    g.s2 = as_global(d.s2);
     g.aff_c32 = as_global(d.aff_c32);
     return g;
 }
@@ -634,6 +646,18 @@ __device__ __forceinline__ void chan_unpack(int2 p, int &order, int32_t &s0, int
     s1 = p.y;
 }
 
+// a_j and criticality of channel j from its packed reactants.  s2 (a
+// model-wide pointer, null unless some channel has three reactant molecules)
+// selects the extended formulas with a scalar branch.
+template <class R>
+__device__ __forceinline__ R chan_prop(int2 p, R cj, const int32_t *xs, gptr<const int32_t> s2, int32_t j, bool &cr) {
+    int order;
+    int32_t s0, s1;
+    chan_unpack(p, order, s0, s1);
+    if (s2 != nullptr) return propensity_ext<R>(order, cj, xs[s0], xs[s1], order == 4 ? xs[s2[j]] : 1, cr);
+    return propensity<R>(order, cj, xs[s0], xs[s1], cr);
+}
+
 // Where the passes read propensities: an array (kept in LDS, written by the
 // propensity pass and the direct-event updates), or a recomputation from the
 // populations in LDS and the shared channel constants.  Both give the same
@@ -651,13 +675,11 @@ struct PropRecomputed {
     gptr<const v2i> rs;
     gptr<const R> c;
     const int32_t *xs;
+    gptr<const int32_t> s2;
     __device__ __forceinline__ R get(int32_t j) const {
-        int order;
-        int32_t s0, s1;
         bool cr;
         const v2i p = rs[j];
-        chan_unpack(make_int2(p.x, p.y), order, s0, s1);
-        return propensity<R>(order, c[j], xs[s0], xs[s1], cr);
+        return chan_prop<R>(make_int2(p.x, p.y), c[j], xs, s2, j, cr);
     }
     __device__ __forceinline__ void set(int32_t, R) const {}
 };
@@ -687,11 +709,8 @@ __device__ __forceinline__ void column_add(int32_t *x, int32_t p, int32_t k) {
 template <class R, class P>
 __device__ __forceinline__ bool prop_one(const GModel &m, int32_t j, int2 rsj, R cj, int4 q, const int32_t *xs,
                                          const P &pa, double *mu, double *sig, double &s, double &sc) {
-    int order;
-    int32_t s0, s1;
     bool cr;
-    chan_unpack(rsj, order, s0, s1);
-    const R aj = propensity<R>(order, cj, xs[s0], xs[s1], cr);
+    const R aj = chan_prop<R>(rsj, cj, xs, m.s2, j, cr);
     pa.set(j, aj);
     s += (double)aj;
     if (cr) sc += (double)aj;
@@ -781,19 +800,13 @@ __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c,
             // the tile buffer (recomputed propensities) keeps only what the
             // gather needs: a_j of non-critical channels, 0 for critical ones
             if (ja < t1) {
-                int order;
-                int32_t s0, s1;
-                chan_unpack(ra, order, s0, s1);
-                aa = propensity<R>(order, ca, xs[s0], xs[s1], cra);
+                aa = chan_prop<R>(ra, ca, xs, m.s2, ja, cra);
                 A[ja - aoff] = (P::kStored || !cra) ? aa : (R)0;
                 s += (double)aa;
                 if (cra) sc += (double)aa;
             }
             if (jb < t1) {
-                int order;
-                int32_t s0, s1;
-                chan_unpack(rb, order, s0, s1);
-                ab = propensity<R>(order, cb, xs[s0], xs[s1], crb);
+                ab = chan_prop<R>(rb, cb, xs, m.s2, jb, crb);
                 A[jb - aoff] = (P::kStored || !crb) ? ab : (R)0;
                 s += (double)ab;
                 if (crb) sc += (double)ab;
@@ -873,11 +886,8 @@ __device__ __forceinline__ void prop_update(const GModel &m, gptr<const R> c, co
                 ck = m.aff_c64[d];
             else
                 ck = m.aff_c32[d];
-            int order;
-            int32_t s0, s1;
             bool cr;
-            chan_unpack(make_int2(p.x, p.y), order, s0, s1);
-            pa.set(k, propensity<R>(order, ck, xs[s0], xs[s1], cr));
+            pa.set(k, chan_prop<R>(make_int2(p.x, p.y), ck, xs, m.s2, k, cr));
         }
         return;
     }
@@ -887,11 +897,8 @@ __device__ __forceinline__ void prop_update(const GModel &m, gptr<const R> c, co
         const int32_t d1 = m.dep_ptr[i + 1];
         for (int32_t d = m.dep_ptr[i] + lane; d < d1; d += GS) {
             const int32_t k = m.dep[d];
-            int order;
-            int32_t s0, s1;
             bool cr;
-            chan_unpack(m.rs_at(k), order, s0, s1);
-            pa.set(k, propensity<R>(order, c[k], xs[s0], xs[s1], cr));
+            pa.set(k, chan_prop<R>(m.rs_at(k), c[k], xs, m.s2, k, cr));
         }
     }
 }
@@ -916,10 +923,11 @@ __device__ __forceinline__ double score_term(bool cr, R aj, R tr, double tau, in
 // caller synchronizes before reading what f wrote.
 // CRIT_A: f needs a_j of critical channels too (else they get 0 and a
 // recomputed propensity is not evaluated for them).
-template <class R, int GS, bool CRIT_A, class P, class F>
+// EXACT: the exact sampler above mean 80 (a separate instance, so that the
+// default pass carries none of its registers).
+template <class R, int GS, bool CRIT_A, bool EXACT, class P, class F>
 __device__ __forceinline__ void for_each_firing(const GModel &m, const Grp<GS> &G, const P &pa, const uint64_t *critq,
-                                                uint2 key, uint32_t epoch, uint32_t c3, R tr, int32_t jfire, bool exact,
-                                                F &&f) {
+                                                uint2 key, uint32_t epoch, uint32_t c3, R tr, int32_t jfire, F &&f) {
     const uint32_t ck = c3 | kKindPoisson;
     for (int32_t c0 = 0; c0 < m.M; c0 += 4 * GS) {
         const int32_t jb = c0 + 4 * G.lane;
@@ -945,7 +953,7 @@ __device__ __forceinline__ void for_each_firing(const GModel &m, const Grp<GS> &
                     aj = i == 0 ? a4[0] : (i == 1 ? a4[1] : (i == 2 ? a4[2] : a4[3]));
                 const int32_t kj =
                     cr ? (j == jfire ? 1 : 0)
-                       : poisson_draw<R>(aj * tr, quad_pick(h, i), key, (uint32_t)j, epoch, ck, exact);
+                       : poisson_draw<R>(aj * tr, quad_pick(h, i), key, (uint32_t)j, epoch, ck, EXACT);
                 f(j, cr, aj, kj);
             }
         }
@@ -974,11 +982,13 @@ __device__ __forceinline__ void grid_record(const ModelItem &mi, int64_t r, int 
     }
 }
 
-template <class R, int GS, bool SENS, class P>
+template <class R, int GS, bool SENS, bool EXT, class P>
 __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, int32_t *xs, int32_t *dx,
                               int32_t *snap, uint64_t *critq, double *mu, double *sig, double *wsum, R *tile, int tile_grains,
                               const P &pa) {
-    const GModel m = global_model(mi.dm);
+    GModel m_ = global_model(mi.dm);
+    if constexpr (!EXT) m_.s2 = nullptr;  // the extended formulas are compiled out
+    const GModel &m = m_;
     // Group barrier.  Without sensitivities the lanes of a one-wavefront group
     // only exchange data through LDS, whose operations a wavefront performs in
     // order, so a code-motion barrier suffices (no wait on outstanding global
@@ -995,7 +1005,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
     const uint32_t c3 = mi.model_id << 2;
     const double tf = mi.t_final;
     const R eps = (R)mi.eps;
-    const bool exact = mi.poisson_exact != 0;
+    const bool exact = EXT && mi.poisson_exact != 0;
     // Likelihood-ratio score S_j = dlog p(path)/dc_j, and D_j = dt/dc_j of the
     // current time (nonzero through tau1-limited leaps; used by the clamped
     // last leap and the truncated last direct interval).
@@ -1202,8 +1212,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
             const int32_t jfire = fire ? jc : -1;
             const R tr = (R)tau;
             double qpart = 0.0;
-            for_each_firing<R, GS, SENS>(m, G, pa, critq, key, epoch, c3, tr, jfire, exact,
-                                   [&](int32_t j, bool cr, R aj, int32_t kj) {
+            auto fire_one = [&](int32_t j, bool cr, R aj, int32_t kj) {
                                        if (kj) {
                                            if (m.colq) {
                                                const int4 q = m.colq_at(j);
@@ -1221,18 +1230,26 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                                            qpart += term;
                                            if (c[j] != (R)0) S[j] += term / (double)c[j];
                                        }
-                                   });
+                                   };
+            if (exact)
+                for_each_firing<R, GS, SENS, true>(m, G, pa, critq, key, epoch, c3, tr, jfire, fire_one);
+            else
+                for_each_firing<R, GS, SENS, false>(m, G, pa, critq, key, epoch, c3, tr, jfire, fire_one);
             sync_();  // dx complete
             bool neg = false;
             for (int32_t i = lane; i < m.N; i += GS) neg |= (xs[i] + dx[i]) < 0;
             neg = G.any(neg);
             if (neg) {
                 for (int32_t i = lane; i < m.N; i += GS) dx[i] = 0;
-                if (SENS)  // take back this attempt's score terms (same draws)
-                    for_each_firing<R, GS, SENS>(m, G, pa, critq, key, epoch, c3, tr, jfire, exact,
-                                           [&](int32_t j, bool cr, R aj, int32_t kj) {
-                                               if (c[j] != (R)0) S[j] -= score_term<R>(cr, aj, tr, tau, kj, exact) / (double)c[j];
-                                           });
+                if (SENS) {  // take back this attempt's score terms (same draws)
+                    auto undo = [&](int32_t j, bool cr, R aj, int32_t kj) {
+                        if (c[j] != (R)0) S[j] -= score_term<R>(cr, aj, tr, tau, kj, exact) / (double)c[j];
+                    };
+                    if (exact)
+                        for_each_firing<R, GS, SENS, true>(m, G, pa, critq, key, epoch, c3, tr, jfire, undo);
+                    else
+                        for_each_firing<R, GS, SENS, false>(m, G, pa, critq, key, epoch, c3, tr, jfire, undo);
+                }
                 epoch++;
                 nrej++;
                 sync_();
@@ -1320,7 +1337,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
 // LDS.  Block groups only:
 // their barriers are full workgroup barriers, which order global memory
 // between the waves of a block as they order LDS.
-template <class R, int BS, int GS, bool ALDS, bool SENS, int WPE, bool HBM>
+template <class R, int BS, int GS, bool ALDS, bool SENS, int WPE, bool HBM, bool EXT>
 __global__ __launch_bounds__(BS, WPE) void k_persistent(PersistentLaunch L) {
     static_assert(!HBM || (GS == BS && GS > 64), "HBM state: block groups");
     extern __shared__ __align__(16) unsigned char lds[];
@@ -1356,27 +1373,31 @@ __global__ __launch_bounds__(BS, WPE) void k_persistent(PersistentLaunch L) {
         const ModelItem mi = ld_uniform(&L.items[uni(lo)]);
         if constexpr (ALDS) {
             const PropStored<R> pa{reinterpret_cast<R *>(base + lay.off_a)};
-            simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, critq, mu, sig, wsum, tile, L.tile_grains, pa);
+            simulate_item<R, GS, SENS, EXT>(mi, item - mi.item_base, G, xs, dx, snap, critq, mu, sig, wsum, tile,
+                                            L.tile_grains, pa);
         } else {
             const GModel gm = global_model(mi.dm);
-            const PropRecomputed<R> pa{gm.rs, gm.template rates<R>(), xs};
-            simulate_item<R, GS, SENS>(mi, item - mi.item_base, G, xs, dx, snap, critq, mu, sig, wsum, tile, L.tile_grains, pa);
+            const PropRecomputed<R> pa{gm.rs, gm.template rates<R>(), xs, EXT ? gm.s2 : nullptr};
+            simulate_item<R, GS, SENS, EXT>(mi, item - mi.item_base, G, xs, dx, snap, critq, mu, sig, wsum, tile,
+                                            L.tile_grains, pa);
         }
     }
 }
 
 // ---------------------------------------------------------- dispatch ----
+constexpr bool kExt = TL_PERSISTENT_EXT != 0;
+
 template <class R, bool SENS>
 const void *kernel_ptr(int bs, int gs, bool alds, int wpe, bool hbm) {
     if (hbm) {
         if (bs != 1024 || gs != 1024 || wpe != 1) return nullptr;
-        return alds ? (const void *)k_persistent<R, 1024, 1024, true, SENS, 1, true>
-                    : (const void *)k_persistent<R, 1024, 1024, false, SENS, 1, true>;
+        return alds ? (const void *)k_persistent<R, 1024, 1024, true, SENS, 1, true, kExt>
+                    : (const void *)k_persistent<R, 1024, 1024, false, SENS, 1, true, kExt>;
     }
 #define TL_K(BS_, GS_, W_)                                                                             \
     if (bs == BS_ && gs == GS_ && wpe == W_)                                                           \
-        return alds ? (const void *)k_persistent<R, BS_, GS_, true, SENS, W_, false>                   \
-                    : (const void *)k_persistent<R, BS_, GS_, false, SENS, W_, false>;
+        return alds ? (const void *)k_persistent<R, BS_, GS_, true, SENS, W_, false, kExt>             \
+                    : (const void *)k_persistent<R, BS_, GS_, false, SENS, W_, false, kExt>;
     TL_K(64, 64, 4)
     TL_K(64, 64, 5)
     TL_K(256, 64, 4)
@@ -1393,7 +1414,8 @@ const void *kernel_ptr(int bs, int gs, bool alds, int wpe, bool hbm) {
     return nullptr;
 }
 
-const void *kernel_for(const PersistentConfig &c, bool is64, bool sens) {
+// the kernels of this translation unit
+const void *kernel_for_tu(const PersistentConfig &c, bool is64, bool sens) {
     if (is64)
         return sens ? kernel_ptr<double, true>(c.block, c.group, c.a_lds, c.wpe, c.hbm)
                     : kernel_ptr<double, false>(c.block, c.group, c.a_lds, c.wpe, c.hbm);
@@ -1401,6 +1423,7 @@ const void *kernel_for(const PersistentConfig &c, bool is64, bool sens) {
                 : kernel_ptr<float, false>(c.block, c.group, c.a_lds, c.wpe, c.hbm);
 }
 
+#if !TL_PERSISTENT_EXT
 // ------------------------------------------------------- gradient reduce --
 __global__ __launch_bounds__(256) void k_track_mean(int64_t Rn, int32_t T, const int32_t *__restrict__ xt,
                                                     double *__restrict__ mean) {
@@ -1438,8 +1461,18 @@ __global__ __launch_bounds__(256) void k_grad(int64_t Rn, int32_t T, int32_t M, 
         se[(int64_t)q * M + j] = sqrt(fmax(var, 0.0) / (double)Rn);
     }
 }
+#endif
 
 }  // namespace
+
+#if TL_PERSISTENT_EXT
+const void *kernel_for_ext(const PersistentConfig &c, bool is64, bool sens) { return kernel_for_tu(c, is64, sens); }
+#else
+const void *kernel_for_ext(const PersistentConfig &c, bool is64, bool sens);  // tl_persistent_ext.hip
+
+static const void *kernel_for(const PersistentConfig &c, bool is64, bool sens) {
+    return c.ext ? kernel_for_ext(c, is64, sens) : kernel_for_tu(c, is64, sens);
+}
 
 size_t persistent_lds_bytes(const PersistentConfig &c, bool is64, int32_t n_max, int32_t m_max) {
     if (c.hbm) return hbm_lds_bytes(m_max, c.group, c.a_lds, is64 ? 8 : 4, c.tile_grains);
@@ -1474,5 +1507,7 @@ void launch_grad_reduce(int64_t R, int32_t n_track, int32_t M, const int32_t *x_
     dim3 g((unsigned)((M + 255) / 256), (unsigned)n_track);
     k_grad<<<g, 256, 0, s>>>(R, n_track, M, x_track, score, mean_scratch, dmean, dmean_se);
 }
+
+#endif  // !TL_PERSISTENT_EXT
 
 }  // namespace tl

@@ -21,6 +21,17 @@ constexpr int kT = 256;  // threads per block for the realization axis
 inline unsigned blocks_for(int64_t n, int t = kT) { return (unsigned)((n + t - 1) / t); }
 
 // ---------------------------------------------------------- propensities --
+// a_j of channel j for realization r from the structure-of-arrays populations
+// (extended formulas when the model has three-reactant channels).
+template <class R>
+__device__ __forceinline__ R soa_prop(const DevModel &m, int32_t j, R cj, const int32_t *x, int64_t Rn, int64_t r,
+                                      bool &cr) {
+    const int order = m.order[j];
+    const int32_t x0 = x[(int64_t)m.s0[j] * Rn + r], x1 = x[(int64_t)m.s1[j] * Rn + r];
+    if (m.s2) return propensity_ext<R>(order, cj, x0, x1, order == 4 ? x[(int64_t)m.s2[j] * Rn + r] : 1, cr);
+    return propensity<R>(order, cj, x0, x1, cr);
+}
+
 template <class R>
 __global__ __launch_bounds__(kT) void k_prop(DevModel m, int64_t Rn, int32_t nchunk, const int32_t *__restrict__ x,
                                              R *__restrict__ a, uint8_t *__restrict__ crit, double *__restrict__ a0,
@@ -36,7 +47,7 @@ __global__ __launch_bounds__(kT) void k_prop(DevModel m, int64_t Rn, int32_t nch
     double s = 0.0, sc = 0.0;
     for (int32_t j = j0; j < j1; ++j) {
         bool cr;
-        const R aj = propensity<R>(m.order[j], rc[j], x[(int64_t)m.s0[j] * Rn + r], x[(int64_t)m.s1[j] * Rn + r], cr);
+        const R aj = soa_prop<R>(m, j, rc[j], x, Rn, r, cr);
         a[(int64_t)j * Rn + r] = aj;
         crit[(int64_t)j * Rn + r] = cr;
         s += (double)aj;
@@ -222,7 +233,7 @@ __device__ double prop_soa(const DevModel &m, const int32_t *x, R *a, uint8_t *c
     double s = 0.0, sc = 0.0;
     for (int32_t j = 0; j < m.M; ++j) {
         bool cr;
-        const R aj = propensity<R>(m.order[j], rc[j], x[(int64_t)m.s0[j] * Rn + r], x[(int64_t)m.s1[j] * Rn + r], cr);
+        const R aj = soa_prop<R>(m, j, rc[j], x, Rn, r, cr);
         a[(int64_t)j * Rn + r] = aj;
         if (crit) crit[(int64_t)j * Rn + r] = cr;
         s += (double)aj;

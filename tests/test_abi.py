@@ -64,7 +64,7 @@ def test_validation_status_codes():
         return T.Model.from_network(Network(3, reacs, np.array(x0, np.int32)))
 
     with pytest.raises(T.TLError) as e:
-        mk([Reaction({0: 1, 1: 1, 2: 1}, {}, 1.0)])  # three reactants
+        mk([Reaction({0: 2, 1: 2}, {}, 1.0)])  # four reactant molecules
     assert e.value.status == 3
     order, s0, s1, _, _ = mk([Reaction({}, {0: 1}, 1.0), Reaction({0: 1}, {}, 1.0)]).structure()
     assert list(order) == [0, 1]  # a source is order 0

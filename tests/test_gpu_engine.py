@@ -230,6 +230,22 @@ def test_exact_sampler_law(cuda):
     assert abs(k.mean() - 500) < 4 * np.sqrt(500 / R)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("mode,bs", [("persistent", 64), ("persistent", 256), ("persistent", 1024), ("plain", 0)])
+def test_three_reactant_channels_match_oracle(cuda, prec, mode, bs):
+    # Sa+Sb+Sc and 2Sa+Sb channels (the extended propensity formulas and
+    # their third reactant) in every form
+    from helpers import trimolecular_net
+    m, om = pair(trimolecular_net())
+    assert list(m.structure()[0][:6]) == [4, 1, 5, 1, 4, 1]
+    R, tf = 1024, 2.0
+    res = m.simulate(tf, R, seed=808, precision=prec, mode=mode, block_size=bs)
+    ora = om.simulate(R, tf, seed=808, precision=prec)
+    assert res.n_failed == 0 and (ora["status"] == 0).all()
+    same = _compare(res.final_x, ora["x_final"])
+    assert np.array_equal(res.steps[same], ora["steps"][same])
+
+
 def test_conservation_laws(cuda):
     # cyclic chain: sum_i x_i constant; colloid: sum_i i * x_i constant
     cyc = models.cyclic_chain(256, seed=1)

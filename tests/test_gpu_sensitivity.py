@@ -73,6 +73,20 @@ def test_exact_sampler_scores(cuda):
     assert abs(res.dmean_dc[0, 0] - 0.05) < 4.5 * res.dmean_dc_se[0, 0]
 
 
+@pytest.mark.parametrize("bs", [64, 1024])
+def test_three_reactant_scores_match_oracle(cuda, bs):
+    from helpers import trimolecular_net
+    m, om = pair(trimolecular_net())
+    R = 512
+    res = m.simulate(1.0, R, seed=31, sensitivities=True, return_scores=True, block_size=bs)
+    ora = om.simulate(R, 1.0, seed=31, sensitivities=True)
+    same = np.all(res.final_x == ora["x_final"], axis=1) & (res.steps == ora["steps"])
+    assert same.mean() >= 0.97, same.mean()
+    S, So = res.score[same], ora["score"][same]
+    err = np.abs(S - So) / (np.abs(So).max(axis=1, keepdims=True) + 1e-300)
+    assert np.quantile(err.max(axis=1), 0.99) < 1e-8
+
+
 def test_dmean_matches_closed_form_and_fd(cuda):
     N, tf = 8, 0.5
     rng = np.random.default_rng(4)

@@ -104,10 +104,14 @@ def _random_tiny(seed, with_catalyst=True, with_sources=True):
     M = int(rng.integers(4, 9))
     rx = []
     for _ in range(M):
-        t = int(rng.integers(0, 4)) if with_sources else int(rng.integers(1, 4))
+        t = int(rng.integers(0, 6)) if with_sources else int(rng.integers(1, 4))
         sp = rng.permutation(N)
         if t == 0:
             reac = {}
+        elif t == 4:  # three reactant molecules: Sa + Sb + Sc
+            reac = {int(sp[0]): 1, int(sp[1]): 1, int(sp[2]): 1}
+        elif t == 5:  # 2Sa + Sb
+            reac = {int(sp[0]): 2, int(sp[1]): 1}
         elif t == 1:
             reac = {int(sp[0]): 1}
         elif t == 2:
@@ -115,7 +119,7 @@ def _random_tiny(seed, with_catalyst=True, with_sources=True):
         else:
             reac = {int(sp[0]): 2}
         prod = {}
-        for s in sp[2:2 + int(rng.integers(0, 3))]:
+        for s in sp[3:3 + int(rng.integers(0, 3))] if t == 4 else sp[2:2 + int(rng.integers(0, 3))]:
             prod[int(s)] = int(rng.integers(1, 3))
         if with_catalyst and reac and rng.random() < 0.25:
             s0 = next(iter(reac))
@@ -129,6 +133,13 @@ def _exact_prop(rx, x):
     items = list(rx.reactants.items())
     if not items:  # a source
         return c, False
+    if len(items) == 3:  # Sa + Sb + Sc
+        (i, _), (j, _), (k, _) = items
+        return c * x[i] * x[j] * x[k], x[i] < 5 or x[j] < 5 or x[k] < 5
+    if len(items) == 2 and 2 in dict(items).values():  # 2Sa + Sb
+        (i, _), = [it for it in items if it[1] == 2]
+        (j, _), = [it for it in items if it[1] == 1]
+        return c * x[i] * (x[i] - 1) / 2 * x[j], 2 * x[i] < 5 or x[j] < 5
     if len(items) == 2:
         (i, _), (j, _) = items
         return c * x[i] * x[j], x[i] < 5 or x[j] < 5
@@ -176,11 +187,20 @@ def test_stages_exact_rational(seed):
                 continue
             o = net.reactions[i]  # GetG is indexed by species through the channel order array
             kinds = list(o.reactants.values())
-            order = 0 if not kinds else (2 if len(kinds) == 2 else (3 if kinds[0] == 2 else 1))
-            g = {0: Fraction(1), 1: Fraction(1), 2: Fraction(2), 3: None}[order]  # a source takes g = 1
+            if not kinds:
+                order = 0
+            elif len(kinds) == 3:
+                order = 4
+            elif len(kinds) == 2:
+                order = 5 if 2 in kinds else 2
+            else:
+                order = 3 if kinds[0] == 2 else 1
+            g = {0: Fraction(1), 1: Fraction(1), 2: Fraction(2), 3: None, 4: Fraction(3), 5: None}[order]
             xi = xr[i]
             if order == 3:
                 g = None if xi == 1 else Fraction(2) + Fraction(1, xi - 1)
+            if order == 5:
+                g = None if xi == 1 else Fraction(3, 2) * (Fraction(2) + Fraction(1, xi - 1))
             q = Fraction(0) if g is None else eps * xi / g
             tmp = max(q, Fraction(1))
             t1 = None if mu == 0 else tmp / abs(mu)
@@ -304,6 +324,48 @@ def test_ptrs_is_poisson(mean):
     assert _chi2_poisson(x, mean) > 1e-3
     assert abs(x.mean() - mean) < 4 * np.sqrt(mean / x.size)
     assert abs(x.var() / mean - 1) < 0.015
+
+
+def _death_chain_law(rates, t):
+    """Law at t of a pure death chain n -> n+1 (states 0..len(rates)) with
+    state-n exit rates `rates[n]`, from state 0: exp(t Q) of the master
+    equation's generator."""
+    n = len(rates) + 1
+    Q = np.zeros((n, n))
+    for i, r in enumerate(rates):
+        Q[i, i] = -r
+        Q[i, i + 1] = r
+    return linalg.expm(t * Q)[0]
+
+
+@pytest.mark.parametrize("kind", ["abc", "2ab"])
+def test_three_reactant_law(kind):
+    """Channels with three reactant molecules in the direct-method regime,
+    against the chemical master equation solved exactly: A + B + C -> 0 at
+    rate k a b c from (6, 6, 6), and 2A + B -> 0 at rate k a(a-1)/2 b from
+    (8, 5); the number of events by time t has the law exp(tQ) of a death
+    chain.  Chi-square over that law (an error in either propensity formula
+    changes the rates and fails it)."""
+    if kind == "abc":
+        k, t = 0.004, 1.0
+        net = Network(3, [Reaction({0: 1, 1: 1, 2: 1}, {}, k)], np.array([6, 6, 6], np.int32))
+        rates = [k * (6 - m) ** 3 for m in range(6)]
+    else:
+        k, t = 0.05, 1.0
+        net = Network(2, [Reaction({0: 2, 1: 1}, {}, k)], np.array([8, 5], np.int32))
+        rates = [k * (8 - 2 * m) * (7 - 2 * m) / 2 * (5 - m) for m in range(4)]
+    om = oracle_of(net)
+    res = om.simulate(20000, t, seed=17)
+    assert (res["status"] == 0).all()
+    events = net.x0[0] - res["x_final"][:, 0]
+    if kind == "2ab":
+        events //= 2
+    law = _death_chain_law(rates, t)
+    obs = np.bincount(events, minlength=law.size).astype(float)
+    keep = law * events.size >= 5
+    exp_ = np.append(law[keep] * events.size, events.size - (law[keep] * events.size).sum())
+    obs_k = np.append(obs[keep], events.size - obs[keep].sum())
+    assert stats.chisquare(obs_k, exp_).pvalue > 1e-3
 
 
 def _immigration_death(k, g, s0):
@@ -462,10 +524,12 @@ def test_model_validation():
 
     build([Reaction({0: 1}, {1: 1}, 1.0)])
     build([Reaction({}, {1: 1}, 1.0)])  # a source (order 0) is accepted
+    om = build([Reaction({0: 1, 1: 1, 2: 1}, {}, 1.0), Reaction({0: 1, 1: 2}, {}, 1.0)])  # Sa+Sb+Sc, Sa+2Sb
+    assert list(om.order) == [4, 5] and list(om.s0) == [0, 1] and list(om.s1) == [1, 0]
     with pytest.raises(ValueError, match="status 3"):
-        build([Reaction({0: 1, 1: 1, 2: 1}, {}, 1.0)])  # three reactants
+        build([Reaction({0: 2, 1: 2}, {}, 1.0)])  # 2S + 2S'
     with pytest.raises(ValueError, match="status 3"):
-        build([Reaction({0: 1, 1: 2}, {}, 1.0)])  # S + 2S'
+        build([Reaction({0: 1, 1: 1, 2: 1}, {}, 1.0)] + [Reaction({0: 2, 2: 2}, {}, 1.0)])
     with pytest.raises(ValueError, match="status 4"):
         build([Reaction({0: 3}, {}, 1.0)])  # stoichiometry -3 does not pack in 3 bits
     cat = Network(2, [Reaction({0: 1}, {0: 1, 1: 1}, 1.0)], np.array([5, 0], np.int32))
