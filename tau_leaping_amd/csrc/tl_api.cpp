@@ -281,8 +281,9 @@ tl_status check_params(const tl_model *m, const tl_sim_params *p) {
     return TL_OK;
 }
 
-// Whether a run needs the kernels with three-reactant channels and the
-// exact sampler compiled in.
+// Whether a run needs the kernels with sources, three-reactant channels and
+// the exact sampler compiled in (s2 is present exactly when the model has a
+// source or a three-reactant channel).
 bool needs_ext(const tl_model *m, const tl_sim_params *p) {
     return !m->s2.empty() || p->poisson == TL_POISSON_EXACT;
 }

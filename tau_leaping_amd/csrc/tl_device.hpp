@@ -45,7 +45,7 @@ struct DevModel {
     const int8_t *order;     // [M] 1, 2, 3
     const int32_t *s0;       // [M]
     const int32_t *s1;       // [M]
-    const int32_t *s2;       // [M] third reactant of Sa+Sb+Sc channels; null unless some order >= 4
+    const int32_t *s2;       // [M] third reactant of Sa+Sb+Sc channels; null unless some order is 0, 4 or 5
     const int32_t *row_ptr;  // [N+1] species-major
     const int32_t *csr;      // [nnz] (channel << 3) | ((v + 2) & 7)
     const int32_t *col_ptr;  // [M+1] channel-major
@@ -109,28 +109,33 @@ template <> struct Real<float> {
     static __device__ __forceinline__ const float *rates(const DevModel &m) { return m.c32; }
 };
 
-// Propensity of one channel; operation order (c*x0)*x1 and ((c*x0)*(x0-1))/2
-// as in IsCriticalAndPropensity.  Order 0 (a source with no reactant) has
-// a = c and is never critical.
+// Propensity of one channel of order 1-3; operation order (c*x0)*x1 and
+// ((c*x0)*(x0-1))/2 as in IsCriticalAndPropensity.
 // Written without branches (channels of different orders share a wave); the
 // roundings are those of the three formulas: c*x0, (c*x0)*f, then an exact
 // halving for order 3.
 template <class R>
 __device__ __forceinline__ R propensity(int order, R c, int32_t x0, int32_t x1, bool &crit) {
-    const bool o0 = order == 0, o2 = order == 2, o3 = order == 3;
-    crit = !o0 && (o3 ? (x0 * 2 < kNc) : ((x0 < kNc) || (o2 && x1 < kNc)));
-    const R t = o0 ? c : c * (R)x0;
+    const bool o2 = order == 2, o3 = order == 3;
+    crit = o3 ? (x0 * 2 < kNc) : ((x0 < kNc) || (o2 && x1 < kNc));
+    const R t = c * (R)x0;
     const R f = (R)(o2 ? x1 : x0 - 1);
     const R t2 = (o2 || o3) ? t * f : t;
     return o3 ? t2 * (R)0.5 : t2;
 }
 
-// Channels with three reactant molecules (models with m.s2 != null):
-// order 4 Sa+Sb+Sc ((c*x0)*x1)*x2, order 5 2Sa+Sb (((c*x0)*(x0-1))*0.5)*x1
-// (x0 the species taken twice); critical when a reactant count is below
-// nc times its coefficient.  Other orders as propensity().
+// The forms beyond the reference's (models with m.s2 != null): order 0, a
+// source, a = c and never critical; channels with three reactant molecules,
+// order 4 Sa+Sb+Sc ((c*x0)*x1)*x2 and order 5 2Sa+Sb
+// (((c*x0)*(x0-1))*0.5)*x1 (x0 the species taken twice), critical when a
+// reactant count is below nc times its coefficient.  Orders 1-3 as
+// propensity().
 template <class R>
 __device__ __forceinline__ R propensity_ext(int order, R c, int32_t x0, int32_t x1, int32_t x2, bool &crit) {
+    if (order == 0) {
+        crit = false;
+        return c;
+    }
     if (order == 4) {
         crit = (x0 < kNc) || (x1 < kNc) || (x2 < kNc);
         return ((c * (R)x0) * (R)x1) * (R)x2;

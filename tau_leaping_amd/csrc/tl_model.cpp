@@ -130,10 +130,11 @@ tl_status build_model(tl_model &m, std::string &msg) {
                   ": unsupported reactant pattern (need ->, S->, 2S->, Sa+Sb->, Sa+Sb+Sc-> or 2Sa+Sb->)";
             return TL_ERR_UNSUPPORTED_ORDER;
         }
-        ext |= m.order[j] >= 4;
+        ext |= m.order[j] == 0 || m.order[j] >= 4;
     }
-    // third reactant of Sa+Sb+Sc channels; empty unless some channel has
-    // three reactant molecules (the kernels then take the extended formulas)
+    // third reactant of Sa+Sb+Sc channels; empty unless some channel is a
+    // source or has three reactant molecules (the kernels then take the
+    // extended formulas)
     if (ext) m.s2.swap(third);
     else m.s2.clear();
     // species-major transpose, rows in channel order

@@ -36,7 +36,7 @@ struct tl_model {
     // derived
     std::vector<int8_t> order;
     std::vector<int32_t> s0, s1;
-    std::vector<int32_t> s2;  // third reactant of order-4 channels; empty unless an order >= 4 exists
+    std::vector<int32_t> s2;  // third reactant of order-4 channels; empty unless an order 0, 4 or 5 exists
     std::vector<int32_t> row_ptr, csr;  // species-major, (channel << 3) | ((v + 2) & 7)
     std::vector<int32_t> csc;           // channel-major, (species << 3) | ((v + 2) & 7)
     std::vector<int8_t> g_order;        // order[i] for i < M else 0

@@ -24,8 +24,8 @@
 
 // This file is compiled twice: as is (TL_PERSISTENT_EXT 0, the kernels for
 // the reference's reaction forms and sampler, plus the host side), and from
-// tl_persistent_ext.hip (TL_PERSISTENT_EXT 1: the same kernels with channels
-// of three reactant molecules and the exact Poisson sampler).  Keeping those
+// tl_persistent_ext.hip (TL_PERSISTENT_EXT 1: the same kernels with sources,
+// channels of three reactant molecules and the exact Poisson sampler).  Keeping those
 // paths out of the default instances keeps their registers out of its hot
 // loops.
 #ifndef TL_PERSISTENT_EXT
@@ -647,8 +647,8 @@ __device__ __forceinline__ void chan_unpack(int2 p, int &order, int32_t &s0, int
 }
 
 // a_j and criticality of channel j from its packed reactants.  s2 (a
-// model-wide pointer, null unless some channel has three reactant molecules)
-// selects the extended formulas with a scalar branch.
+// model-wide pointer, null unless some channel is a source or has three
+// reactant molecules) selects the extended formulas with a scalar branch.
 template <class R>
 __device__ __forceinline__ R chan_prop(int2 p, R cj, const int32_t *xs, gptr<const int32_t> s2, int32_t j, bool &cr) {
     int order;
