@@ -228,3 +228,16 @@ def test_large_generated_model(tmp_path):
     assert (m.n_species, m.n_channels) == (20000, 100000)
     assert t_gen < 10 and t_read < 10, (t_gen, t_read)
     assert_same(p)
+
+
+def test_extended_forms_round_trip(tmp_path):
+    """Sources (an empty Reactants list) and channels with three reactant
+    molecules read back as orders 0, 4 and 5."""
+    from helpers import trimolecular_net
+
+    net = trimolecular_net()
+    p = tmp_path / "tri.xml"
+    p.write_text(net.to_stochkit_xml())
+    m = assert_same(p)
+    order = list(m.structure()[0])
+    assert order[:7] == [4, 1, 5, 1, 4, 1, 0] and order[-1] == 5
