@@ -64,11 +64,11 @@ enum { TL_POISSON_NORMAL = 0, TL_POISSON_EXACT = 1 };
 
 /* Engine forms: PLAIN = per-model host loop of stage kernels over
  * structure-of-arrays state in HBM; PERSISTENT = one work-stealing kernel that
- * runs whole trajectories with each realization's state in LDS (all models of
- * a batch in one launch); AUTO = PERSISTENT, except that a model whose state
- * does not fit any persistent launch shape (LDS) runs PLAIN when the call asks
- * for neither sensitivities nor a time grid (PERSISTENT returns
- * TL_ERR_TOO_LARGE for such a model). */
+ * runs whole trajectories with each realization's state in LDS, or in a slice
+ * of HBM per block for models too large for LDS (all models of a batch in one
+ * launch); AUTO = PERSISTENT, except that a model for which no persistent
+ * launch shape exists at all runs PLAIN when the call asks for neither
+ * sensitivities nor a time grid (PERSISTENT returns TL_ERR_TOO_LARGE). */
 enum { TL_MODE_AUTO = 0, TL_MODE_PLAIN = 1, TL_MODE_PERSISTENT = 2 };
 
 typedef struct tl_model tl_model;

@@ -336,9 +336,10 @@ def _to_result(r: _Result, bufs, track) -> SimResult:
 def batch_simulate(model_list: Sequence[Model], kwargs_list: Sequence[dict], mode="auto",
                    want_final=True) -> List[SimResult]:
     """Run many models; ``mode='persistent'`` is one work-stealing launch for
-    the whole batch, ``'plain'`` the per-model stage-kernel loop, ``'auto'``
-    (default) persistent except for models whose state does not fit LDS,
-    which run plain (unless sensitivities or a time grid are asked for)."""
+    the whole batch (a model too large for LDS keeps its realizations' state
+    in HBM), ``'plain'`` the per-model stage-kernel loop, ``'auto'`` (default)
+    persistent, plain only for a model no persistent launch shape can run
+    (unless sensitivities or a time grid are asked for)."""
     n = len(model_list)
     params = (_Params * n)()
     keep, outs = [], []
