@@ -168,6 +168,31 @@ def test_model_with_many_tiles_stores_a_in_hbm(cuda):
     assert np.array_equal(res.steps[same], ora["steps"][same])
 
 
+def test_large_model_with_extended_forms(cuda):
+    # a model too large for LDS that also has sources and three-reactant
+    # channels: the HBM-state kernels of the extended build, against the oracle
+    import tl_oracle as O
+    from helpers import toy_csc
+
+    N, M = 6000, 12000
+    col_ptr, row_ind, stoich, rates, x0 = toy_csc(N, M - 2, 11)
+    # append a source (-> S0) and S1 + S2 + S3 -> S4
+    col_ptr = np.append(col_ptr, [col_ptr[-1] + 1, col_ptr[-1] + 5]).astype(np.int32)
+    row_ind = np.append(row_ind, [0, 1, 2, 3, 4]).astype(np.int32)
+    stoich = np.append(stoich, [1, -1, -1, -1, 1]).astype(np.int8)
+    rates = np.append(rates, [50.0, 1e-9])
+    m = T.Model.from_csc(N, col_ptr, row_ind, stoich, rates, x0)
+    om = O.OracleModel(N, M, col_ptr, row_ind, stoich, rates, x0)
+    assert list(m.structure()[0][-2:]) == [0, 4]
+    R, tf = 128, 1e-4
+    track = np.arange(0, N, 150)
+    res = m.simulate(tf, R, seed=21, track=track)
+    ora = om.simulate(R, tf, seed=21)
+    assert res.block_size == 1024 and res.n_failed == 0 and (ora["status"] == 0).all()
+    same = _compare(res.final_x, ora["x_final"][:, track], min_same=0.95)
+    assert np.array_equal(res.steps[same], ora["steps"][same])
+
+
 def _source_network():
     # sources, decay, dimerisation (2A -> B), a reversible association and a
     # catalytic step (B + D -> B + E)
