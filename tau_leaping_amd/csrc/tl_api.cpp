@@ -410,7 +410,7 @@ tl_status run_plain(const tl_model *m, const tl_sim_params *p, tl_result *res) {
         launch_prop(is64, dm, R, nchunk, x, a, crit, a0, a0c, part, partc, st.active, s);
         launch_tau(is64, dm, R, p->eps, x, a, crit, tau_i, which_i, tau1, nullptr, nullptr, st.active, s);
         launch_plain_select(is64, dm, R, seed, p->model_id, p->realization_offset, p->t_final, p->max_iter, x, a, crit,
-                            a0, a0c, tau1, st, s);
+                            a0, a0c, tau1, st, nchunk, partc, s);
         launch_fire(is64, dm, R, seed, p->model_id, p->realization_offset, st.epoch, a, crit, st.tau, st.jc, k, st.mode,
                     p->poisson == TL_POISSON_EXACT, s);
         launch_update(dm, R, x, k, xn, st.neg, st.mode, s);

@@ -85,7 +85,8 @@ struct PlainState {
 void launch_plain_init(const DevModel &m, int64_t R, int32_t *x, PlainState st, hipStream_t s);
 void launch_plain_select(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id,
                          int64_t rbase, double t_final, int64_t max_iter, int32_t *x, void *a, const uint8_t *crit,
-                         const double *a0, const double *a0c, const double *tau1, PlainState st, hipStream_t s);
+                         const double *a0, const double *a0c, const double *tau1, PlainState st, int32_t nchunk,
+                         const double *partc, hipStream_t s);
 void launch_plain_commit(const DevModel &m, int64_t R, int32_t *x, const int32_t *x_new, PlainState st,
                          double t_final, hipStream_t s);
 void launch_gather_track(int64_t R, int32_t n_track, const int32_t *track, const int32_t *x, int32_t *out,

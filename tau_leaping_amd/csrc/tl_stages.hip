@@ -136,7 +136,7 @@ __global__ __launch_bounds__(kT) void k_tau_min(int32_t N, int64_t Rn, const R *
 template <class R>
 __device__ void select_leap(const DevModel &m, uint2 key, uint32_t c3, uint32_t epoch, const R *a, const uint8_t *crit,
                             int64_t Rn, int64_t r, double a0c, double tau1, double t_rem, double &tau, int32_t &jc_out,
-                            uint8_t &clamped) {
+                            uint8_t &clamped, int32_t nchunk = 1, const double *partc = nullptr) {
     double r1, r2;
     step_uniforms(key, epoch, c3 | kKindLeap, r1, r2);
     int32_t jc = m.M;
@@ -144,7 +144,20 @@ __device__ void select_leap(const DevModel &m, uint2 key, uint32_t c3, uint32_t 
     if (a0c > 0.0) {
         const double target = a0c * r1;
         double cum = 0.0;
-        for (int32_t j = 0; j < m.M; ++j) {
+        int32_t jstart = 0;
+        if (partc && nchunk > 1) {
+            // skip whole channel chunks by their critical sums (the propensity
+            // pass's partials), then walk channels from the chunk reached
+            const int32_t ch = (m.M + nchunk - 1) / nchunk;
+            for (int32_t c = 0; c + 1 < nchunk; ++c) {
+                const double p = partc[(int64_t)c * Rn + r];
+                if (cum + p > target) break;
+                cum += p;
+                jstart += ch;
+            }
+            jstart = min(jstart, m.M);
+        }
+        for (int32_t j = jstart; j < m.M; ++j) {
             if (!crit[(int64_t)j * Rn + r]) continue;
             cum += (double)a[(int64_t)j * Rn + r];
             if (cum > target) {
@@ -330,7 +343,7 @@ __global__ __launch_bounds__(kT) void k_plain_select(DevModel m, int64_t Rn, uin
                                                      int32_t *__restrict__ x, R *__restrict__ a,
                                                      const uint8_t *__restrict__ crit, const double *__restrict__ a0,
                                                      const double *__restrict__ a0c, const double *__restrict__ tau1,
-                                                     PlainState st) {
+                                                     PlainState st, int32_t nchunk, const double *__restrict__ partc) {
     const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
     if (r >= Rn) return;
     st.neg[r] = 0;
@@ -385,7 +398,7 @@ __global__ __launch_bounds__(kT) void k_plain_select(DevModel m, int64_t Rn, uin
     double tau;
     int32_t jc;
     uint8_t cl;
-    select_leap<R>(m, key, c3, st.epoch[r], a, crit, Rn, r, a0c[r], tau1r, tf - t, tau, jc, cl);
+    select_leap<R>(m, key, c3, st.epoch[r], a, crit, Rn, r, a0c[r], tau1r, tf - t, tau, jc, cl, nchunk, partc);
     st.tau[r] = tau;
     st.jc[r] = jc;
     st.clamped[r] = cl;
@@ -532,13 +545,14 @@ void launch_plain_init(const DevModel &m, int64_t R, int32_t *x, PlainState st, 
 
 void launch_plain_select(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase,
                          double t_final, int64_t max_iter, int32_t *x, void *a, const uint8_t *crit, const double *a0,
-                         const double *a0c, const double *tau1, PlainState st, hipStream_t s) {
+                         const double *a0c, const double *tau1, PlainState st, int32_t nchunk, const double *partc,
+                         hipStream_t s) {
     if (is64)
         k_plain_select<double><<<blocks_for(R), kT, 0, s>>>(m, R, seed, model_id, rbase, t_final, max_iter, x,
-                                                            (double *)a, crit, a0, a0c, tau1, st);
+                                                            (double *)a, crit, a0, a0c, tau1, st, nchunk, partc);
     else
         k_plain_select<float><<<blocks_for(R), kT, 0, s>>>(m, R, seed, model_id, rbase, t_final, max_iter, x,
-                                                           (float *)a, crit, a0, a0c, tau1, st);
+                                                           (float *)a, crit, a0, a0c, tau1, st, nchunk, partc);
 }
 
 void launch_plain_commit(const DevModel &m, int64_t R, int32_t *x, const int32_t *x_new, PlainState st,
