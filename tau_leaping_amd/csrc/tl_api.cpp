@@ -374,6 +374,14 @@ tl_status run_plain(const tl_model *m, const tl_sim_params *p, tl_result *res) {
         TL_TRY(B.alloc(part, (size_t)nchunk * R));
         TL_TRY(B.alloc(partc, (size_t)nchunk * R));
     }
+    // species-row chunks for the tau minimum, likewise
+    const int32_t nrc = (int32_t)std::min<int64_t>(std::max<int64_t>(1, (262144 + R - 1) / R), std::max(1, m->N / 64));
+    unsigned char *pbest = nullptr;
+    int32_t *pidx = nullptr;
+    if (nrc > 1) {
+        TL_TRY(B.alloc(pbest, (size_t)nrc * R * rs));
+        TL_TRY(B.alloc(pidx, (size_t)nrc * R));
+    }
     TL_TRY(B.alloc(d_track, T));
     TL_TRY(B.alloc(d_outx, (size_t)T * R));
     PlainState st;
@@ -408,7 +416,8 @@ tl_status run_plain(const tl_model *m, const tl_sim_params *p, tl_result *res) {
     const uint64_t seed = p->seed;
     for (int64_t it = 0;; ++it) {
         launch_prop(is64, dm, R, nchunk, x, a, crit, a0, a0c, part, partc, st.active, s);
-        launch_tau(is64, dm, R, p->eps, x, a, crit, tau_i, which_i, tau1, nullptr, nullptr, st.active, s);
+        launch_tau(is64, dm, R, p->eps, x, a, crit, tau_i, which_i, tau1, nullptr, nullptr, st.active, s, nrc, pbest,
+                   pidx);
         launch_plain_select(is64, dm, R, seed, p->model_id, p->realization_offset, p->t_final, p->max_iter, x, a, crit,
                             a0, a0c, tau1, st, nchunk, partc, s);
         launch_fire(is64, dm, R, seed, p->model_id, p->realization_offset, st.epoch, a, crit, st.tau, st.jc, k, st.mode,

@@ -55,9 +55,12 @@ tl_status scratch(DeviceContext &ctx, int slot, size_t bytes, void *&ptr);
 void launch_prop(bool is64, const DevModel &m, int64_t R, int32_t nchunk, const int32_t *x, void *a,
                  uint8_t *crit, double *a0, double *a0c, double *part, double *partc, const uint8_t *active,
                  hipStream_t s);
+// nrc > 1 with scratch pbest [nrc][R] (propensity type) and pidx [nrc][R]:
+// the species minimum in two passes over nrc row chunks.
 void launch_tau(bool is64, const DevModel &m, int64_t R, double eps, const int32_t *x, const void *a,
                 const uint8_t *crit, void *tau_i, uint8_t *which_i, double *tau1, int32_t *argmin,
-                int32_t *which, const uint8_t *active, hipStream_t s);
+                int32_t *which, const uint8_t *active, hipStream_t s, int32_t nrc = 1, void *pbest = nullptr,
+                int32_t *pidx = nullptr);
 void launch_select(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase,
                    const uint32_t *epoch, const void *a, const uint8_t *crit, const double *a0c,
                    const double *tau1, const double *t_rem, double *tau, int32_t *jc, uint8_t *clamped,

@@ -132,6 +132,56 @@ __global__ __launch_bounds__(kT) void k_tau_min(int32_t N, int64_t Rn, const R *
     if (which) which[r] = bw;
 }
 
+// The same minimum in two passes for many species: chunk c of the rows
+// (blockIdx.y) writes its first minimum, then one thread per realization
+// combines the chunks in order.  Strict comparisons in ascending row order
+// give the sequential result exactly.
+template <class R>
+__global__ __launch_bounds__(kT) void k_tau_min_part(int32_t N, int64_t Rn, int32_t nrc, const R *__restrict__ tau_i,
+                                                     R *__restrict__ pbest, int32_t *__restrict__ pidx,
+                                                     const uint8_t *__restrict__ active) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= Rn) return;
+    if (active && !active[r]) return;
+    const int32_t c = blockIdx.y;
+    const int32_t ch = (N + nrc - 1) / nrc;
+    const int32_t i0 = c * ch, i1 = min(N, i0 + ch);
+    R best = (R)kFltMax;
+    int32_t bi = -1;
+    for (int32_t i = i0; i < i1; ++i) {
+        const R ti = tau_i[(int64_t)i * Rn + r];
+        if (ti < best) {
+            best = ti;
+            bi = i;
+        }
+    }
+    pbest[(int64_t)c * Rn + r] = best;
+    pidx[(int64_t)c * Rn + r] = bi;
+}
+
+template <class R>
+__global__ __launch_bounds__(kT) void k_tau_min_fin(int64_t Rn, int32_t nrc, const R *__restrict__ pbest,
+                                                    const int32_t *__restrict__ pidx,
+                                                    const uint8_t *__restrict__ which_i, double *__restrict__ tau1,
+                                                    int32_t *__restrict__ argmin, int32_t *__restrict__ which,
+                                                    const uint8_t *__restrict__ active) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= Rn) return;
+    if (active && !active[r]) return;
+    R best = (R)kFltMax;
+    int32_t bi = -1;
+    for (int32_t c = 0; c < nrc; ++c) {
+        const R v = pbest[(int64_t)c * Rn + r];
+        if (v < best) {
+            best = v;
+            bi = pidx[(int64_t)c * Rn + r];
+        }
+    }
+    tau1[r] = (double)best;
+    if (argmin) argmin[r] = bi;
+    if (which) which[r] = bi >= 0 ? which_i[(int64_t)bi * Rn + r] : 0;
+}
+
 // --------------------------------------------------------------- select --
 template <class R>
 __device__ void select_leap(const DevModel &m, uint2 key, uint32_t c3, uint32_t epoch, const R *a, const uint8_t *crit,
@@ -479,16 +529,31 @@ void launch_prop(bool is64, const DevModel &m, int64_t R, int32_t nchunk, const 
 
 void launch_tau(bool is64, const DevModel &m, int64_t R, double eps, const int32_t *x, const void *a,
                 const uint8_t *crit, void *tau_i, uint8_t *which_i, double *tau1, int32_t *argmin, int32_t *which,
-                const uint8_t *active, hipStream_t s) {
+                const uint8_t *active, hipStream_t s, int32_t nrc, void *pbest, int32_t *pidx) {
     dim3 g(blocks_for(R), row_grid(m.N));
+    const bool two = nrc > 1 && pbest && pidx;
     if (is64) {
         k_tau_rows<double><<<g, kT, 0, s>>>(m, R, eps, x, (const double *)a, crit, (double *)tau_i, which_i, active);
-        k_tau_min<double><<<blocks_for(R), kT, 0, s>>>(m.N, R, (const double *)tau_i, which_i, tau1, argmin, which,
-                                                       active);
+        if (two) {
+            k_tau_min_part<double><<<dim3(blocks_for(R), nrc), kT, 0, s>>>(m.N, R, nrc, (const double *)tau_i,
+                                                                          (double *)pbest, pidx, active);
+            k_tau_min_fin<double><<<blocks_for(R), kT, 0, s>>>(R, nrc, (const double *)pbest, pidx, which_i, tau1,
+                                                               argmin, which, active);
+        } else {
+            k_tau_min<double><<<blocks_for(R), kT, 0, s>>>(m.N, R, (const double *)tau_i, which_i, tau1, argmin, which,
+                                                           active);
+        }
     } else {
         k_tau_rows<float><<<g, kT, 0, s>>>(m, R, (float)eps, x, (const float *)a, crit, (float *)tau_i, which_i, active);
-        k_tau_min<float><<<blocks_for(R), kT, 0, s>>>(m.N, R, (const float *)tau_i, which_i, tau1, argmin, which,
-                                                      active);
+        if (two) {
+            k_tau_min_part<float><<<dim3(blocks_for(R), nrc), kT, 0, s>>>(m.N, R, nrc, (const float *)tau_i,
+                                                                         (float *)pbest, pidx, active);
+            k_tau_min_fin<float><<<blocks_for(R), kT, 0, s>>>(R, nrc, (const float *)pbest, pidx, which_i, tau1,
+                                                              argmin, which, active);
+        } else {
+            k_tau_min<float><<<blocks_for(R), kT, 0, s>>>(m.N, R, (const float *)tau_i, which_i, tau1, argmin, which,
+                                                          active);
+        }
     }
 }
 
