@@ -66,9 +66,12 @@ enum { TL_POISSON_NORMAL = 0, TL_POISSON_EXACT = 1 };
  * structure-of-arrays state in HBM; PERSISTENT = one work-stealing kernel that
  * runs whole trajectories with each realization's state in LDS, or in a slice
  * of HBM per block for models too large for LDS (all models of a batch in one
- * launch); AUTO = PERSISTENT, except that a model for which no persistent
- * launch shape exists at all runs PLAIN when the call asks for neither
- * sensitivities nor a time grid (PERSISTENT returns TL_ERR_TOO_LARGE). */
+ * launch); AUTO = PERSISTENT, except, when the call asks for neither
+ * sensitivities nor a time grid, for a model no persistent launch shape can
+ * run (PERSISTENT returns TL_ERR_TOO_LARGE) and for a model too large for
+ * LDS on which PLAIN ran the first 16 steps of a sample (4 realizations per
+ * CU, taken when R is at least twice that) more than 15 % faster: those run
+ * PLAIN.  The choice is cached per model version. */
 enum { TL_MODE_AUTO = 0, TL_MODE_PLAIN = 1, TL_MODE_PERSISTENT = 2 };
 
 typedef struct tl_model tl_model;

@@ -9,7 +9,7 @@
 // (StatToFile, statistics.cpp:89-111).  Extensions: --precision fp64|fp32,
 // --seed N, --poisson normal|exact (exact: PTRS instead of the normal
 // approximation above mean 80), --mode auto|persistent|plain (auto: persistent,
-// plain for a model no persistent launch shape can run), --block-size N, --sensitivities (writes
+// plain where it runs a model too large for LDS faster), --block-size N, --sensitivities (writes
 // dmean_<log>.txt: one line per tracked species, dE[X(t_final)]/dc_j),
 // --json (one JSON record of the run on stdout).
 //

@@ -980,15 +980,67 @@ void tl_sim_params_default(tl_sim_params *p) {
     p->mode = TL_MODE_AUTO;
 }
 
-// AUTO mode: whether a model runs in the plain form because its state fits
-// no persistent launch shape (and the call needs nothing only the persistent
-// form provides).
+// AUTO choice for a model whose state does not fit LDS, cached per model
+// version, precision and sampler build.
+struct FormKey {
+    const tl_model *m;
+    uint64_t version;
+    bool is64, ext;
+    bool operator<(const FormKey &o) const {
+        return std::tie(m, version, is64, ext) < std::tie(o.m, o.version, o.is64, o.ext);
+    }
+};
+std::mutex g_form_mu;
+std::map<FormKey, bool> g_form_plain;
+
+// AUTO mode: whether a model runs in the plain form (the call must need
+// nothing only the persistent form provides: scores, a time grid).  Always
+// when no persistent launch shape fits; and for a model whose state does not
+// fit LDS, when the plain form (realizations across the lanes, every gather
+// coalesced) runs the first steps of a sample more than 15 % faster than the
+// HBM-state persistent kernel.  Measured on 1e6 channels, all leaps: plain 4x
+// faster; a model that falls back to direct events stays persistent (the
+// plain form pays an O(M) pass per realization thread per event).
 static tl_status auto_plain(const tl_model *m, const tl_sim_params *p, bool &plain) {
     plain = false;
     if (p->sensitivities || p->n_intervals > 0) return TL_OK;
     DeviceContext *dc;
     TL_TRY(device_context(p->device, dc));
-    plain = candidates(*dc, p->precision == TL_FP64, false, m->N, m->M, needs_ext(m, p)).empty();
+    const bool is64 = p->precision == TL_FP64, ext = needs_ext(m, p);
+    const auto cands = candidates(*dc, is64, false, m->N, m->M, ext);
+    if (cands.empty()) {
+        plain = true;
+        return TL_OK;
+    }
+    if (!cands.front().hbm) return TL_OK;
+    const int64_t sample = std::min<int64_t>(p->n_realizations, (int64_t)dc->n_cu * 4);
+    if (p->n_realizations < 2 * sample) return TL_OK;  // the trial would cost as much as it saves
+    const FormKey key{m, m->version, is64, ext};
+    {
+        std::lock_guard<std::mutex> g(g_form_mu);
+        auto it = g_form_plain.find(key);
+        if (it != g_form_plain.end()) {
+            plain = it->second;
+            return TL_OK;
+        }
+    }
+    // both forms on the first 16 steps of the same sample of realizations
+    tl_sim_params t = *p;
+    t.n_realizations = sample;
+    t.max_iter = p->max_iter > 0 ? std::min<int64_t>(p->max_iter, 16) : 16;
+    t.n_track = 1;
+    const int32_t q0 = 0;
+    t.track = &q0;
+    tl_result rp = {}, rs = {};
+    t.mode = TL_MODE_PLAIN;
+    TL_TRY(run_plain(m, &t, &rp));
+    t.mode = TL_MODE_PERSISTENT;
+    TL_TRY(run_persistent(&m, 1, &t, &rs));
+    const double plain_rate = rp.kernel_ms > 0 ? (double)rp.total_steps / rp.kernel_ms : 0.0;
+    const double pers_rate = rs.kernel_ms > 0 ? (double)rs.total_steps / rs.kernel_ms : 0.0;
+    plain = plain_rate > 1.15 * pers_rate;
+    std::lock_guard<std::mutex> g(g_form_mu);
+    g_form_plain[key] = plain;
     return TL_OK;
 }
 

@@ -193,6 +193,28 @@ def test_large_model_with_extended_forms(cuda):
     assert np.array_equal(res.steps[same], ora["steps"][same])
 
 
+def test_auto_times_both_forms_for_large_models(cuda):
+    # 2e4 species x 1e5 channels, 4096 realizations, all leaps: AUTO times the
+    # plain form and the HBM-state persistent kernel on the first steps of a
+    # sample and runs the faster; the result is that form's, bit for bit, and
+    # its first realizations are the oracle's
+    import tl_oracle as O
+    from helpers import toy_csc
+
+    N, M, R, tf = 20000, 100000, 4096, 1e-5
+    csc = toy_csc(N, M, 17)
+    m = T.Model.from_csc(N, *csc)
+    track = np.arange(0, N, 500)
+    res = m.simulate(tf, R, seed=8, track=track)
+    form = "plain" if res.block_size == 256 else "persistent"
+    ref = m.simulate(tf, R, seed=8, track=track, mode=form)
+    assert res.n_failed == 0
+    assert np.array_equal(res.final_x, ref.final_x) and np.array_equal(res.steps, ref.steps)
+    ora = O.OracleModel(N, M, *csc).simulate(16, tf, seed=8)
+    same = np.all(res.final_x[:16] == ora["x_final"][:, track], axis=1)
+    assert same.mean() >= 0.8, same.mean()
+
+
 def _source_network():
     # sources, decay, dimerisation (2A -> B), a reversible association and a
     # catalytic step (B + D -> B + E)
