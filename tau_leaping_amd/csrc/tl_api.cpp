@@ -981,13 +981,16 @@ void tl_sim_params_default(tl_sim_params *p) {
 }
 
 // AUTO choice for a model whose state does not fit LDS, cached per model
-// version, precision and sampler build.
+// (handle, sizes and version: a later model at a freed handle's address has
+// other sizes or starts a fresh choice only by chance, and a stale choice
+// costs speed, never results), precision and kernel build.
 struct FormKey {
     const tl_model *m;
+    int32_t N, M, nnz;
     uint64_t version;
     bool is64, ext;
     bool operator<(const FormKey &o) const {
-        return std::tie(m, version, is64, ext) < std::tie(o.m, o.version, o.is64, o.ext);
+        return std::tie(m, N, M, nnz, version, is64, ext) < std::tie(o.m, o.N, o.M, o.nnz, o.version, o.is64, o.ext);
     }
 };
 std::mutex g_form_mu;
@@ -1015,7 +1018,7 @@ static tl_status auto_plain(const tl_model *m, const tl_sim_params *p, bool &pla
     if (!cands.front().hbm) return TL_OK;
     const int64_t sample = std::min<int64_t>(p->n_realizations, (int64_t)dc->n_cu * 4);
     if (p->n_realizations < 2 * sample) return TL_OK;  // the trial would cost as much as it saves
-    const FormKey key{m, m->version, is64, ext};
+    const FormKey key{m, m->N, m->M, m->nnz, m->version, is64, ext};
     {
         std::lock_guard<std::mutex> g(g_form_mu);
         auto it = g_form_plain.find(key);
