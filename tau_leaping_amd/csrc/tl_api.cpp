@@ -981,9 +981,9 @@ void tl_sim_params_default(tl_sim_params *p) {
 }
 
 // AUTO choice for a model whose state does not fit LDS, cached per model
-// (handle, sizes and version: a later model at a freed handle's address has
-// other sizes or starts a fresh choice only by chance, and a stale choice
-// costs speed, never results), precision and kernel build.
+// handle, sizes and version, precision and kernel build.  A model created
+// later at a freed handle's address with the same sizes would reuse the
+// choice; that affects speed only, never results.
 struct FormKey {
     const tl_model *m;
     int32_t N, M, nnz;
