@@ -310,7 +310,7 @@ __device__ double prop_soa(const DevModel &m, const int32_t *x, R *a, uint8_t *c
 // false when the realization reached t_final instead.
 template <class R>
 __device__ bool ssa_event_soa(const DevModel &m, uint2 key, uint32_t c3, uint32_t &epoch, int32_t *x, int64_t Rn,
-                              int64_t r, double &t, double tf, const R *a, double a0) {
+                              int64_t r, double &t, double tf, const R *a, double a0, int32_t *fired = nullptr) {
     if (!(a0 > 0.0)) {
         t = tf;
         return false;
@@ -341,7 +341,25 @@ __device__ bool ssa_event_soa(const DevModel &m, uint2 key, uint32_t c3, uint32_
         x[(int64_t)entry_index(p) * Rn + r] += entry_value(p);
     }
     t += dt;
+    if (fired) *fired = jf;
     return true;
+}
+
+// Propensities after channel jf fired: only the channels that read a species
+// of jf's column change (affected-channel lists; a full pass without them),
+// then a0 as the sequential FP64 sum over all channels, as prop_soa gives it.
+template <class R>
+__device__ double refresh_soa(const DevModel &m, const int32_t *x, R *a, int64_t Rn, int64_t r, int32_t jf) {
+    if (!m.aff_ptr) return prop_soa<R>(m, x, a, nullptr, Rn, r, nullptr);
+    const R *rc = Real<R>::rates(m);
+    for (int32_t d = m.aff_ptr[jf]; d < m.aff_ptr[jf + 1]; ++d) {
+        const int32_t k = m.aff[d];
+        bool cr;
+        a[(int64_t)k * Rn + r] = soa_prop<R>(m, k, rc[k], x, Rn, r, cr);
+    }
+    double s = 0.0;
+    for (int32_t j = 0; j < m.M; ++j) s += (double)a[(int64_t)j * Rn + r];
+    return s;
 }
 
 template <class R>
@@ -427,9 +445,10 @@ __global__ __launch_bounds__(kT) void k_plain_select(DevModel m, int64_t Rn, uin
         uint32_t ep = st.epoch[r];
         double a0s = a0r;
         int64_t n = 0;
+        int32_t jf = -1;
         for (int s = 0; s < kSsaSteps; ++s) {
-            if (s > 0) a0s = prop_soa<R>(m, x, a, nullptr, Rn, r, nullptr);
-            if (!ssa_event_soa<R>(m, key, c3, ep, x, Rn, r, t, tf, a, a0s)) break;
+            if (s > 0) a0s = refresh_soa<R>(m, x, a, Rn, r, jf);
+            if (!ssa_event_soa<R>(m, key, c3, ep, x, Rn, r, t, tf, a, a0s, &jf)) break;
             ++n;
         }
         st.t[r] = t;
