@@ -489,9 +489,10 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
     const int shapes[][2] = {{64, 64}, {256, 64}, {256, 256}, {512, 512}, {1024, 1024}};
     // one-wavefront groups come in register budgets of 4, 5 and 6 waves per
     // SIMD; the first listed is the default.  Measured on the 128 x 256 toy
-    // network, FP64 without scores: 6 waves 3783 ms per launch, 5 waves 3907
-    // ms; with scores (more live values) 5 stays first, as in FP32.
-    const int wpe64_64[] = {6, 5, 4}, wpe64_64s[] = {5, 6, 4}, wpe64_32[] = {5, 6, 4};
+    // network without scores: 6 waves 3783 / 3324 ms per launch (FP64 /
+    // FP32), 5 waves 3907 / 3554 ms; with scores (more live values) 5 stays
+    // first.
+    const int wpe64[] = {6, 5, 4}, wpe64s[] = {5, 6, 4};
     // order within a shape = preference: a[M] in LDS first, except for
     // 1024-lane groups, where recomputed propensities with a large tile win
     // (the 31,250-channel colloid)
@@ -500,8 +501,7 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
             const int al = (sh[1] >= 1024) ? ai : 1 - ai;
             // 512-lane groups: also compiled for two blocks per CU
             const int wpe512[] = {4, 1};
-            const int *wl = sh[1] == 64 ? (is64 ? (sens ? wpe64_64s : wpe64_64) : wpe64_32)
-                                        : (sh[1] == 512 ? wpe512 : nullptr);
+            const int *wl = sh[1] == 64 ? (sens ? wpe64s : wpe64) : (sh[1] == 512 ? wpe512 : nullptr);
             const int nw = sh[1] == 64 ? 3 : (sh[1] == 512 ? 2 : 1);
             for (int w = 0; w < nw; ++w) {
                 PersistentConfig c{sh[0], sh[1], al != 0, wl ? wl[w] : 1};
