@@ -26,6 +26,8 @@ plain sample); the heterogeneous 64-model batch in one persistent launch
 against the plain per-model loop; large-model ingestion (generator and
 reader at 1e6 channels, the HBM-state persistent kernel on 256 realizations,
 and the default mode on 4096, where a timed trial picks the form).
+`--workload colloid` and `--workload large` time the 31,250-channel colloid
+and that 1e6-channel model instead of the toy network.
 """
 import argparse
 import json
@@ -212,9 +214,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--realizations", type=int, default=100000)
     ap.add_argument("--precision", default="fp64")
-    ap.add_argument("--workload", default="toy", choices=["toy", "colloid"],
+    ap.add_argument("--workload", default="toy", choices=["toy", "colloid", "large"],
                     help="toy: 128 x 256 toy network, 100,000 realizations, t_final 0.1 (the default line); "
-                         "colloid: 250-species colloid (31,250 channels), 4,096 realizations, t_final 0.05")
+                         "colloid: 250-species colloid (31,250 channels), 4,096 realizations, t_final 0.05; "
+                         "large: toy_system_gen --toy network of 200,000 species x 1,000,000 channels, 4,096 "
+                         "realizations, t_final 1e-6, default mode (a timed choice of form)")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=4000)
@@ -242,10 +246,23 @@ def main():
     if args.workload == "colloid":
         net = models.colloidal(250, seed=44)
         R, tf = (args.realizations if args.realizations != 100000 else 4096), 0.05
+    elif args.workload == "large":
+        import subprocess
+        import tempfile
+        gen = os.path.join(ROOT, "tau_leaping_amd", "bin", "toy_system_gen")
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "toy200000x1000000.xml")
+            with open(path, "w") as f:
+                subprocess.run([gen, "-r", "200000", "-c", "1000000", "--toy", "--seed", "3", "--log-rates", "0.001",
+                                "1"], stdout=f, stderr=subprocess.DEVNULL, check=True)
+            m = T.Model.from_stochkit(path)
+        net = None
+        R, tf = (args.realizations if args.realizations != 100000 else 4096), 1e-6
     else:
         net = models.toy_network(128, 256, seed=42)
         R, tf = args.realizations, 0.1
-    m = T.Model.from_network(net)
+    if net is not None:
+        m = T.Model.from_network(net)
     fp64 = args.precision == "fp64"
     kw = dict(seed=1234, precision=args.precision, track=np.arange(4, dtype=np.int32))
 
@@ -263,12 +280,17 @@ def main():
     steps_total = sum(r.total_steps for r in res)
     kernel_ms = [r.kernel_ms for r in res]
     value = steps_total / wall
-    B = bytes_per_leap(net.n_species, net.n_channels, fp64)
+    B = bytes_per_leap(m.n_species, m.n_channels, fp64)
     avg_ms = float(np.mean(kernel_ms))
     achieved = B * (steps_total / len(res)) / (avg_ms * 1e-3) / 1e9
+    plain_form = res[-1].block_size == 256 and args.workload == "large"
     if args.workload == "colloid":
         workload = (f"colloid 250 species x {net.n_channels} channels, {R} realizations, t_final 0.05, eps 0.03, "
                     f"{args.precision}, persistent engine")
+    elif args.workload == "large":
+        workload = (f"toy network 200000 species x 1000000 channels (toy_system_gen --toy --seed 3), {R} "
+                    f"realizations, t_final 1e-06, eps 0.03, {args.precision}, default mode "
+                    f"({'plain form' if plain_form else 'persistent engine'} chosen)")
     else:
         workload = (f"toy network 128 species x 256 channels, {R} realizations, t_final 0.1, eps 0.03, "
                     f"{args.precision}, persistent engine")
@@ -285,12 +307,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if fp64 else "f32",
         "data": "synthetic",
-        "config": {"workload": workload, "realizations": R, "species": net.n_species, "channels": net.n_channels,
+        "config": {"workload": workload, "realizations": R, "species": m.n_species, "channels": m.n_channels,
                    "t_final": tf, "leaps_per_step": steps_total // len(res),
                    "ssa_events_per_step": res[-1].total_ssa_events, "block_size": res[-1].block_size},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(workload),
-                     "kernel": "k_persistent", "kernel_ms": avg_ms, "bytes_per_leap": B},
+                     "kernel": "plain stage kernels" if plain_form else "k_persistent", "kernel_ms": avg_ms,
+                     "bytes_per_leap": B},
     }
     if not args.no_extras and args.workload == "toy":
         line["extras"] = extras_run(T, models, np, m, dict(n_realizations=R, kw=kw))
