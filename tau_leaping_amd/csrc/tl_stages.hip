@@ -249,22 +249,27 @@ __global__ __launch_bounds__(kT) void k_fire(DevModel m, int64_t Rn, uint64_t se
                                              const uint8_t *__restrict__ crit, const double *__restrict__ tau,
                                              const int32_t *__restrict__ jc, int32_t *__restrict__ k,
                                              const uint8_t *__restrict__ mode, int exact) {
+    // a thread takes channels 4q..4q+3 of one realization: their first draws
+    // share one Philox block (head_block), computed once
     const int64_t rt = (Rn + kT - 1) / kT;
-    const int32_t j = (int32_t)(blockIdx.x / rt);
+    const int32_t q = (int32_t)(blockIdx.x / rt);
     const int64_t r = (int64_t)(blockIdx.x % rt) * kT + threadIdx.x;
     if (r >= Rn) return;
     if (mode && mode[r] != 1) return;
-    const int64_t o = (int64_t)j * Rn + r;
-    int32_t kj;
-    if (crit[o]) {
-        kj = (jc[r] == j) ? 1 : 0;
-    } else {
-        const uint2 key = rng_key(seed, rbase + r);
-        const uint32_t head = quad_pick(head_block(key, (uint32_t)j >> 2, epoch[r], model_id << 2), j & 3);
-        kj = poisson_draw<R>(a[o] * (R)tau[r], head, key, (uint32_t)j, epoch[r], (model_id << 2) | kKindPoisson,
-                             exact != 0);
+    const uint2 key = rng_key(seed, rbase + r);
+    const uint32_t ep = epoch[r];
+    const int32_t jcr = jc[r];
+    const R tr = (R)tau[r];
+    const uint4 h = head_block(key, (uint32_t)q, ep, model_id << 2);
+    for (int i = 0; i < 4; ++i) {
+        const int32_t j = 4 * q + i;
+        if (j >= m.M) break;
+        const int64_t o = (int64_t)j * Rn + r;
+        const int32_t kj = crit[o] ? ((jcr == j) ? 1 : 0)
+                                   : poisson_draw<R>(a[o] * tr, quad_pick(h, i), key, (uint32_t)j, ep,
+                                                     (model_id << 2) | kKindPoisson, exact != 0);
+        k[o] = kj;
     }
-    k[o] = kj;
 }
 
 // --------------------------------------------------------------- update --
@@ -590,7 +595,7 @@ void launch_select(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint3
 void launch_fire(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase,
                  const uint32_t *epoch, const void *a, const uint8_t *crit, const double *tau, const int32_t *jc,
                  int32_t *k, const uint8_t *mode, bool exact, hipStream_t s) {
-    const int64_t nb = (int64_t)blocks_for(R) * m.M;
+    const int64_t nb = (int64_t)blocks_for(R) * ((m.M + 3) / 4);
     if (is64)
         k_fire<double><<<(unsigned)nb, kT, 0, s>>>(m, R, seed, model_id, rbase, epoch, (const double *)a, crit, tau, jc,
                                                    k, mode, exact ? 1 : 0);
