@@ -350,8 +350,12 @@ tl_status run_plain(const tl_model *m, const tl_sim_params *p, tl_result *res) {
     const size_t rs = is64 ? 8 : 4;
     const std::vector<int32_t> trk = track_list(m, p);
     const int32_t T = (int32_t)trk.size();
-    // channel chunks so that the propensity pass has >= ~256k threads
-    int32_t nchunk = (int32_t)std::min<int64_t>(std::max<int64_t>(1, (262144 + R - 1) / R), std::max(1, m->M / 64));
+    // channel chunks so that the propensity pass has >= ~256k threads, and at
+    // least ~sqrt(M) of them (the critical pick walks chunk sums, then one
+    // chunk's channels) while the partial sums stay within 4M entries
+    const int64_t by_threads = std::max<int64_t>(1, (262144 + R - 1) / R);
+    const int64_t by_walk = std::min<int64_t>((int64_t)std::ceil(std::sqrt((double)m->M)), std::max<int64_t>(1, 4194304 / R));
+    int32_t nchunk = (int32_t)std::min<int64_t>(std::max(by_threads, by_walk), std::max(1, m->M / 64));
     DevBufs B;
     int32_t *x, *xn, *k, *d_track, *d_outx;
     void *a, *tau_i;
