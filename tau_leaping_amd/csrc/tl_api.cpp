@@ -364,13 +364,25 @@ tl_status run_plain(const tl_model *m, const tl_sim_params *p, tl_result *res) {
     TL_TRY(B.alloc(x, (size_t)m->N * R));
     TL_TRY(B.alloc(xn, (size_t)m->N * R));
     TL_TRY(B.alloc(k, (size_t)m->M * R));
-    unsigned char *araw, *tauraw;
+    // species-row chunks for the tau minimum: with >= ~1M threads over at
+    // least 64 rows each, the species sums and each chunk's minimum are one
+    // pass and no per-species bound is stored; otherwise the bounds are
+    // stored and combined over row chunks of >= ~256k threads
+    const int32_t row_cap = std::max(1, m->N / 64);
+    const int32_t nrc_fused = (int32_t)std::min<int64_t>((1048576 + R - 1) / R, row_cap);
+    const bool fused = nrc_fused > 1 && (int64_t)nrc_fused * R >= 262144;
+    const int32_t nrc =
+        fused ? nrc_fused : (int32_t)std::min<int64_t>(std::max<int64_t>(1, (262144 + R - 1) / R), row_cap);
+    unsigned char *araw, *tauraw = nullptr;
     TL_TRY(B.alloc(araw, (size_t)m->M * R * rs));
-    TL_TRY(B.alloc(tauraw, (size_t)m->N * R * rs));
     a = araw;
+    which_i = nullptr;
+    if (!fused) {
+        TL_TRY(B.alloc(tauraw, (size_t)m->N * R * rs));
+        TL_TRY(B.alloc(which_i, (size_t)m->N * R));
+    }
     tau_i = tauraw;
     TL_TRY(B.alloc(crit, (size_t)m->M * R));
-    TL_TRY(B.alloc(which_i, (size_t)m->N * R));
     TL_TRY(B.alloc(a0, R));
     TL_TRY(B.alloc(a0c, R));
     TL_TRY(B.alloc(tau1, R));
@@ -378,13 +390,13 @@ tl_status run_plain(const tl_model *m, const tl_sim_params *p, tl_result *res) {
         TL_TRY(B.alloc(part, (size_t)nchunk * R));
         TL_TRY(B.alloc(partc, (size_t)nchunk * R));
     }
-    // species-row chunks for the tau minimum, likewise
-    const int32_t nrc = (int32_t)std::min<int64_t>(std::max<int64_t>(1, (262144 + R - 1) / R), std::max(1, m->N / 64));
     unsigned char *pbest = nullptr;
     int32_t *pidx = nullptr;
+    uint8_t *pwhich = nullptr;
     if (nrc > 1) {
         TL_TRY(B.alloc(pbest, (size_t)nrc * R * rs));
         TL_TRY(B.alloc(pidx, (size_t)nrc * R));
+        if (fused) TL_TRY(B.alloc(pwhich, (size_t)nrc * R));
     }
     TL_TRY(B.alloc(d_track, T));
     TL_TRY(B.alloc(d_outx, (size_t)T * R));
@@ -421,7 +433,7 @@ tl_status run_plain(const tl_model *m, const tl_sim_params *p, tl_result *res) {
     for (int64_t it = 0;; ++it) {
         launch_prop(is64, dm, R, nchunk, x, a, crit, a0, a0c, part, partc, st.active, s);
         launch_tau(is64, dm, R, p->eps, x, a, crit, tau_i, which_i, tau1, nullptr, nullptr, st.active, s, nrc, pbest,
-                   pidx);
+                   pidx, pwhich);
         launch_plain_select(is64, dm, R, seed, p->model_id, p->realization_offset, p->t_final, p->max_iter, x, a, crit,
                             a0, a0c, tau1, st, nchunk, partc, s);
         launch_fire(is64, dm, R, seed, p->model_id, p->realization_offset, st.epoch, a, crit, st.tau, st.jc, k, st.mode,

@@ -56,11 +56,13 @@ void launch_prop(bool is64, const DevModel &m, int64_t R, int32_t nchunk, const 
                  uint8_t *crit, double *a0, double *a0c, double *part, double *partc, const uint8_t *active,
                  hipStream_t s);
 // nrc > 1 with scratch pbest [nrc][R] (propensity type) and pidx [nrc][R]:
-// the species minimum in two passes over nrc row chunks.
+// the species minimum in two passes over nrc row chunks.  With pwhich
+// [nrc][R] as well, the species sums and each chunk's minimum are one pass
+// and tau_i / which_i are not used (may be null).
 void launch_tau(bool is64, const DevModel &m, int64_t R, double eps, const int32_t *x, const void *a,
                 const uint8_t *crit, void *tau_i, uint8_t *which_i, double *tau1, int32_t *argmin,
                 int32_t *which, const uint8_t *active, hipStream_t s, int32_t nrc = 1, void *pbest = nullptr,
-                int32_t *pidx = nullptr);
+                int32_t *pidx = nullptr, uint8_t *pwhich = nullptr);
 void launch_select(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase,
                    const uint32_t *epoch, const void *a, const uint8_t *crit, const double *a0c,
                    const double *tau1, const double *t_rem, double *tau, int32_t *jc, uint8_t *clamped,

@@ -109,6 +109,53 @@ __global__ __launch_bounds__(kT) void k_tau_rows(DevModel m, int64_t Rn, R eps, 
     }
 }
 
+// The species sums and the minimum over one contiguous chunk of rows in one
+// pass (blockIdx.y = chunk), for many species: each thread keeps its chunk's
+// first minimum and the bound it came from; k_tau_min_fin combines the chunks
+// in order.  Nothing per species is written.
+template <class R>
+__global__ __launch_bounds__(kT) void k_tau_rows_part(DevModel m, int64_t Rn, R eps, int32_t nrc,
+                                                      const int32_t *__restrict__ x, const R *__restrict__ a,
+                                                      const uint8_t *__restrict__ crit, R *__restrict__ pbest,
+                                                      int32_t *__restrict__ pidx, uint8_t *__restrict__ pwhich,
+                                                      const uint8_t *__restrict__ active) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= Rn) return;
+    if (active && !active[r]) return;
+    const int32_t c = blockIdx.y;
+    const int32_t ch = (m.N + nrc - 1) / nrc;
+    const int32_t i0 = c * ch, i1 = min(m.N, i0 + ch);
+    R best = (R)kFltMax;
+    int32_t bi = -1;
+    int bw = 0;
+    for (int32_t i = i0; i < i1; ++i) {
+        double mu = 0, sig = 0;
+        const int32_t e1 = m.row_ptr[i + 1];
+        for (int32_t e = m.row_ptr[i]; e < e1; ++e) {
+            const int32_t p = m.csr[e];
+            const int32_t j = entry_index(p);
+            const double v = (double)entry_value(p);
+            const double tmp = v * (double)a[(int64_t)j * Rn + r] * (double)(!crit[(int64_t)j * Rn + r]);
+            sig += v * tmp;
+            mu += tmp;
+        }
+        R ti;
+        int w = 0;
+        if (i < m.M)
+            ti = tau_species<R>(eps, x[(int64_t)i * Rn + r], m.g_order[i], (R)mu, (R)sig, w);
+        else
+            ti = (R)INFINITY;
+        if (ti < best) {
+            best = ti;
+            bi = i;
+            bw = w;
+        }
+    }
+    pbest[(int64_t)c * Rn + r] = best;
+    pidx[(int64_t)c * Rn + r] = bi;
+    pwhich[(int64_t)c * Rn + r] = (uint8_t)bw;
+}
+
 template <class R>
 __global__ __launch_bounds__(kT) void k_tau_min(int32_t N, int64_t Rn, const R *__restrict__ tau_i,
                                                 const uint8_t *__restrict__ which_i, double *__restrict__ tau1,
@@ -162,24 +209,29 @@ __global__ __launch_bounds__(kT) void k_tau_min_part(int32_t N, int64_t Rn, int3
 template <class R>
 __global__ __launch_bounds__(kT) void k_tau_min_fin(int64_t Rn, int32_t nrc, const R *__restrict__ pbest,
                                                     const int32_t *__restrict__ pidx,
-                                                    const uint8_t *__restrict__ which_i, double *__restrict__ tau1,
+                                                    const uint8_t *__restrict__ which_i,
+                                                    const uint8_t *__restrict__ pwhich, double *__restrict__ tau1,
                                                     int32_t *__restrict__ argmin, int32_t *__restrict__ which,
                                                     const uint8_t *__restrict__ active) {
+    // which bound won: which_i[winner] after k_tau_min_part, the chunk's
+    // pwhich after k_tau_rows_part
     const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
     if (r >= Rn) return;
     if (active && !active[r]) return;
     R best = (R)kFltMax;
-    int32_t bi = -1;
+    int32_t bi = -1, bc = -1;
     for (int32_t c = 0; c < nrc; ++c) {
         const R v = pbest[(int64_t)c * Rn + r];
         if (v < best) {
             best = v;
             bi = pidx[(int64_t)c * Rn + r];
+            bc = c;
         }
     }
     tau1[r] = (double)best;
     if (argmin) argmin[r] = bi;
-    if (which) which[r] = bi >= 0 ? which_i[(int64_t)bi * Rn + r] : 0;
+    if (which)
+        which[r] = bi < 0 ? 0 : pwhich ? pwhich[(int64_t)bc * Rn + r] : which_i[(int64_t)bi * Rn + r];
 }
 
 // --------------------------------------------------------------- select --
@@ -596,34 +648,41 @@ void launch_prop(bool is64, const DevModel &m, int64_t R, int32_t nchunk, const 
     if (nchunk > 1) k_sum_parts<<<blocks_for(R), kT, 0, s>>>(R, nchunk, part, partc, a0, a0c, active);
 }
 
+namespace {
+template <class R>
+void tau_stage(const DevModel &m, int64_t Rn, double eps, const int32_t *x, const void *a, const uint8_t *crit,
+               void *tau_i, uint8_t *which_i, double *tau1, int32_t *argmin, int32_t *which, const uint8_t *active,
+               hipStream_t s, int32_t nrc, void *pbest, int32_t *pidx, uint8_t *pwhich) {
+    const R *ar = (const R *)a;
+    if (nrc > 1 && pbest && pidx && pwhich) {
+        k_tau_rows_part<R><<<dim3(blocks_for(Rn), nrc), kT, 0, s>>>(m, Rn, (R)eps, nrc, x, ar, crit, (R *)pbest, pidx,
+                                                                    pwhich, active);
+        k_tau_min_fin<R><<<blocks_for(Rn), kT, 0, s>>>(Rn, nrc, (const R *)pbest, pidx, nullptr, pwhich, tau1, argmin,
+                                                       which, active);
+        return;
+    }
+    k_tau_rows<R><<<dim3(blocks_for(Rn), row_grid(m.N)), kT, 0, s>>>(m, Rn, (R)eps, x, ar, crit, (R *)tau_i, which_i,
+                                                                     active);
+    if (nrc > 1 && pbest && pidx) {
+        k_tau_min_part<R><<<dim3(blocks_for(Rn), nrc), kT, 0, s>>>(m.N, Rn, nrc, (const R *)tau_i, (R *)pbest, pidx,
+                                                                   active);
+        k_tau_min_fin<R><<<blocks_for(Rn), kT, 0, s>>>(Rn, nrc, (const R *)pbest, pidx, which_i, nullptr, tau1, argmin,
+                                                       which, active);
+    } else {
+        k_tau_min<R><<<blocks_for(Rn), kT, 0, s>>>(m.N, Rn, (const R *)tau_i, which_i, tau1, argmin, which, active);
+    }
+}
+}  // namespace
+
 void launch_tau(bool is64, const DevModel &m, int64_t R, double eps, const int32_t *x, const void *a,
                 const uint8_t *crit, void *tau_i, uint8_t *which_i, double *tau1, int32_t *argmin, int32_t *which,
-                const uint8_t *active, hipStream_t s, int32_t nrc, void *pbest, int32_t *pidx) {
-    dim3 g(blocks_for(R), row_grid(m.N));
-    const bool two = nrc > 1 && pbest && pidx;
-    if (is64) {
-        k_tau_rows<double><<<g, kT, 0, s>>>(m, R, eps, x, (const double *)a, crit, (double *)tau_i, which_i, active);
-        if (two) {
-            k_tau_min_part<double><<<dim3(blocks_for(R), nrc), kT, 0, s>>>(m.N, R, nrc, (const double *)tau_i,
-                                                                          (double *)pbest, pidx, active);
-            k_tau_min_fin<double><<<blocks_for(R), kT, 0, s>>>(R, nrc, (const double *)pbest, pidx, which_i, tau1,
-                                                               argmin, which, active);
-        } else {
-            k_tau_min<double><<<blocks_for(R), kT, 0, s>>>(m.N, R, (const double *)tau_i, which_i, tau1, argmin, which,
-                                                           active);
-        }
-    } else {
-        k_tau_rows<float><<<g, kT, 0, s>>>(m, R, (float)eps, x, (const float *)a, crit, (float *)tau_i, which_i, active);
-        if (two) {
-            k_tau_min_part<float><<<dim3(blocks_for(R), nrc), kT, 0, s>>>(m.N, R, nrc, (const float *)tau_i,
-                                                                         (float *)pbest, pidx, active);
-            k_tau_min_fin<float><<<blocks_for(R), kT, 0, s>>>(R, nrc, (const float *)pbest, pidx, which_i, tau1,
-                                                              argmin, which, active);
-        } else {
-            k_tau_min<float><<<blocks_for(R), kT, 0, s>>>(m.N, R, (const float *)tau_i, which_i, tau1, argmin, which,
-                                                          active);
-        }
-    }
+                const uint8_t *active, hipStream_t s, int32_t nrc, void *pbest, int32_t *pidx, uint8_t *pwhich) {
+    if (is64)
+        tau_stage<double>(m, R, eps, x, a, crit, tau_i, which_i, tau1, argmin, which, active, s, nrc, pbest, pidx,
+                          pwhich);
+    else
+        tau_stage<float>(m, R, eps, x, a, crit, tau_i, which_i, tau1, argmin, which, active, s, nrc, pbest, pidx,
+                         pwhich);
 }
 
 void launch_select(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase,

@@ -149,6 +149,27 @@ def test_model_larger_than_lds(cuda, prec):
         assert np.array_equal(r.steps[same], ora["steps"][same])
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_plain_form_one_pass_species_minimum(cuda, prec):
+    # 6000 species, 4096 realizations: the plain form sums the species rows
+    # and takes each row chunk's tau minimum in one pass (93 chunks of 65
+    # rows, no per-species bound stored); its realizations follow the
+    # persistent kernel's and the oracle's
+    net = models.toy_network(6000, 12000, seed=41)
+    m, om = pair(net)
+    R, tf = 4096, 1e-4
+    track = np.arange(0, 6000, 150)
+    plain = m.simulate(tf, R, seed=5, track=track, precision=prec, mode="plain")
+    pers = m.simulate(tf, R, seed=5, track=track, precision=prec, mode="persistent")
+    assert plain.n_failed == 0 and pers.n_failed == 0
+    same = _compare(plain.final_x, pers.final_x)
+    assert np.array_equal(plain.steps[same], pers.steps[same])
+    ora = om.simulate(64, tf, seed=5, precision=prec)
+    assert (ora["status"] == 0).all()
+    same = _compare(plain.final_x[:64], ora["x_final"][:, track], min_same=0.95)
+    assert np.array_equal(plain.steps[:64][same], ora["steps"][same])
+
+
 def test_model_with_many_tiles_stores_a_in_hbm(cuda):
     # 6000 species x 300,000 channels: 19 propensity tiles, so the HBM-state
     # kernel keeps a[M] in global memory and sums the species rows in one
