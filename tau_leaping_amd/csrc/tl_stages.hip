@@ -383,6 +383,42 @@ __global__ __launch_bounds__(kT) void k_update(DevModel m, int64_t Rn, const int
     }
 }
 
+// The same for two adjacent realizations per thread (even R): 8-byte loads
+// and stores, 512 bytes per wavefront access.  A pair with one realization
+// not leaping writes that one's x_out too (never read: the commit copies
+// only leaping realizations) but flags only leaping ones.
+__global__ __launch_bounds__(kT) void k_update2(DevModel m, int64_t Rn, const int32_t *__restrict__ x,
+                                                const int32_t *__restrict__ k, int32_t *__restrict__ x_out,
+                                                uint8_t *__restrict__ neg, const uint8_t *__restrict__ mode) {
+    const int64_t r = 2 * ((int64_t)blockIdx.x * kT + threadIdx.x);
+    if (r >= Rn) return;
+    bool on0 = true, on1 = true;
+    if (mode) {
+        on0 = mode[r] == 1;
+        on1 = mode[r + 1] == 1;
+        if (!on0 && !on1) return;
+    }
+    bool n0 = false, n1 = false;
+    for (int32_t i = blockIdx.y; i < m.N; i += gridDim.y) {
+        int32_t s0 = 0, s1 = 0;
+        const int32_t e1 = m.row_ptr[i + 1];
+        for (int32_t e = m.row_ptr[i]; e < e1; ++e) {
+            const int32_t p = m.csr[e];
+            const int32_t v = entry_value(p);
+            const int2 kk = *reinterpret_cast<const int2 *>(k + (int64_t)entry_index(p) * Rn + r);
+            s0 += v * kk.x;
+            s1 += v * kk.y;
+        }
+        const int2 xx = *reinterpret_cast<const int2 *>(x + (int64_t)i * Rn + r);
+        const int2 xo = make_int2(xx.x + s0, xx.y + s1);
+        *reinterpret_cast<int2 *>(x_out + (int64_t)i * Rn + r) = xo;
+        n0 |= xo.x < 0;
+        n1 |= xo.y < 0;
+    }
+    if (n0 && on0) neg[r] = 1;
+    if (n1 && on1) neg[r + 1] = 1;
+}
+
 // ------------------------------------------------------------ direct SSA --
 template <class R>
 __device__ double prop_soa(const DevModel &m, const int32_t *x, R *a, uint8_t *crit, int64_t Rn, int64_t r,
@@ -710,8 +746,12 @@ void launch_fire(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_
 
 void launch_update(const DevModel &m, int64_t R, const int32_t *x, const int32_t *k, int32_t *x_out, uint8_t *neg,
                    const uint8_t *mode, hipStream_t s) {
-    dim3 g(blocks_for(R), row_grid(m.N));
-    k_update<<<g, kT, 0, s>>>(m, R, x, k, x_out, neg, mode);
+    // pairs need 8-byte aligned rows: R even (the buffers are 256-byte aligned)
+    const bool pairs = R % 2 == 0 && ((uintptr_t)x | (uintptr_t)k | (uintptr_t)x_out) % 8 == 0;
+    if (pairs)
+        k_update2<<<dim3(blocks_for(R / 2), row_grid(m.N)), kT, 0, s>>>(m, R, x, k, x_out, neg, mode);
+    else
+        k_update<<<dim3(blocks_for(R), row_grid(m.N)), kT, 0, s>>>(m, R, x, k, x_out, neg, mode);
 }
 
 void launch_ssa(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase, int32_t *x,

@@ -83,6 +83,22 @@ def test_select_fire_update(cuda, name, prec):
     assert np.array_equal(_n(neg), neg_o)
 
 
+@pytest.mark.parametrize("R", [999, 1000])
+def test_update_odd_and_even_realizations(cuda, R):
+    # even R runs two realizations per thread, odd R one; both exact, with
+    # some updates driving a species negative
+    net = SMALL_NETS["toy"]()
+    m, om = pair(net)
+    rng = np.random.default_rng(R)
+    x = random_states(net.n_species, R, seed=13)
+    k = rng.poisson(3.0, size=(m.n_channels, R)).astype(np.int32)
+    xo_o, neg_o = om.update(x, k)
+    assert neg_o.any() and not neg_o.all()
+    xo, neg = T.stage_update(m, _t(x, cuda), _t(k, cuda))
+    assert np.array_equal(_n(xo), xo_o)
+    assert np.array_equal(_n(neg), neg_o)
+
+
 @pytest.mark.parametrize("name", ["dimer", "toy", "colloid"])
 @pytest.mark.parametrize("prec", PRECS)
 def test_direct_events(cuda, name, prec):
