@@ -156,6 +156,90 @@ __global__ __launch_bounds__(kT) void k_tau_rows_part(DevModel m, int64_t Rn, R 
     pwhich[(int64_t)c * Rn + r] = (uint8_t)bw;
 }
 
+// The same for two adjacent realizations per thread (even R): 2-wide loads
+// of propensities, flags and populations; each realization's arithmetic is
+// the single kernel's.
+template <class R>
+struct Pair;
+template <>
+struct Pair<double> {
+    using T = double2;
+};
+template <>
+struct Pair<float> {
+    using T = float2;
+};
+
+template <class R>
+__global__ __launch_bounds__(kT) void k_tau_rows_part2(DevModel m, int64_t Rn, R eps, int32_t nrc,
+                                                       const int32_t *__restrict__ x, const R *__restrict__ a,
+                                                       const uint8_t *__restrict__ crit, R *__restrict__ pbest,
+                                                       int32_t *__restrict__ pidx, uint8_t *__restrict__ pwhich,
+                                                       const uint8_t *__restrict__ active) {
+    using V2 = typename Pair<R>::T;
+    const int64_t r = 2 * ((int64_t)blockIdx.x * kT + threadIdx.x);
+    if (r >= Rn) return;
+    bool on0 = true, on1 = true;
+    if (active) {
+        on0 = active[r] != 0;
+        on1 = active[r + 1] != 0;
+        if (!on0 && !on1) return;
+    }
+    const int32_t c = blockIdx.y;
+    const int32_t ch = (m.N + nrc - 1) / nrc;
+    const int32_t i0 = c * ch, i1 = min(m.N, i0 + ch);
+    R best0 = (R)kFltMax, best1 = (R)kFltMax;
+    int32_t bi0 = -1, bi1 = -1;
+    int bw0 = 0, bw1 = 0;
+    for (int32_t i = i0; i < i1; ++i) {
+        double mu0 = 0, sig0 = 0, mu1 = 0, sig1 = 0;
+        const int32_t e1 = m.row_ptr[i + 1];
+        for (int32_t e = m.row_ptr[i]; e < e1; ++e) {
+            const int32_t p = m.csr[e];
+            const int64_t o = (int64_t)entry_index(p) * Rn + r;
+            const double v = (double)entry_value(p);
+            const V2 av = *reinterpret_cast<const V2 *>(a + o);
+            const uchar2 cv = *reinterpret_cast<const uchar2 *>(crit + o);
+            const double t0 = v * (double)av.x * (double)(!cv.x);
+            const double t1 = v * (double)av.y * (double)(!cv.y);
+            sig0 += v * t0;
+            mu0 += t0;
+            sig1 += v * t1;
+            mu1 += t1;
+        }
+        R ti0, ti1;
+        int w0 = 0, w1 = 0;
+        if (i < m.M) {
+            const int2 xx = *reinterpret_cast<const int2 *>(x + (int64_t)i * Rn + r);
+            ti0 = tau_species<R>(eps, xx.x, m.g_order[i], (R)mu0, (R)sig0, w0);
+            ti1 = tau_species<R>(eps, xx.y, m.g_order[i], (R)mu1, (R)sig1, w1);
+        } else {
+            ti0 = ti1 = (R)INFINITY;
+        }
+        if (ti0 < best0) {
+            best0 = ti0;
+            bi0 = i;
+            bw0 = w0;
+        }
+        if (ti1 < best1) {
+            best1 = ti1;
+            bi1 = i;
+            bw1 = w1;
+        }
+    }
+    const int64_t o = (int64_t)c * Rn + r;
+    if (on0) {
+        pbest[o] = best0;
+        pidx[o] = bi0;
+        pwhich[o] = (uint8_t)bw0;
+    }
+    if (on1) {
+        pbest[o + 1] = best1;
+        pidx[o + 1] = bi1;
+        pwhich[o + 1] = (uint8_t)bw1;
+    }
+}
+
 template <class R>
 __global__ __launch_bounds__(kT) void k_tau_min(int32_t N, int64_t Rn, const R *__restrict__ tau_i,
                                                 const uint8_t *__restrict__ which_i, double *__restrict__ tau1,
@@ -691,8 +775,15 @@ void tau_stage(const DevModel &m, int64_t Rn, double eps, const int32_t *x, cons
                hipStream_t s, int32_t nrc, void *pbest, int32_t *pidx, uint8_t *pwhich) {
     const R *ar = (const R *)a;
     if (nrc > 1 && pbest && pidx && pwhich) {
-        k_tau_rows_part<R><<<dim3(blocks_for(Rn), nrc), kT, 0, s>>>(m, Rn, (R)eps, nrc, x, ar, crit, (R *)pbest, pidx,
-                                                                    pwhich, active);
+        // realization pairs need R even and the buffers 2-element aligned
+        const bool pairs = Rn % 2 == 0 && (uintptr_t)x % 8 == 0 && (uintptr_t)a % (2 * sizeof(R)) == 0 &&
+                           (uintptr_t)crit % 2 == 0;
+        if (pairs)
+            k_tau_rows_part2<R><<<dim3(blocks_for(Rn / 2), nrc), kT, 0, s>>>(m, Rn, (R)eps, nrc, x, ar, crit,
+                                                                             (R *)pbest, pidx, pwhich, active);
+        else
+            k_tau_rows_part<R><<<dim3(blocks_for(Rn), nrc), kT, 0, s>>>(m, Rn, (R)eps, nrc, x, ar, crit, (R *)pbest,
+                                                                        pidx, pwhich, active);
         k_tau_min_fin<R><<<blocks_for(Rn), kT, 0, s>>>(Rn, nrc, (const R *)pbest, pidx, nullptr, pwhich, tau1, argmin,
                                                        which, active);
         return;

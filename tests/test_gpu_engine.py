@@ -149,15 +149,16 @@ def test_model_larger_than_lds(cuda, prec):
         assert np.array_equal(r.steps[same], ora["steps"][same])
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_plain_form_one_pass_species_minimum(cuda, prec):
-    # 6000 species, 4096 realizations: the plain form sums the species rows
+@pytest.mark.parametrize("prec,R", [("fp64", 4096), ("fp32", 4096), ("fp64", 4095)])
+def test_plain_form_one_pass_species_minimum(cuda, prec, R):
+    # 6000 species, ~4096 realizations: the plain form sums the species rows
     # and takes each row chunk's tau minimum in one pass (93 chunks of 65
-    # rows, no per-species bound stored); its realizations follow the
-    # persistent kernel's and the oracle's
+    # rows, no per-species bound stored; two realizations per thread when R
+    # is even, one when odd); its realizations follow the persistent kernel's
+    # and the oracle's
     net = models.toy_network(6000, 12000, seed=41)
     m, om = pair(net)
-    R, tf = 4096, 1e-4
+    tf = 1e-4
     track = np.arange(0, 6000, 150)
     plain = m.simulate(tf, R, seed=5, track=track, precision=prec, mode="plain")
     pers = m.simulate(tf, R, seed=5, track=track, precision=prec, mode="persistent")
