@@ -20,6 +20,18 @@ constexpr int kT = 256;  // threads per block for the realization axis
 
 inline unsigned blocks_for(int64_t n, int t = kT) { return (unsigned)((n + t - 1) / t); }
 
+// Two adjacent realizations per thread (even R) load and store 2-wide.
+template <class R>
+struct Pair;
+template <>
+struct Pair<double> {
+    using T = double2;
+};
+template <>
+struct Pair<float> {
+    using T = float2;
+};
+
 // ---------------------------------------------------------- propensities --
 // a_j of channel j for realization r from the structure-of-arrays populations
 // (extended formulas when the model has three-reactant channels).
@@ -59,6 +71,78 @@ __global__ __launch_bounds__(kT) void k_prop(DevModel m, int64_t Rn, int32_t nch
     } else {
         part[(int64_t)c * Rn + r] = s;
         partc[(int64_t)c * Rn + r] = sc;
+    }
+}
+
+// The same for two adjacent realizations per thread (even R): 2-wide loads
+// of populations and stores of propensities and flags; each realization's
+// arithmetic is the single kernel's.
+__device__ __forceinline__ int2 ld2(const int32_t *p) { return *reinterpret_cast<const int2 *>(p); }
+
+template <class R>
+__global__ __launch_bounds__(kT) void k_prop2(DevModel m, int64_t Rn, int32_t nchunk, const int32_t *__restrict__ x,
+                                              R *__restrict__ a, uint8_t *__restrict__ crit, double *__restrict__ a0,
+                                              double *__restrict__ a0c, double *__restrict__ part,
+                                              double *__restrict__ partc, const uint8_t *__restrict__ active) {
+    using V2 = typename Pair<R>::T;
+    const int64_t r = 2 * ((int64_t)blockIdx.x * kT + threadIdx.x);
+    if (r >= Rn) return;
+    bool on0 = true, on1 = true;
+    if (active) {
+        on0 = active[r] != 0;
+        on1 = active[r + 1] != 0;
+        if (!on0 && !on1) return;
+    }
+    const int32_t c = blockIdx.y;
+    const int32_t ch = (m.M + nchunk - 1) / nchunk;
+    const int32_t j0 = c * ch, j1 = min(m.M, j0 + ch);
+    const R *rc = Real<R>::rates(m);
+    double s0 = 0.0, sc0 = 0.0, s1 = 0.0, sc1 = 0.0;
+    for (int32_t j = j0; j < j1; ++j) {
+        const int order = m.order[j];
+        const R cj = rc[j];
+        const int2 xa = ld2(x + (int64_t)m.s0[j] * Rn + r), xb = ld2(x + (int64_t)m.s1[j] * Rn + r);
+        bool c0, c1;
+        R p0, p1;
+        if (m.s2) {
+            const int2 xc = order == 4 ? ld2(x + (int64_t)m.s2[j] * Rn + r) : make_int2(1, 1);
+            p0 = propensity_ext<R>(order, cj, xa.x, xb.x, xc.x, c0);
+            p1 = propensity_ext<R>(order, cj, xa.y, xb.y, xc.y, c1);
+        } else {
+            p0 = propensity<R>(order, cj, xa.x, xb.x, c0);
+            p1 = propensity<R>(order, cj, xa.y, xb.y, c1);
+        }
+        const int64_t o = (int64_t)j * Rn + r;
+        if (on0 && on1) {
+            V2 av;
+            av.x = p0;
+            av.y = p1;
+            *reinterpret_cast<V2 *>(a + o) = av;
+            *reinterpret_cast<uchar2 *>(crit + o) = make_uchar2(c0, c1);
+        } else {
+            if (on0) {
+                a[o] = p0;
+                crit[o] = c0;
+            }
+            if (on1) {
+                a[o + 1] = p1;
+                crit[o + 1] = c1;
+            }
+        }
+        s0 += (double)p0;
+        if (c0) sc0 += (double)p0;
+        s1 += (double)p1;
+        if (c1) sc1 += (double)p1;
+    }
+    double *ps = nchunk == 1 ? a0 : part + (int64_t)c * Rn;
+    double *pc = nchunk == 1 ? a0c : partc + (int64_t)c * Rn;
+    if (on0) {
+        ps[r] = s0;
+        pc[r] = sc0;
+    }
+    if (on1) {
+        ps[r + 1] = s1;
+        pc[r + 1] = sc1;
     }
 }
 
@@ -159,17 +243,6 @@ __global__ __launch_bounds__(kT) void k_tau_rows_part(DevModel m, int64_t Rn, R 
 // The same for two adjacent realizations per thread (even R): 2-wide loads
 // of propensities, flags and populations; each realization's arithmetic is
 // the single kernel's.
-template <class R>
-struct Pair;
-template <>
-struct Pair<double> {
-    using T = double2;
-};
-template <>
-struct Pair<float> {
-    using T = float2;
-};
-
 template <class R>
 __global__ __launch_bounds__(kT) void k_tau_rows_part2(DevModel m, int64_t Rn, R eps, int32_t nrc,
                                                        const int32_t *__restrict__ x, const R *__restrict__ a,
@@ -760,8 +833,15 @@ unsigned row_grid(int32_t n) { return (unsigned)(n < 65535 ? n : 65535); }
 // ----------------------------------------------------------- launchers ----
 void launch_prop(bool is64, const DevModel &m, int64_t R, int32_t nchunk, const int32_t *x, void *a, uint8_t *crit,
                  double *a0, double *a0c, double *part, double *partc, const uint8_t *active, hipStream_t s) {
-    dim3 g(blocks_for(R), (unsigned)nchunk);
-    if (is64)
+    // realization pairs need R even and the buffers 2-element aligned
+    const size_t rs = is64 ? 8 : 4;
+    const bool pairs = R % 2 == 0 && (uintptr_t)x % 8 == 0 && (uintptr_t)a % (2 * rs) == 0 && (uintptr_t)crit % 2 == 0;
+    const dim3 g(blocks_for(pairs ? R / 2 : R), (unsigned)nchunk);
+    if (pairs && is64)
+        k_prop2<double><<<g, kT, 0, s>>>(m, R, nchunk, x, (double *)a, crit, a0, a0c, part, partc, active);
+    else if (pairs)
+        k_prop2<float><<<g, kT, 0, s>>>(m, R, nchunk, x, (float *)a, crit, a0, a0c, part, partc, active);
+    else if (is64)
         k_prop<double><<<g, kT, 0, s>>>(m, R, nchunk, x, (double *)a, crit, a0, a0c, part, partc, active);
     else
         k_prop<float><<<g, kT, 0, s>>>(m, R, nchunk, x, (float *)a, crit, a0, a0c, part, partc, active);

@@ -30,10 +30,10 @@ def _n(t):
 
 @pytest.mark.parametrize("name", list(SMALL_NETS))
 @pytest.mark.parametrize("prec", PRECS)
-def test_propensities_and_tau(cuda, name, prec):
+@pytest.mark.parametrize("R", [777, 778])  # one / two realizations per thread
+def test_propensities_and_tau(cuda, name, prec, R):
     net = SMALL_NETS[name]()
     m, om = pair(net)
-    R = 777
     x = random_states(net.n_species, R, seed=11)
     a_o, c_o, a0_o, a0c_o = om.propensities(x, prec)
     a, c, a0, a0c = T.stage_propensities(m, _t(x, cuda), prec)
