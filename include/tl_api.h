@@ -71,7 +71,10 @@ enum { TL_POISSON_NORMAL = 0, TL_POISSON_EXACT = 1 };
  * run (PERSISTENT returns TL_ERR_TOO_LARGE) and for a model too large for
  * LDS on which PLAIN ran the first 16 steps of a sample (4 realizations per
  * CU, taken when R is at least twice that) more than 15 % faster: those run
- * PLAIN.  The choice is cached per model version. */
+ * PLAIN.  The choice is cached per model version.  PLAIN holds about
+ * 8 N + (5 + p) M bytes of device memory per realization (p = 8 for FP64, 4
+ * for FP32), plus (p + 1) N when fewer than ~1M threads cover the species
+ * rows in chunks of 64 (then the per-species bounds are stored). */
 enum { TL_MODE_AUTO = 0, TL_MODE_PLAIN = 1, TL_MODE_PERSISTENT = 2 };
 
 typedef struct tl_model tl_model;
