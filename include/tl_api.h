@@ -73,8 +73,9 @@ enum { TL_POISSON_NORMAL = 0, TL_POISSON_EXACT = 1 };
  * CU, taken when R is at least twice that) more than 15 % faster: those run
  * PLAIN.  The choice is cached per model version.  PLAIN holds about
  * 8 N + (5 + p) M bytes of device memory per realization (p = 8 for FP64, 4
- * for FP32), plus (p + 1) N when fewer than ~1M threads cover the species
- * rows in chunks of 64 (then the per-species bounds are stored). */
+ * for FP32), plus (p + 1) N when the species rows, cut into chunks of at
+ * least 64, give fewer than 262,144 (chunk, realization) pairs: then the
+ * per-species bounds are stored. */
 enum { TL_MODE_AUTO = 0, TL_MODE_PLAIN = 1, TL_MODE_PERSISTENT = 2 };
 
 typedef struct tl_model tl_model;
