@@ -43,7 +43,8 @@ enum {
     TL_OK = 0,
     TL_ERR_IO = 1,                /* file missing or unreadable (StochKitReader.cpp:127-131) */
     TL_ERR_PARSE = 2,             /* malformed XML / missing node / unknown rate id / count mismatch */
-    TL_ERR_UNSUPPORTED_ORDER = 3, /* reactant pattern other than ->, S->, 2S->, Sa+Sb-> (CSCMatrix.cu:225-231) */
+    TL_ERR_UNSUPPORTED_ORDER = 3, /* reactant pattern other than ->, S->, 2S->, Sa+Sb->, Sa+Sb+Sc->, 2Sa+Sb->
+                                     (the reference accepts S->, 2S->, Sa+Sb->: CSCMatrix.cu:225-231) */
     TL_ERR_STOICH_RANGE = 4,      /* stoichiometry outside [-2, 5] (3-bit packing, functions.h:126-130) */
     TL_ERR_ALLOC = 5,
     TL_ERR_HIP = 6,
