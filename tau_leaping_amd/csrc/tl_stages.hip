@@ -477,12 +477,13 @@ __global__ __launch_bounds__(kT) void k_select(DevModel m, int64_t Rn, uint64_t 
 
 // ----------------------------------------------------------------- fire --
 // One block covers kT realizations of channels 4q..4q+3: blockIdx.x = q*rt + tile.
-// Registers for six waves per SIMD (the rare exact and normal-approximation
-// paths spill a little): the pass is bound by its loads, which the extra waves
-// keep in flight (1e6 channels: 8 % faster than four waves, eight waves with
-// more spills slower than both).
+// Registers for five waves per SIMD (96 VGPRs, 12 bytes of spills per lane):
+// the pass is bound by its loads, which the extra waves keep in flight.  Whole
+// 1e6-channel runs: five waves 6.79 s, six (92 bytes of spills) 6.91 s; on an
+// earlier build four waves without spills were 2 % slower than six and eight
+// (172 bytes) 29 % slower.
 template <class R>
-__global__ __launch_bounds__(kT, 6) void k_fire(DevModel m, int64_t Rn, uint64_t seed, uint32_t model_id, int64_t rbase,
+__global__ __launch_bounds__(kT, 5) void k_fire(DevModel m, int64_t Rn, uint64_t seed, uint32_t model_id, int64_t rbase,
                                                 const uint32_t *__restrict__ epoch, const R *__restrict__ a,
                                                 const uint8_t *__restrict__ crit, const double *__restrict__ tau,
                                                 const int32_t *__restrict__ jc, int32_t *__restrict__ k,
