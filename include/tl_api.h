@@ -67,17 +67,21 @@ enum { TL_POISSON_NORMAL = 0, TL_POISSON_EXACT = 1 };
  * structure-of-arrays state in HBM; PERSISTENT = one work-stealing kernel that
  * runs whole trajectories with each realization's state in LDS, or in a slice
  * of HBM per block for models too large for LDS (all models of a batch in one
- * launch); AUTO = PERSISTENT, except, when the call asks for neither
- * sensitivities nor a time grid, for a model no persistent launch shape can
- * run (PERSISTENT returns TL_ERR_TOO_LARGE) and for a model too large for
- * LDS on which PLAIN ran the first 16 steps of a sample (4 realizations per
- * CU, taken when R is at least twice that) more than 15 % faster: those run
- * PLAIN.  The choice is cached per model version.  PLAIN holds about
+ * launch); AUTO = PERSISTENT, except for a model no persistent launch shape
+ * can run (PERSISTENT returns TL_ERR_TOO_LARGE) and for a model too large for
+ * LDS whose persistent shape would keep a[M] in HBM (more than 16 propensity
+ * tiles: about 260,000 channels and up), where PLAIN's coalesced
+ * realization-major accesses were measured 4x faster: those run PLAIN.  The
+ * choice is a pure function of the model sizes and the precision, and
+ * tl_result.form reports it.  PLAIN holds about
  * 8 N + (5 + p) M bytes of device memory per realization (p = 8 for FP64, 4
  * for FP32), plus (p + 1) N when the species rows, cut into chunks of at
  * least 64, give fewer than 262,144 (chunk, realization) pairs: then the
  * per-species bounds are stored. */
 enum { TL_MODE_AUTO = 0, TL_MODE_PLAIN = 1, TL_MODE_PERSISTENT = 2 };
+
+/* tl_sim_params.block_size: the timed autotuner instead of the default shape */
+enum { TL_BLOCK_TUNE = -1 };
 
 typedef struct tl_model tl_model;
 
@@ -116,7 +120,10 @@ typedef struct tl_sim_params {
     int32_t sensitivities;      /* accumulate likelihood-ratio scores, fill dmean_dc */
     int32_t device;             /* -d */
     int32_t mode;               /* TL_MODE_* */
-    int32_t block_size;         /* persistent kernel lanes per realization (64..1024); 0: autotune */
+    int32_t block_size;         /* persistent kernel lanes per realization (64..1024); 0: the default
+                                   shape for the model sizes (a pure function of sizes, precision and
+                                   kernel build: a seed gives the same trajectories on every run);
+                                   TL_BLOCK_TUNE: time the shapes on a sample and keep the fastest */
     uint32_t model_id;          /* RNG stream id of the model within a batch */
     int64_t realization_offset; /* first realization id */
     int32_t poisson;            /* TL_POISSON_NORMAL (the reference's approximation above mean 80) or
@@ -147,6 +154,13 @@ typedef struct tl_result {
     double sim_seconds;       /* first launch to results on the host */
     double upload_seconds;    /* one-time constant upload */
     double kernel_ms;         /* device time of the main simulation kernel(s) */
+    /* caller-owned per-realization outputs (NULL = not requested) */
+    int64_t *rejections;      /* [R] rejected leap attempts (negative populations) */
+    int32_t *status;          /* [R] final status of each realization (TL_OK, TL_ERR_ITER_CAP, ...) */
+    double *grid;             /* [R][n_grid][n_track] tracked species of each realization on the time
+                                 grid (n_intervals > 0; the rows mean / stddev reduce) */
+    /* filled by the library */
+    int32_t form;             /* the form that ran: TL_MODE_PLAIN or TL_MODE_PERSISTENT */
 } tl_result;
 
 tl_status tl_simulate(const tl_model *m, const tl_sim_params *p, tl_result *r);
@@ -161,7 +175,10 @@ tl_status tl_batch_simulate(const tl_model *const *models, int32_t n_models,
 
 /* ---- stages on caller-provided DEVICE buffers (structure of arrays) ----
  * x [N][R] int32, a [M][R] (double or float per precision), crit [M][R]
- * uint8, per-realization scalars [R].  Synchronous on the library's stream. */
+ * uint8, per-realization scalars [R].  Synchronous with all device work: a
+ * stage waits for the work already queued on every stream of the device
+ * (the caller's kernels that produce its inputs, or that still use a buffer
+ * it writes), runs on the library's stream and returns when it is done. */
 tl_status tl_stage_propensities(const tl_model *m, int32_t precision, int64_t R, const int32_t *x,
                                 void *a, uint8_t *crit, double *a0, double *a0c);
 tl_status tl_stage_tau(const tl_model *m, int32_t precision, int64_t R, double eps,

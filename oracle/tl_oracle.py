@@ -13,7 +13,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libtl_oracle.so")
+# TLO_ORACLE_LIB names another build of these sources (the tests that plant a
+# mistake in the oracle and check that the pins catch it load such a build)
+_LIB_PATH = os.environ.get("TLO_ORACLE_LIB") or os.path.join(_HERE, "_build", "libtl_oracle.so")
 _lib = None
 
 P = C.c_void_p

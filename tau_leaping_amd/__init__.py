@@ -58,7 +58,9 @@ class _Result(C.Structure):
                 ("steps", C.POINTER(C.c_int64)), ("score", C.POINTER(C.c_double)),
                 ("total_steps", C.c_int64), ("total_ssa_events", C.c_int64), ("total_rejections", C.c_int64),
                 ("n_failed", C.c_int64), ("first_failure", C.c_int32), ("block_size", C.c_int32),
-                ("sim_seconds", C.c_double), ("upload_seconds", C.c_double), ("kernel_ms", C.c_double)]
+                ("sim_seconds", C.c_double), ("upload_seconds", C.c_double), ("kernel_ms", C.c_double),
+                ("rejections", C.POINTER(C.c_int64)), ("status", C.POINTER(C.c_int32)),
+                ("grid", C.POINTER(C.c_double)), ("form", C.c_int32)]
 
 
 _SIGS = {
@@ -164,6 +166,10 @@ class SimResult:
     sim_seconds: float
     upload_seconds: float
     kernel_ms: float
+    rejections: np.ndarray     # [R] rejected leap attempts per realization
+    status: np.ndarray         # [R] final status per realization
+    grid: Optional[np.ndarray]  # [R][n_grid][n_track] per-realization time grid (n_intervals > 0)
+    form: str                  # "plain" or "persistent": the form that ran
 
     @property
     def leaps_per_second(self) -> float:
@@ -297,7 +303,7 @@ class Model:
                               mode=kw.get("mode", "auto"))[0]
 
 
-def _result_buffers(model: Model, p: _Params, trk, want_final=True, want_score=False):
+def _result_buffers(model: Model, p: _Params, trk, want_final=True, want_score=False, want_grid=False):
     T = int(p.n_track) if p.n_track > 0 else model.n_species
     G = p.n_intervals + 1 if p.n_intervals > 0 else 1
     R = int(p.n_realizations)
@@ -306,7 +312,9 @@ def _result_buffers(model: Model, p: _Params, trk, want_final=True, want_score=F
                 dmean_dc=np.zeros((T, model.n_channels)) if p.sensitivities else None,
                 dmean_dc_se=np.zeros((T, model.n_channels)) if p.sensitivities else None,
                 steps=np.zeros(R, np.int64),
-                score=np.zeros((R, model.n_channels)) if (want_score and p.sensitivities) else None)
+                score=np.zeros((R, model.n_channels)) if (want_score and p.sensitivities) else None,
+                rejections=np.zeros(R, np.int64), status=np.zeros(R, np.int32),
+                grid=np.zeros((R, G, T)) if (want_grid and p.n_intervals > 0) else None)
     r = _Result()
 
     def P(a, ct):
@@ -320,8 +328,14 @@ def _result_buffers(model: Model, p: _Params, trk, want_final=True, want_score=F
     r.dmean_dc_se = P(bufs["dmean_dc_se"], C.c_double)
     r.steps = P(bufs["steps"], C.c_int64)
     r.score = P(bufs["score"], C.c_double)
+    r.rejections = P(bufs["rejections"], C.c_int64)
+    r.status = P(bufs["status"], C.c_int32)
+    r.grid = P(bufs["grid"], C.c_double)
     track = trk if trk is not None else np.arange(model.n_species, dtype=np.int32)
     return r, bufs, track
+
+
+_FORMS = {MODE_PLAIN: "plain", MODE_PERSISTENT: "persistent"}
 
 
 def _to_result(r: _Result, bufs, track) -> SimResult:
@@ -330,7 +344,8 @@ def _to_result(r: _Result, bufs, track) -> SimResult:
                      steps=bufs["steps"], score=bufs["score"], total_steps=r.total_steps, total_ssa_events=r.total_ssa_events,
                      total_rejections=r.total_rejections, n_failed=r.n_failed, first_failure=r.first_failure,
                      block_size=r.block_size, sim_seconds=r.sim_seconds, upload_seconds=r.upload_seconds,
-                     kernel_ms=r.kernel_ms)
+                     kernel_ms=r.kernel_ms, rejections=bufs["rejections"], status=bufs["status"], grid=bufs["grid"],
+                     form=_FORMS.get(r.form, "unknown"))
 
 
 def batch_simulate(model_list: Sequence[Model], kwargs_list: Sequence[dict], mode="auto",
@@ -350,10 +365,11 @@ def batch_simulate(model_list: Sequence[Model], kwargs_list: Sequence[dict], mod
         kw = dict(kw)
         kw.pop("mode", None)
         want_score = bool(kw.pop("return_scores", False))
+        want_grid = bool(kw.pop("return_grid", False))
         p, trk = m.params(mode=mode, **kw)
         params[i] = p
         keep.append(trk)
-        r, bufs, track = _result_buffers(m, p, trk, want_final, want_score)
+        r, bufs, track = _result_buffers(m, p, trk, want_final, want_score, want_grid)
         results[i] = r
         outs.append((bufs, track))
     handles = (C.c_void_p * n)(*[m.handle.value for m in model_list])
@@ -457,8 +473,13 @@ def _mean_final_fn():
 
         @staticmethod
         def forward(ctx, rates, model, kw):
+            # simulate at the given rates, then give the model its own back
+            saved = model.rates
             model.rates = rates.detach().to("cpu", torch.float64).numpy()
-            res = model.simulate(sensitivities=True, **kw)
+            try:
+                res = model.simulate(sensitivities=True, **kw)
+            finally:
+                model.rates = saved
             ctx.save_for_backward(torch.from_numpy(res.dmean_dc))
             ctx.rates_dtype = rates.dtype
             ctx.rates_device = rates.device

@@ -264,6 +264,20 @@ struct DevBufs {
         if (st_ != TL_OK) return st_;     \
     } while (0)
 
+// A pair of timing events, destroyed on every return path.
+struct Events {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    tl_status create() {
+        TL_HIP(hipEventCreate(&e0));
+        TL_HIP(hipEventCreate(&e1));
+        return TL_OK;
+    }
+    ~Events() {
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+    }
+};
+
 tl_status check_params(const tl_model *m, const tl_sim_params *p) {
     if (!m || !p) return set_error(TL_ERR_INVALID_ARG, "null model or params");
     if (!(p->t_final > 0.0) || !std::isfinite(p->t_final)) return set_error(TL_ERR_INVALID_ARG, "t_final must be > 0");
@@ -333,6 +347,22 @@ void fill_summary(tl_result *res, const std::vector<int64_t> &steps, const std::
         }
     }
     if (res->steps) std::copy(steps.begin(), steps.end(), res->steps);
+    if (res->rejections) std::copy(rej.begin(), rej.end(), res->rejections);
+    if (res->status) std::copy(status.begin(), status.end(), res->status);
+}
+
+// Moments over realizations of [R][n_grid][T] grid values, per grid point,
+// into res->mean / stddev [n_grid][T] (and the grid times).
+void grid_moments(const std::vector<double> &hg, int64_t R, int32_t n_grid, int32_t T, double t_final,
+                  tl_result *res) {
+    std::vector<double> slice((size_t)R * T);
+    for (int32_t g = 0; g < n_grid; ++g) {
+        for (int64_t r = 0; r < R; ++r)
+            for (int32_t q = 0; q < T; ++q) slice[r * T + q] = hg[((size_t)r * n_grid + g) * T + q];
+        moments(slice.data(), R, T, res->mean ? res->mean + (size_t)g * T : nullptr,
+                res->stddev ? res->stddev + (size_t)g * T : nullptr);
+        if (res->grid_times) res->grid_times[g] = t_final / (n_grid - 1) * g;
+    }
 }
 
 // ------------------------------------------------------------ plain loop --
@@ -343,19 +373,23 @@ tl_status run_plain(const tl_model *m, const tl_sim_params *p, tl_result *res) {
     DevModel dm;
     double up = 0.0;
     TL_TRY(device_model(*m, p->device, dm, &up));
-    if (p->sensitivities) return set_error(TL_ERR_INVALID_ARG, "sensitivities need the persistent form");
-    if (p->n_intervals > 0) return set_error(TL_ERR_INVALID_ARG, "the time grid needs the persistent form");
     const bool is64 = p->precision == TL_FP64;
+    const bool sens = p->sensitivities != 0;
+    const bool grid = p->n_intervals > 0;
     const int64_t R = p->n_realizations;
     const size_t rs = is64 ? 8 : 4;
     const std::vector<int32_t> trk = track_list(m, p);
     const int32_t T = (int32_t)trk.size();
+    const int32_t n_grid = grid ? p->n_intervals + 1 : 1;
+    if (grid && (double)R * n_grid * T * 8.0 > 64e9) return set_error(TL_ERR_TOO_LARGE, "time-grid output exceeds 64 GB");
     // channel chunks so that the propensity pass has >= ~256k threads, and at
     // least ~sqrt(M) of them (the critical pick walks chunk sums, then one
-    // chunk's channels) while the partial sums stay within 4M entries
+    // chunk's channels) while the partial sums stay within 4M entries; at
+    // most 65535 (one grid row each)
     const int64_t by_threads = std::max<int64_t>(1, (262144 + R - 1) / R);
     const int64_t by_walk = std::min<int64_t>((int64_t)std::ceil(std::sqrt((double)m->M)), std::max<int64_t>(1, 4194304 / R));
-    int32_t nchunk = (int32_t)std::min<int64_t>(std::max(by_threads, by_walk), std::max(1, m->M / 64));
+    int32_t nchunk = (int32_t)std::min<int64_t>(std::min<int64_t>(std::max(by_threads, by_walk), std::max(1, m->M / 64)),
+                                                65535);
     DevBufs B;
     int32_t *x, *xn, *k, *d_track, *d_outx;
     void *a, *tau_i;
@@ -368,7 +402,7 @@ tl_status run_plain(const tl_model *m, const tl_sim_params *p, tl_result *res) {
     // least 64 rows each, the species sums and each chunk's minimum are one
     // pass and no per-species bound is stored; otherwise the bounds are
     // stored and combined over row chunks of >= ~256k threads
-    const int32_t row_cap = std::max(1, m->N / 64);
+    const int32_t row_cap = std::min(65535, std::max(1, m->N / 64));
     const int32_t nrc_fused = (int32_t)std::min<int64_t>((1048576 + R - 1) / R, row_cap);
     const bool fused = nrc_fused > 1 && (int64_t)nrc_fused * R >= 262144;
     const int32_t nrc =
@@ -418,28 +452,63 @@ tl_status run_plain(const tl_model *m, const tl_sim_params *p, tl_result *res) {
     TL_TRY(B.alloc(st.neg, R));
     TL_TRY(B.alloc(st.n_done, 1));
     TL_HIP(hipMemcpyAsync(d_track, trk.data(), T * 4, hipMemcpyHostToDevice, s));
+    // sensitivities: scores and dt/dc [M][R], the per-chunk sums of a leap's
+    // terms, and the tau stage's arg-min species and bound
+    double *qpart = nullptr, *d_scoreT = nullptr, *d_mean = nullptr, *d_dmean = nullptr, *d_dse = nullptr;
+    int32_t *argmin = nullptr, *which = nullptr;
+    if (sens) {
+        TL_TRY(B.alloc(st.S, (size_t)m->M * R));
+        TL_TRY(B.alloc(st.D, (size_t)m->M * R));
+        TL_TRY(B.alloc(qpart, (size_t)nchunk * R));
+        TL_TRY(B.alloc(argmin, R));
+        TL_TRY(B.alloc(which, R));
+        TL_TRY(B.alloc(d_mean, T));
+        TL_TRY(B.alloc(d_dmean, (size_t)T * m->M));
+        TL_TRY(B.alloc(d_dse, (size_t)T * m->M));
+    }
+    double *d_grid = nullptr;
+    if (grid) {
+        TL_TRY(B.alloc(d_grid, (size_t)R * n_grid * T));
+        TL_TRY(B.alloc(st.next_g, R));
+        TL_TRY(B.alloc(st.snap, (size_t)T * R));
+        st.grid = d_grid;
+        st.track = d_track;
+        st.n_track = T;
+        st.n_intervals = p->n_intervals;
+    }
 
-    hipEvent_t e0, e1;
-    TL_HIP(hipEventCreate(&e0));
-    TL_HIP(hipEventCreate(&e1));
+    Events ev;
+    TL_TRY(ev.create());
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     const double t0 = now_s();
     TL_HIP(hipEventRecord(e0, s));
     launch_fill_u64(st.n_done, 0ull, 1, s);
     launch_plain_init(dm, R, x, st, s);
+    if (sens) {
+        launch_fill_f64(st.S, 0.0, (int64_t)m->M * R, s);
+        launch_fill_f64(st.D, 0.0, (int64_t)m->M * R, s);
+    }
+    if (grid) launch_fill_u8((uint8_t *)st.next_g, 0, R * 4, s);
     unsigned long long *h_done = nullptr;
     TL_HIP(hipHostMalloc((void **)&h_done, sizeof(unsigned long long)));
+    std::unique_ptr<unsigned long long, hipError_t (*)(void *)> done_guard(h_done, hipHostFree);
     *h_done = 0;
     const uint64_t seed = p->seed;
+    const bool exact = p->poisson == TL_POISSON_EXACT;
     for (int64_t it = 0;; ++it) {
         launch_prop(is64, dm, R, nchunk, x, a, crit, a0, a0c, part, partc, st.active, s);
-        launch_tau(is64, dm, R, p->eps, x, a, crit, tau_i, which_i, tau1, nullptr, nullptr, st.active, s, nrc, pbest,
+        launch_tau(is64, dm, R, p->eps, x, a, crit, tau_i, which_i, tau1, argmin, which, st.active, s, nrc, pbest,
                    pidx, pwhich);
         launch_plain_select(is64, dm, R, seed, p->model_id, p->realization_offset, p->t_final, p->max_iter, x, a, crit,
                             a0, a0c, tau1, st, nchunk, partc, s);
         launch_fire(is64, dm, R, seed, p->model_id, p->realization_offset, st.epoch, a, crit, st.tau, st.jc, k, st.mode,
-                    p->poisson == TL_POISSON_EXACT, s);
+                    exact, s);
         launch_update(dm, R, x, k, xn, st.neg, st.mode, s);
+        if (grid) launch_plain_grid(dm, R, x, xn, st, p->t_final, s);
         launch_plain_commit(dm, R, x, xn, st, p->t_final, s);
+        if (sens) launch_plain_score(is64, dm, R, nchunk, a, crit, k, argmin, which, qpart, st, exact, s);
+        // a failed launch would leave the done counter short: stop on it
+        TL_HIP(hipGetLastError());
         if ((it & 15) == 15) {
             TL_HIP(hipMemcpyAsync(h_done, st.n_done, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
             TL_HIP(hipStreamSynchronize(s));
@@ -448,6 +517,14 @@ tl_status run_plain(const tl_model *m, const tl_sim_params *p, tl_result *res) {
     }
     TL_HIP(hipEventRecord(e1, s));
     launch_gather_track(R, T, d_track, x, d_outx, s);
+    if (grid) launch_plain_grid_final(dm, R, x, st, p->t_final, s);
+    const bool want_grad = sens && res && (res->dmean_dc || res->dmean_dc_se);
+    if (sens && res && (res->score || want_grad)) {
+        TL_TRY(B.alloc(d_scoreT, (size_t)R * m->M));
+        launch_transpose_f64(R, m->M, st.S, d_scoreT, s);
+        if (want_grad) launch_grad_reduce(R, T, m->M, d_outx, d_scoreT, d_mean, d_dmean, d_dse, s);
+    }
+    TL_HIP(hipGetLastError());
     std::vector<int32_t> hx((size_t)T * R), hst(R);
     std::vector<int64_t> hsteps(R), hssa(R), hrej(R);
     TL_HIP(hipMemcpyAsync(hx.data(), d_outx, hx.size() * 4, hipMemcpyDeviceToHost, s));
@@ -455,19 +532,34 @@ tl_status run_plain(const tl_model *m, const tl_sim_params *p, tl_result *res) {
     TL_HIP(hipMemcpyAsync(hssa.data(), st.ssa, R * 8, hipMemcpyDeviceToHost, s));
     TL_HIP(hipMemcpyAsync(hrej.data(), st.rej, R * 8, hipMemcpyDeviceToHost, s));
     TL_HIP(hipMemcpyAsync(hst.data(), st.status, R * 4, hipMemcpyDeviceToHost, s));
+    std::vector<double> hg;
+    if (grid) {
+        hg.resize((size_t)R * n_grid * T);
+        TL_HIP(hipMemcpyAsync(hg.data(), d_grid, hg.size() * 8, hipMemcpyDeviceToHost, s));
+    }
+    if (res && sens) {
+        if (res->score) TL_HIP(hipMemcpyAsync(res->score, d_scoreT, (size_t)R * m->M * 8, hipMemcpyDeviceToHost, s));
+        if (res->dmean_dc)
+            TL_HIP(hipMemcpyAsync(res->dmean_dc, d_dmean, (size_t)T * m->M * 8, hipMemcpyDeviceToHost, s));
+        if (res->dmean_dc_se)
+            TL_HIP(hipMemcpyAsync(res->dmean_dc_se, d_dse, (size_t)T * m->M * 8, hipMemcpyDeviceToHost, s));
+    }
     TL_HIP(hipStreamSynchronize(s));
     float ms = 0.f;
     hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    hipHostFree(h_done);
     if (res) {
         res->kernel_ms = ms;
         res->upload_seconds = up;
         res->block_size = 256;
+        res->form = TL_MODE_PLAIN;
         if (res->final_x) std::copy(hx.begin(), hx.end(), res->final_x);
-        moments(hx.data(), R, T, res->mean, res->stddev);
-        if (res->grid_times) res->grid_times[0] = p->t_final;
+        if (grid) {
+            if (res->grid) std::copy(hg.begin(), hg.end(), res->grid);
+            grid_moments(hg, R, n_grid, T, p->t_final, res);
+        } else {
+            moments(hx.data(), R, T, res->mean, res->stddev);
+            if (res->grid_times) res->grid_times[0] = p->t_final;
+        }
         fill_summary(res, hsteps, hssa, hrej, hst);
         res->sim_seconds = now_s() - t0;
     }
@@ -489,10 +581,11 @@ struct Entry {
 };
 
 struct TuneKey {
-    bool is64, sens, ext;
+    bool is64, sens, ext, timed;
     int32_t n_max, m_max;
     bool operator<(const TuneKey &o) const {
-        return std::tie(is64, sens, ext, n_max, m_max) < std::tie(o.is64, o.sens, o.ext, o.n_max, o.m_max);
+        return std::tie(is64, sens, ext, timed, n_max, m_max) <
+               std::tie(o.is64, o.sens, o.ext, o.timed, o.n_max, o.m_max);
     }
 };
 std::mutex g_tune_mu;
@@ -685,12 +778,17 @@ tl_status launch_batch(DeviceContext &dc, const PersistentConfig &cfg, bool is64
     return run_launches(dc, is64, sens, cl, use_graph, ms_out);
 }
 
-// Time each launch shape on a bounded sample (first few steps of a subset
-// of the work items) and keep the fastest in realization-steps per second.
-tl_status autotune(DeviceContext &dc, bool is64, bool sens, bool ext, const std::vector<ModelItem> &items_in,
-                   int64_t n_items, int32_t n_max, int32_t m_max, unsigned long long *d_counter, int64_t *d_steps_all,
-                   PersistentConfig &best) {
-    TuneKey key{is64, sens, ext, n_max, m_max};
+// The launch shape of a size class.  Default (timed = false): the measured
+// preferences of `heuristic`, a pure function of the sizes, precision and
+// kernel build, so that a seed gives the same trajectories on every run and
+// every box.  timed = true (block_size TL_BLOCK_TUNE): time each launch shape
+// on a bounded sample (first few steps of a subset of the work items) and
+// keep the fastest in realization-steps per second; shapes that reduce the
+// species sums in another order then round differently, so the trajectories
+// depend on the timing.
+tl_status autotune(DeviceContext &dc, bool is64, bool sens, bool ext, bool timed, const std::vector<ModelItem> &items_in,
+                   int64_t n_items, int32_t n_max, int32_t m_max, unsigned long long *d_counter, PersistentConfig &best) {
+    TuneKey key{is64, sens, ext, timed, n_max, m_max};
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
         auto it = g_tuned.find(key);
@@ -702,7 +800,7 @@ tl_status autotune(DeviceContext &dc, bool is64, bool sens, bool ext, const std:
     auto all = candidates(dc, is64, sens, n_max, m_max, ext);
     if (all.empty()) return set_error(TL_ERR_TOO_LARGE, "model too large for any persistent launch shape");
     best = heuristic(all, m_max);
-    if (best.hbm) {  // global-memory state shapes are not timed (no trial launches)
+    if (best.hbm || !timed) {  // global-memory state shapes are not timed (no trial launches)
         std::lock_guard<std::mutex> g(g_tune_mu);
         g_tuned[key] = best;
         return TL_OK;
@@ -755,7 +853,6 @@ tl_status autotune(DeviceContext &dc, bool is64, bool sens, bool ext, const std:
         hipFree(d_trial);
         if (first_rate > 0 && best_rate < 1.15 * first_rate) best = first;
     }
-    (void)d_steps_all;
     std::lock_guard<std::mutex> g(g_tune_mu);
     g_tuned[key] = best;
     return TL_OK;
@@ -901,7 +998,8 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
             if (!found)
                 return set_error(TL_ERR_INVALID_ARG, "block_size " + std::to_string(bs_req) + " does not fit this model");
         } else {
-            TL_TRY(autotune(*dc, is64, sens, ext, items, n_items, n_max, m_max, d_counter, nullptr, cfg));
+            TL_TRY(autotune(*dc, is64, sens, ext, bs_req == TL_BLOCK_TUNE, items, n_items, n_max, m_max, d_counter,
+                            cfg));
         }
         ClassLaunch c;
         c.cfg = cfg;
@@ -946,15 +1044,8 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
         tl_result *res = e.res;
         if (res->final_x) std::copy(hx.begin(), hx.end(), res->final_x);
         if (e.p->n_intervals > 0) {
-            // per grid point over realizations: hg is [R][n_grid][T]
-            std::vector<double> slice((size_t)R * T);
-            for (int32_t g = 0; g < n_grid; ++g) {
-                for (int64_t r = 0; r < R; ++r)
-                    for (int32_t q = 0; q < T; ++q) slice[r * T + q] = hg[((size_t)r * n_grid + g) * T + q];
-                moments(slice.data(), R, T, res->mean ? res->mean + (size_t)g * T : nullptr,
-                        res->stddev ? res->stddev + (size_t)g * T : nullptr);
-                if (res->grid_times) res->grid_times[g] = e.p->t_final / e.p->n_intervals * g;
-            }
+            if (res->grid) std::copy(hg.begin(), hg.end(), res->grid);
+            grid_moments(hg, R, n_grid, T, e.p->t_final, res);
         } else {
             moments(hx.data(), R, T, res->mean, res->stddev);
             if (res->grid_times) res->grid_times[0] = e.p->t_final;
@@ -962,6 +1053,7 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
         fill_summary(res, hsteps, hssa, hrej, hst);
         res->kernel_ms = ms;
         res->upload_seconds = up_total;
+        res->form = TL_MODE_PERSISTENT;
         const PersistentConfig &ecfg = entry_cfg[&e - &E[0]];
         res->block_size = ecfg.group == 64 ? 64 : ecfg.block;
         res->sim_seconds = now_s() - t0;
@@ -999,70 +1091,26 @@ void tl_sim_params_default(tl_sim_params *p) {
     p->mode = TL_MODE_AUTO;
 }
 
-// AUTO choice for a model whose state does not fit LDS, cached per model
-// handle, sizes and version, precision and kernel build.  A model created
-// later at a freed handle's address with the same sizes would reuse the
-// choice; that affects speed only, never results.
-struct FormKey {
-    const tl_model *m;
-    int32_t N, M, nnz;
-    uint64_t version;
-    bool is64, ext;
-    bool operator<(const FormKey &o) const {
-        return std::tie(m, N, M, nnz, version, is64, ext) < std::tie(o.m, o.N, o.M, o.nnz, o.version, o.is64, o.ext);
-    }
-};
-std::mutex g_form_mu;
-std::map<FormKey, bool> g_form_plain;
-
-// AUTO mode: whether a model runs in the plain form (the call must need
-// nothing only the persistent form provides: scores, a time grid).  Always
-// when no persistent launch shape fits; and for a model whose state does not
-// fit LDS, when the plain form (realizations across the lanes, every gather
-// coalesced) runs the first steps of a sample more than 15 % faster than the
-// HBM-state persistent kernel.  Measured on 1e6 channels, all leaps: plain 4x
-// faster; a model that falls back to direct events stays persistent (the
-// plain form pays an O(M) pass per realization thread per event).
+// AUTO mode: whether a model runs in the plain form.  Always when no
+// persistent launch shape fits; and for a model whose state does not fit LDS
+// when the persistent kernel would keep a[M] in HBM (more than 16 propensity
+// tiles).  Measured on 1e6 channels (62 tiles), all leaps: the plain form
+// (realizations across the lanes, every gather coalesced) 4x faster than the
+// HBM-state persistent kernel; on 1e5 channels (7 tiles) within 15 %, where
+// the persistent kernel stays (a direct-method event costs the plain form an
+// O(M) pass per realization thread).  A pure function of the sizes, the
+// precision and the kernel build: the same call always runs the same form.
 static tl_status auto_plain(const tl_model *m, const tl_sim_params *p, bool &plain) {
     plain = false;
-    if (p->sensitivities || p->n_intervals > 0) return TL_OK;
     DeviceContext *dc;
     TL_TRY(device_context(p->device, dc));
     const bool is64 = p->precision == TL_FP64, ext = needs_ext(m, p);
-    const auto cands = candidates(*dc, is64, false, m->N, m->M, ext);
+    const auto cands = candidates(*dc, is64, p->sensitivities != 0, m->N, m->M, ext);
     if (cands.empty()) {
         plain = true;
         return TL_OK;
     }
-    if (!cands.front().hbm) return TL_OK;
-    const int64_t sample = std::min<int64_t>(p->n_realizations, (int64_t)dc->n_cu * 4);
-    if (p->n_realizations < 2 * sample) return TL_OK;  // the trial would cost as much as it saves
-    const FormKey key{m, m->N, m->M, m->nnz, m->version, is64, ext};
-    {
-        std::lock_guard<std::mutex> g(g_form_mu);
-        auto it = g_form_plain.find(key);
-        if (it != g_form_plain.end()) {
-            plain = it->second;
-            return TL_OK;
-        }
-    }
-    // both forms on the first 16 steps of the same sample of realizations
-    tl_sim_params t = *p;
-    t.n_realizations = sample;
-    t.max_iter = p->max_iter > 0 ? std::min<int64_t>(p->max_iter, 16) : 16;
-    t.n_track = 1;
-    const int32_t q0 = 0;
-    t.track = &q0;
-    tl_result rp = {}, rs = {};
-    t.mode = TL_MODE_PLAIN;
-    TL_TRY(run_plain(m, &t, &rp));
-    t.mode = TL_MODE_PERSISTENT;
-    TL_TRY(run_persistent(&m, 1, &t, &rs));
-    const double plain_rate = rp.kernel_ms > 0 ? (double)rp.total_steps / rp.kernel_ms : 0.0;
-    const double pers_rate = rs.kernel_ms > 0 ? (double)rs.total_steps / rs.kernel_ms : 0.0;
-    plain = plain_rate > 1.15 * pers_rate;
-    std::lock_guard<std::mutex> g(g_form_mu);
-    g_form_plain[key] = plain;
+    plain = cands.front().hbm && cands.front().a_lds;
     return TL_OK;
 }
 
@@ -1127,6 +1175,10 @@ static tl_status stage_setup(const tl_model *m, int32_t precision, int64_t R, De
     int dev = 0;
     hipGetDevice(&dev);
     TL_TRY(device_context(dev, dc));
+    // the caller's buffers may still be written (or, freshly recycled by a
+    // caching allocator, read) by its own work on other streams: the stage
+    // runs after all of it
+    TL_HIP(hipDeviceSynchronize());
     return device_model(*m, dev, dm, nullptr);
 }
 

@@ -307,6 +307,21 @@ __device__ __forceinline__ void step_uniforms(uint2 key, uint32_t epoch, uint32_
     r2 = unif64(o.y);
 }
 
+// Likelihood-ratio score term d_j * m_j of one channel's firing count k_j at
+// mean m_j = a_j * tau, d_j = d log p(k_j) / d m_j: k - m for Poisson counts
+// (and every channel with the exact sampler) and critical channels; for the
+// normal approximation at mean >= 80 the density of N(m, m) at k + 1/2,
+// y - m + ((y - m)^2 / m - 1) / 2 with y = k + 1/2.
+template <class R>
+__device__ __forceinline__ double score_term(bool cr, R aj, R tr, double tau, int32_t kj, bool exact) {
+    const double mj = (double)aj * tau;
+    if (!exact && !cr && aj * tr >= (R)kPoissonSwitch) {
+        const double d = ((double)kj + 0.5) - mj;
+        return d + 0.5 * (d * d / mj - 1.0);
+    }
+    return (double)kj - mj;
+}
+
 __device__ __forceinline__ int32_t entry_index(int32_t p) { return p >> 3; }
 __device__ __forceinline__ int32_t entry_value(int32_t p) { return (p & 7) - 2; }
 

@@ -86,16 +86,41 @@ struct PlainState {
     int32_t *status, *retries, *jc;
     uint8_t *active, *mode, *clamped, *neg;
     unsigned long long *n_done;
+    // sensitivities (null without): score S and dt/dc D, [M][R] FP64
+    double *S = nullptr, *D = nullptr;
+    // time grid (grid null without): the tracked species [n_track] of each
+    // realization at k * t_final / n_intervals into grid [R][n_intervals+1]
+    // [n_track]; next_g [R] is the next grid index, snap [n_track][R] the
+    // tracked state before a direct-method block
+    double *grid = nullptr;
+    int32_t *next_g = nullptr, *snap = nullptr;
+    const int32_t *track = nullptr;
+    int32_t n_track = 0, n_intervals = 0;
 };
 void launch_plain_init(const DevModel &m, int64_t R, int32_t *x, PlainState st, hipStream_t s);
 void launch_plain_select(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id,
                          int64_t rbase, double t_final, int64_t max_iter, int32_t *x, void *a, const uint8_t *crit,
                          const double *a0, const double *a0c, const double *tau1, PlainState st, int32_t nchunk,
                          const double *partc, hipStream_t s);
+// Grid points passed by the accepted leaps of this iteration (before the
+// commit overwrites x with x_new).
+void launch_plain_grid(const DevModel &m, int64_t R, const int32_t *x, const int32_t *x_new, PlainState st,
+                       double t_final, hipStream_t s);
 void launch_plain_commit(const DevModel &m, int64_t R, int32_t *x, const int32_t *x_new, PlainState st,
                          double t_final, hipStream_t s);
+// Likelihood-ratio terms of the accepted leaps of this iteration (after the
+// commit): qpart [nchunk][R] scratch; argmin / which of the tau stage.
+void launch_plain_score(bool is64, const DevModel &m, int64_t R, int32_t nchunk, const void *a, const uint8_t *crit,
+                        const int32_t *k, const int32_t *argmin, const int32_t *which, double *qpart, PlainState st,
+                        bool exact, hipStream_t s);
+// Grid points at and after each realization's final time take its final state.
+void launch_plain_grid_final(const DevModel &m, int64_t R, const int32_t *x, PlainState st, double t_final,
+                             hipStream_t s);
 void launch_gather_track(int64_t R, int32_t n_track, const int32_t *track, const int32_t *x, int32_t *out,
                          hipStream_t s);
+// out [R][M] = in [M][R]
+void launch_transpose_f64(int64_t R, int32_t M, const double *in, double *out, hipStream_t s);
+void launch_fill_f64(double *p, double v, int64_t n, hipStream_t s);
 
 // ----------------------------------------------------------- persistent --
 // One work item = one realization of one model.

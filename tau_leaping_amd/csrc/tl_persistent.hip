@@ -904,18 +904,6 @@ __device__ __forceinline__ void prop_update(const GModel &m, gptr<const R> c, co
 }
 
 
-// Score term d_j * m_j of one channel (see the likelihood-ratio notes in
-// simulate_item).
-template <class R>
-__device__ __forceinline__ double score_term(bool cr, R aj, R tr, double tau, int32_t kj, bool exact) {
-    const double mj = (double)aj * tau;
-    if (!exact && !cr && aj * tr >= (R)kPoissonSwitch) {
-        const double d = ((double)kj + 0.5) - mj;
-        return d + 0.5 * (d * d / mj - 1.0);
-    }
-    return (double)kj - mj;
-}
-
 // Visit every channel of one leap attempt with its firing count k_j.  Lane l
 // takes the four consecutive channels 4l..4l+3 of each span of 4*GS
 // channels, so the Philox block holding their first draws (channels 4b..4b+3

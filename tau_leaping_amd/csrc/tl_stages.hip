@@ -490,11 +490,12 @@ __global__ __launch_bounds__(kT, 5) void k_fire(DevModel m, int64_t Rn, uint64_t
                                                 const uint8_t *__restrict__ mode, int exact) {
     // a thread takes channels 4q..4q+3 of one realization: their first draws
     // share one Philox block (head_block), computed once while the channels'
-    // propensities and criticality flags load
-    const int64_t rt = (Rn + kT - 1) / kT;
-    const int32_t q = (int32_t)(blockIdx.x / rt);
-    const int64_t r = (int64_t)(blockIdx.x % rt) * kT + threadIdx.x;
-    if (r >= Rn) return;
+    // propensities and criticality flags load.  blockIdx.x = realization
+    // tile, q = blockIdx.y + gridDim.y * blockIdx.z (grid rows hold at most
+    // 65535 quads).
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    const int32_t q = (int32_t)(blockIdx.y + gridDim.y * blockIdx.z);
+    if (r >= Rn || 4 * q >= m.M) return;
     if (mode && mode[r] != 1) return;
     const int nq = min(4, m.M - 4 * q);
     R av[4];
@@ -596,10 +597,14 @@ __device__ double prop_soa(const DevModel &m, const int32_t *x, R *a, uint8_t *c
 }
 
 // One direct-method event on column r (the fixed DirectMethodStep): returns
-// false when the realization reached t_final instead.
+// false when the realization reached t_final instead.  With scores S (and
+// dt/dc D), [M][R]: the event adds 1/c_jf for the fired channel and the
+// exposure -a_j dt / c_j of every channel; the truncated last interval
+// rem = t_final - t adds d/dc_j of -a0 rem, a0 D_j - a_j rem / c_j.
 template <class R>
 __device__ bool ssa_event_soa(const DevModel &m, uint2 key, uint32_t c3, uint32_t &epoch, int32_t *x, int64_t Rn,
-                              int64_t r, double &t, double tf, const R *a, double a0, int32_t *fired = nullptr) {
+                              int64_t r, double &t, double tf, const R *a, double a0, int32_t *fired = nullptr,
+                              double *S = nullptr, const double *D = nullptr) {
     if (!(a0 > 0.0)) {
         t = tf;
         return false;
@@ -608,7 +613,15 @@ __device__ bool ssa_event_soa(const DevModel &m, uint2 key, uint32_t c3, uint32_
     step_uniforms(key, epoch, c3 | kKindSsa, r1, r2);
     epoch++;
     const double dt = -log(r2) / a0;
+    const R *rc = Real<R>::rates(m);
     if (dt >= tf - t) {
+        if (S) {
+            const double rem = tf - t;
+            for (int32_t j = 0; j < m.M; ++j) {
+                const int64_t o = (int64_t)j * Rn + r;
+                if (rc[j] != (R)0) S[o] += a0 * D[o] - (double)a[o] * rem / (double)rc[j];
+            }
+        }
         t = tf;
         return false;
     }
@@ -635,6 +648,13 @@ __device__ bool ssa_event_soa(const DevModel &m, uint2 key, uint32_t c3, uint32_
     for (int32_t e = m.col_ptr[jf]; e < m.col_ptr[jf + 1]; ++e) {
         const int32_t p = m.csc[e];
         x[(int64_t)entry_index(p) * Rn + r] += entry_value(p);
+    }
+    if (S) {
+        for (int32_t j = 0; j < m.M; ++j) {
+            const int64_t o = (int64_t)j * Rn + r;
+            if (rc[j] != (R)0) S[o] -= (double)a[o] * dt / (double)rc[j];
+        }
+        if (rc[jf] != (R)0) S[(int64_t)jf * Rn + r] += 1.0 / (double)rc[jf];
     }
     t += dt;
     if (fired) *fired = jf;
@@ -701,6 +721,29 @@ __global__ __launch_bounds__(kT) void k_plain_init(DevModel m, int64_t Rn, int32
     st.neg[r] = 0;
 }
 
+// Grid points of realization r passed by a step from (t_a, state xa) to
+// (t_b, state xb): linear interpolation between the two states, as
+// AllStat2StepStat does between logged states (statistics.cpp:15-64).
+// xa(q), xb(q) give tracked species q.
+template <class FA, class FB>
+__device__ void grid_step(const PlainState &st, int64_t r, double tf, double t_a, double t_b, FA xa, FB xb) {
+    const int32_t n = st.n_intervals, T = st.n_track;
+    const double h = tf / (double)n;
+    int32_t g = st.next_g[r];
+    while (g < n) {
+        const double tg = h * (double)g;
+        if (!(tg < t_b)) break;
+        const double w = (t_b > t_a) ? (tg - t_a) / (t_b - t_a) : 0.0;
+        double *out = st.grid + ((size_t)r * (n + 1) + g) * T;
+        for (int32_t q = 0; q < T; ++q) {
+            const double va = (double)xa(q), vb = (double)xb(q);
+            out[q] = va + (vb - va) * w;
+        }
+        ++g;
+    }
+    st.next_g[r] = g;
+}
+
 template <class R>
 __global__ __launch_bounds__(kT) void k_plain_select(DevModel m, int64_t Rn, uint64_t seed, uint32_t model_id,
                                                      int64_t rbase, double tf, int64_t max_iter,
@@ -724,7 +767,9 @@ __global__ __launch_bounds__(kT) void k_plain_select(DevModel m, int64_t Rn, uin
     }
     const double a0r = a0[r];
     double t = st.t[r];
+    auto xtrk = [&](int32_t q) { return x[(int64_t)st.track[q] * Rn + r]; };
     if (!(a0r > 0.0)) {  // absorbing state
+        if (st.grid) grid_step(st, r, tf, t, tf, xtrk, xtrk);
         st.t[r] = tf;
         st.steps[r] += 1;
         st.active[r] = 0;
@@ -738,15 +783,19 @@ __global__ __launch_bounds__(kT) void k_plain_select(DevModel m, int64_t Rn, uin
     const uint2 key = rng_key(seed, rbase + r);
     const uint32_t c3 = model_id << 2;
     if (tau1r < kSsaFactor / a0r) {  // direct block
+        const double t_before = t;
+        if (st.grid)
+            for (int32_t q = 0; q < st.n_track; ++q) st.snap[(int64_t)q * Rn + r] = xtrk(q);
         uint32_t ep = st.epoch[r];
         double a0s = a0r;
         int64_t n = 0;
         int32_t jf = -1;
         for (int s = 0; s < kSsaSteps; ++s) {
             if (s > 0) a0s = refresh_soa<R>(m, x, a, Rn, r, jf);
-            if (!ssa_event_soa<R>(m, key, c3, ep, x, Rn, r, t, tf, a, a0s, &jf)) break;
+            if (!ssa_event_soa<R>(m, key, c3, ep, x, Rn, r, t, tf, a, a0s, &jf, st.S, st.D)) break;
             ++n;
         }
+        if (st.grid) grid_step(st, r, tf, t_before, t, [&](int32_t q) { return st.snap[(int64_t)q * Rn + r]; }, xtrk);
         st.t[r] = t;
         st.epoch[r] = ep;
         st.ssa[r] += n;
@@ -809,6 +858,134 @@ __global__ __launch_bounds__(kT) void k_plain_commit(DevModel m, int64_t Rn, int
             atomicAdd(st.n_done, 1ull);
         }
     }
+}
+
+// Grid points passed by each accepted leap (runs before the commit, while x
+// still holds the state before the leap and x_new the state after it).
+__global__ __launch_bounds__(kT) void k_plain_grid(int64_t Rn, const int32_t *__restrict__ x,
+                                                   const int32_t *__restrict__ x_new, PlainState st, double tf) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= Rn) return;
+    if (st.mode[r] != 1 || st.neg[r]) return;
+    const double t = st.t[r];
+    const double tn = st.clamped[r] ? tf : t + st.tau[r];
+    if (!(tn > t)) return;  // the commit ends this realization as stalled
+    grid_step(st, r, tf, t, tn, [&](int32_t q) { return x[(int64_t)st.track[q] * Rn + r]; },
+              [&](int32_t q) { return x_new[(int64_t)st.track[q] * Rn + r]; });
+}
+
+// Grid points at and after the final time of a realization that ended OK
+// take its final state.
+__global__ __launch_bounds__(kT) void k_plain_grid_final(int64_t Rn, const int32_t *__restrict__ x, PlainState st) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= Rn) return;
+    if (st.status[r] != kOk) return;
+    const int32_t n = st.n_intervals, T = st.n_track;
+    for (int32_t g = st.next_g[r]; g <= n; ++g)
+        for (int32_t q = 0; q < T; ++q)
+            st.grid[((size_t)r * (n + 1) + g) * T + q] = (double)x[(int64_t)st.track[q] * Rn + r];
+}
+
+// Likelihood-ratio terms of an accepted leap, channel chunk blockIdx.y:
+// S_j += term_j / c_j and the chunk's sum of terms into qpart (see
+// score_term).
+template <class R>
+__global__ __launch_bounds__(kT) void k_plain_score_leap(DevModel m, int64_t Rn, int32_t nchunk,
+                                                         const R *__restrict__ a, const uint8_t *__restrict__ crit,
+                                                         const int32_t *__restrict__ k, double *__restrict__ qpart,
+                                                         PlainState st, int exact) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= Rn) return;
+    if (st.mode[r] != 1 || st.neg[r]) return;
+    const int32_t c = blockIdx.y;
+    const int32_t ch = (m.M + nchunk - 1) / nchunk;
+    const int32_t j0 = c * ch, j1 = min(m.M, j0 + ch);
+    const R *rc = Real<R>::rates(m);
+    const double tau = st.tau[r];
+    const R tr = (R)tau;
+    double q = 0.0;
+    for (int32_t j = j0; j < j1; ++j) {
+        const int64_t o = (int64_t)j * Rn + r;
+        const double term = score_term<R>(crit[o] != 0, a[o], tr, tau, k[o], exact != 0);
+        q += term;
+        if (rc[j] != (R)0) st.S[o] += term / (double)rc[j];
+    }
+    qpart[(int64_t)c * Rn + r] = q;
+}
+
+// The step-size terms of an accepted leap, with Q = the sum of its terms:
+// the clamped last leap (tau = t_final - t) adds -(Q / tau) D_j; a leap of
+// size tau1 adds (Q / tau) dtau1/dc_j over the arg-min species row (mean
+// bound -tau sgn(mu) v h / |mu|, variance bound -tau v^2 h / sigma^2,
+// h = a_j / c_j) and accumulates dtau1/dc_j into D.
+template <class R>
+__global__ __launch_bounds__(kT) void k_plain_score_fin(DevModel m, int64_t Rn, int32_t nchunk,
+                                                        const R *__restrict__ a, const uint8_t *__restrict__ crit,
+                                                        const int32_t *__restrict__ argmin,
+                                                        const int32_t *__restrict__ which,
+                                                        const double *__restrict__ qpart, PlainState st) {
+    const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (r >= Rn) return;
+    if (st.mode[r] != 1 || st.neg[r]) return;
+    double Q = 0.0;
+    for (int32_t c = 0; c < nchunk; ++c) Q += qpart[(int64_t)c * Rn + r];
+    const double tau = st.tau[r];
+    if (st.clamped[r]) {
+        for (int32_t j = 0; j < m.M; ++j) {
+            const int64_t o = (int64_t)j * Rn + r;
+            st.S[o] -= (Q / tau) * st.D[o];
+        }
+        return;
+    }
+    const int32_t istar = argmin[r];
+    if (st.jc[r] >= 0 || istar < 0) return;
+    const R *rc = Real<R>::rates(m);
+    double mu = 0.0, sig = 0.0;
+    const int32_t e0 = m.row_ptr[istar], e1 = m.row_ptr[istar + 1];
+    for (int32_t e = e0; e < e1; ++e) {
+        const int32_t p = m.csr[e];
+        const int64_t o = (int64_t)entry_index(p) * Rn + r;
+        if (crit[o]) continue;
+        const double v = (double)entry_value(p);
+        mu += v * (double)a[o];
+        sig += v * v * (double)a[o];
+    }
+    const int w = which[r];
+    for (int32_t e = e0; e < e1; ++e) {
+        const int32_t p = m.csr[e], j = entry_index(p);
+        const int64_t o = (int64_t)j * Rn + r;
+        if (crit[o] || rc[j] == (R)0) continue;
+        const double v = (double)entry_value(p);
+        const double h = (double)a[o] / (double)rc[j];
+        const double dtau = (w == 0) ? -tau * (mu > 0.0 ? 1.0 : -1.0) * v * h / fabs(mu) : -tau * v * v * h / sig;
+        st.S[o] += (Q / tau) * dtau;
+        st.D[o] += dtau;
+    }
+}
+
+// out[r][j] = in[j][r] through an LDS tile of 64 x 64 (both sides coalesced).
+__global__ __launch_bounds__(256) void k_transpose_f64(int64_t Rn, int32_t M, const double *__restrict__ in,
+                                                       double *__restrict__ out) {
+    __shared__ double tile[64][65];
+    const int64_t r0 = (int64_t)blockIdx.x * 64;
+    const int32_t j0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int jj = ty; jj < 64; jj += 4) {
+        const int32_t j = j0 + jj;
+        const int64_t r = r0 + tx;
+        tile[jj][tx] = (j < M && r < Rn) ? in[(int64_t)j * Rn + r] : 0.0;
+    }
+    __syncthreads();
+    for (int rr = ty; rr < 64; rr += 4) {
+        const int64_t r = r0 + rr;
+        const int32_t j = j0 + tx;
+        if (j < M && r < Rn) out[r * M + j] = tile[tx][rr];
+    }
+}
+
+__global__ __launch_bounds__(kT) void k_fill_f64(double *p, double v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 __global__ __launch_bounds__(kT) void k_gather_track(int64_t Rn, int32_t T, const int32_t *__restrict__ track,
@@ -907,13 +1084,14 @@ void launch_select(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint3
 void launch_fire(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase,
                  const uint32_t *epoch, const void *a, const uint8_t *crit, const double *tau, const int32_t *jc,
                  int32_t *k, const uint8_t *mode, bool exact, hipStream_t s) {
-    const int64_t nb = (int64_t)blocks_for(R) * ((m.M + 3) / 4);
+    const int32_t nquad = (m.M + 3) / 4;
+    const dim3 g(blocks_for(R), row_grid(nquad), (unsigned)((nquad + 65534) / 65535));
     if (is64)
-        k_fire<double><<<(unsigned)nb, kT, 0, s>>>(m, R, seed, model_id, rbase, epoch, (const double *)a, crit, tau, jc,
-                                                   k, mode, exact ? 1 : 0);
+        k_fire<double><<<g, kT, 0, s>>>(m, R, seed, model_id, rbase, epoch, (const double *)a, crit, tau, jc, k, mode,
+                                        exact ? 1 : 0);
     else
-        k_fire<float><<<(unsigned)nb, kT, 0, s>>>(m, R, seed, model_id, rbase, epoch, (const float *)a, crit, tau, jc,
-                                                  k, mode, exact ? 1 : 0);
+        k_fire<float><<<g, kT, 0, s>>>(m, R, seed, model_id, rbase, epoch, (const float *)a, crit, tau, jc, k, mode,
+                                       exact ? 1 : 0);
 }
 
 void launch_update(const DevModel &m, int64_t R, const int32_t *x, const int32_t *k, int32_t *x_out, uint8_t *neg,
@@ -964,6 +1142,39 @@ void launch_plain_commit(const DevModel &m, int64_t R, int32_t *x, const int32_t
                          double t_final, hipStream_t s) {
     dim3 g(blocks_for(R), row_grid(m.N));
     k_plain_commit<<<g, kT, 0, s>>>(m, R, x, x_new, st, t_final);
+}
+
+void launch_plain_grid(const DevModel &, int64_t R, const int32_t *x, const int32_t *x_new, PlainState st,
+                       double t_final, hipStream_t s) {
+    k_plain_grid<<<blocks_for(R), kT, 0, s>>>(R, x, x_new, st, t_final);
+}
+
+void launch_plain_grid_final(const DevModel &, int64_t R, const int32_t *x, PlainState st, double, hipStream_t s) {
+    k_plain_grid_final<<<blocks_for(R), kT, 0, s>>>(R, x, st);
+}
+
+void launch_plain_score(bool is64, const DevModel &m, int64_t R, int32_t nchunk, const void *a, const uint8_t *crit,
+                        const int32_t *k, const int32_t *argmin, const int32_t *which, double *qpart, PlainState st,
+                        bool exact, hipStream_t s) {
+    const dim3 g(blocks_for(R), (unsigned)nchunk);
+    if (is64) {
+        k_plain_score_leap<double><<<g, kT, 0, s>>>(m, R, nchunk, (const double *)a, crit, k, qpart, st, exact ? 1 : 0);
+        k_plain_score_fin<double><<<blocks_for(R), kT, 0, s>>>(m, R, nchunk, (const double *)a, crit, argmin, which,
+                                                               qpart, st);
+    } else {
+        k_plain_score_leap<float><<<g, kT, 0, s>>>(m, R, nchunk, (const float *)a, crit, k, qpart, st, exact ? 1 : 0);
+        k_plain_score_fin<float><<<blocks_for(R), kT, 0, s>>>(m, R, nchunk, (const float *)a, crit, argmin, which,
+                                                              qpart, st);
+    }
+}
+
+void launch_transpose_f64(int64_t R, int32_t M, const double *in, double *out, hipStream_t s) {
+    const dim3 g((unsigned)((R + 63) / 64), (unsigned)((M + 63) / 64));
+    k_transpose_f64<<<g, 256, 0, s>>>(R, M, in, out);
+}
+
+void launch_fill_f64(double *p, double v, int64_t n, hipStream_t s) {
+    if (n > 0) k_fill_f64<<<blocks_for(n), kT, 0, s>>>(p, v, n);
 }
 
 void launch_gather_track(int64_t R, int32_t n_track, const int32_t *track, const int32_t *x, int32_t *out,

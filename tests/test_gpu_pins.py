@@ -1,0 +1,108 @@
+"""GPU parity for leap rejection, per-realization time grids and the
+normal-branch score, through the C ABI, against the oracle and against the
+closed forms of tests/test_oracle_pins.py.
+
+  * rejections: toy networks at eps 0.3 and 0.6 reject leaps (negative
+    populations); every launch shape and the plain form, in FP64 and FP32,
+    reject the same attempts as the oracle, realization by realization, on
+    identical paths; the decay model's joint law of (rejections, state after
+    the first step) holds on the GPU;
+  * grids: each realization's grid equals the oracle's on identical paths,
+    and the one-molecule decay's grid is 1 - g/T1 exactly;
+  * normal-branch score: per-realization scores of one clamped leap at mean
+    500 are the exact discrete-law score to within the approximation's 0.03.
+"""
+import numpy as np
+import pytest
+
+import tau_leaping_amd as T
+from helpers import pair
+from tau_leaping_amd import models
+from test_oracle_pins import (REJ_EPS, REJ_TF, check_against_law, exact_normal_term, grid_closed_form,
+                              normal_branch_net, one_molecule_net, rejection_law, rejection_net)
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [("persistent", 64), ("persistent", 256), ("persistent", 1024), ("plain", 0)]
+
+
+@pytest.mark.parametrize("eps", [0.3, 0.6])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("mode,bs", SHAPES)
+def test_rejections_match_oracle(cuda, eps, prec, mode, bs):
+    net = models.toy_network(32, 64, seed=7)
+    m, om = pair(net)
+    R, tf = 512, 0.1
+    res = m.simulate(tf, R, seed=1234, eps=eps, precision=prec, mode=mode, block_size=bs)
+    ora = om.simulate(R, tf, seed=1234, eps=eps, precision=prec)
+    assert res.n_failed == 0 and (ora["status"] == 0).all()
+    assert res.form == mode
+    assert ora["rejections"].sum() > 50
+    assert res.total_rejections == int(res.rejections.sum()) > 50
+    same = np.all(res.final_x == ora["x_final"], axis=1) & (res.steps == ora["steps"])
+    assert same.mean() >= 0.97, same.mean()
+    assert np.array_equal(res.rejections[same], ora["rejections"][same])
+    # realizations that rejected are among the identical ones
+    assert (ora["rejections"][same] > 0).sum() > 20
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("mode,bs", SHAPES)
+def test_rejection_law_on_gpu(cuda, prec, mode, bs):
+    m, om = pair(rejection_net())
+    R = 100000
+    res = m.simulate(REJ_TF, R, seed=2024, eps=REJ_EPS, precision=prec, max_iter=1, mode=mode, block_size=bs)
+    assert (res.steps == 1).all()
+    rej, x = res.rejections, res.final_x[:, 0]
+    assert np.all(res.status[rej == 2] == 0) and np.all(res.status[rej < 2] == 8)
+    check_against_law(rej, x, rejection_law(), R)
+    ora = om.simulate(4000, REJ_TF, eps=REJ_EPS, seed=2024, precision=prec, max_iter=1)
+    assert np.mean(ora["x_final"][:, 0] == x[:4000]) >= 0.999
+    assert np.mean(ora["rejections"] == rej[:4000]) >= 0.999
+
+
+@pytest.mark.parametrize("mode,bs", SHAPES)
+def test_grids_match_oracle(cuda, mode, bs):
+    net = models.toy_network(24, 50, seed=4)
+    m, om = pair(net)
+    trk = np.array([0, 3, 7], np.int32)
+    R, n = 400, 5
+    res = m.simulate(0.1, R, seed=3, track=trk, n_intervals=n, return_grid=True, mode=mode, block_size=bs)
+    ora = om.simulate(R, 0.1, seed=3, track=trk, n_intervals=n)
+    assert res.n_failed == 0 and res.form == mode
+    same = np.all(res.final_x == ora["x_final"][:, trk], axis=1) & (res.steps == ora["steps"])
+    assert same.mean() >= 0.97
+    g, go = res.grid[same], ora["grid"][same]
+    # the same path: step times agree to the rounding of the tau bound's sums
+    np.testing.assert_allclose(g, go, rtol=1e-9, atol=1e-9)
+    assert np.mean(np.all(g == go, axis=(1, 2))) >= 0.5
+    np.testing.assert_allclose(res.grid_times, np.linspace(0, 0.1, n + 1), rtol=1e-15)
+    # the reported moments are those of the returned grids
+    np.testing.assert_allclose(res.mean, res.grid.mean(axis=0), rtol=1e-12)
+    np.testing.assert_allclose(res.stddev, res.grid.std(axis=0, ddof=1), rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("mode,bs", SHAPES)
+def test_grid_interior_closed_form_on_gpu(cuda, mode, bs):
+    c, tf, n, R, seed = 2.0, 1.0, 20, 3000, 99
+    m = T.Model.from_network(one_molecule_net(c))
+    res = m.simulate(tf, R, seed=seed, n_intervals=n, track=[0], return_grid=True, mode=mode, block_size=bs)
+    exact = grid_closed_form(seed, R, c, tf, n)
+    # the device logarithm may differ from the host's in the last bit
+    np.testing.assert_allclose(res.grid[:, :, 0], exact, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("bs", [64, 1024])
+def test_normal_branch_score_on_gpu(cuda, prec, bs):
+    c, tf, R = 1e4, 0.05, 20000
+    m, om = pair(normal_branch_net(c))
+    res = m.simulate(tf, R, seed=17, sensitivities=True, return_scores=True, precision=prec, block_size=bs)
+    assert (res.steps == 1).all() and res.n_failed == 0
+    k = res.final_x[:, 0].astype(np.int64) - 10 ** 6
+    term = res.score[:, 0] * c
+    assert np.abs(term - exact_normal_term(k, c * tf)).max() < 0.03
+    ora = om.simulate(R, tf, seed=17, sensitivities=True, precision=prec)
+    same = res.final_x[:, 0] == ora["x_final"][:, 0]
+    assert same.mean() >= 0.99
+    np.testing.assert_allclose(res.score[same, 0], ora["score"][same, 0], rtol=1e-12, atol=1e-15)

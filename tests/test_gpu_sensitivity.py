@@ -18,14 +18,20 @@ from tau_leaping_amd import models
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name,tf", [("dimer", 2.0), ("toy", 0.05), ("cyclic", 0.5)])
-@pytest.mark.parametrize("bs", [64, 1024])
-def test_scores_match_oracle(cuda, name, tf, bs):
+FORMS = [("persistent", 64), ("persistent", 1024), ("plain", 0)]
+
+
+@pytest.mark.parametrize("name,tf", [("dimer", 2.0), ("toy", 0.05), ("cyclic", 0.5), ("colloid", 0.05)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("mode,bs", FORMS)
+def test_scores_match_oracle(cuda, name, tf, prec, mode, bs):
     net = SMALL_NETS[name]()
     m, om = pair(net)
     R = 512
-    res = m.simulate(tf, R, seed=99, sensitivities=True, return_scores=True, block_size=bs)
-    ora = om.simulate(R, tf, seed=99, sensitivities=True)
+    res = m.simulate(tf, R, seed=99, sensitivities=True, return_scores=True, block_size=bs, mode=mode,
+                     precision=prec)
+    assert res.form == mode
+    ora = om.simulate(R, tf, seed=99, sensitivities=True, precision=prec)
     assert res.n_failed == 0 and (ora["status"] == 0).all()
     same = np.all(res.final_x == ora["x_final"], axis=1) & (res.steps == ora["steps"])
     assert same.mean() >= 0.97, same.mean()
@@ -37,8 +43,8 @@ def test_scores_match_oracle(cuda, name, tf, bs):
     assert np.quantile(err.max(axis=1), 0.99) < 1e-8, np.quantile(err.max(axis=1), 0.99)
 
 
-@pytest.mark.parametrize("bs", [64, 1024])
-def test_source_scores(cuda, bs):
+@pytest.mark.parametrize("mode,bs", FORMS)
+def test_source_scores(cuda, mode, bs):
     # a source's score term is 1/k per event minus the exposure t (a = k):
     # per-realization scores equal the oracle's, and dE/dk, dE/dg match the
     # closed form of the source with decay
@@ -47,7 +53,7 @@ def test_source_scores(cuda, bs):
     net = Network(1, [Reaction({}, {0: 1}, k), Reaction({0: 1}, {}, g)], np.array([s0], np.int32))
     m, om = pair(net)
     R = 16384
-    res = m.simulate(tf, R, seed=12, sensitivities=True, return_scores=True, block_size=bs)
+    res = m.simulate(tf, R, seed=12, sensitivities=True, return_scores=True, block_size=bs, mode=mode)
     ora = om.simulate(R, tf, seed=12, sensitivities=True)
     same = np.all(res.final_x == ora["x_final"], axis=1) & (res.steps == ora["steps"])
     assert same.mean() >= 0.97
@@ -73,12 +79,12 @@ def test_exact_sampler_scores(cuda):
     assert abs(res.dmean_dc[0, 0] - 0.05) < 4.5 * res.dmean_dc_se[0, 0]
 
 
-@pytest.mark.parametrize("bs", [64, 1024])
-def test_three_reactant_scores_match_oracle(cuda, bs):
+@pytest.mark.parametrize("mode,bs", FORMS)
+def test_three_reactant_scores_match_oracle(cuda, mode, bs):
     from helpers import trimolecular_net
     m, om = pair(trimolecular_net())
     R = 512
-    res = m.simulate(1.0, R, seed=31, sensitivities=True, return_scores=True, block_size=bs)
+    res = m.simulate(1.0, R, seed=31, sensitivities=True, return_scores=True, block_size=bs, mode=mode)
     ora = om.simulate(R, 1.0, seed=31, sensitivities=True)
     same = np.all(res.final_x == ora["x_final"], axis=1) & (res.steps == ora["steps"])
     assert same.mean() >= 0.97, same.mean()
@@ -127,3 +133,9 @@ def test_autograd_function(cuda):
     ref = m.simulate(sensitivities=True, **kw)
     assert np.allclose(y.detach().numpy(), ref.mean[-1])
     assert np.allclose(rates.grad.numpy(), w.numpy() @ ref.dmean_dc)
+    # the differentiable call simulates at the given rates and leaves the
+    # model's own rates as they were
+    np.testing.assert_array_equal(m.rates, rates.detach().numpy())
+    y2 = T.mean_final_state(rates.detach() * 2.0, m, **kw)
+    np.testing.assert_array_equal(m.rates, rates.detach().numpy())
+    assert not np.allclose(y2.numpy(), y.detach().numpy())

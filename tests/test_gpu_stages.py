@@ -135,3 +135,30 @@ def test_poisson_sampler_stream(cuda):
     k = _n(T.stage_fire(m, 5, _t(epoch, cuda), _t(a, cuda), _t(crit, cuda), _t(tau, cuda), _t(jc, cuda)))
     k_o = om.fire(5, epoch, a, crit, tau, jc)
     assert np.array_equal(k, k_o)
+
+
+def test_stage_waits_for_callers_stream(cuda):
+    # the input is produced by torch kernels queued just before the stage
+    # call (a long matrix product first, so they are still running when the
+    # stage is called); the stage must see the finished input
+    import torch
+
+    net = models_toy()
+    m, om = pair(net)
+    R = 4096
+    x = random_states(net.n_species, R, seed=5)
+    a_o, c_o, _, _ = om.propensities(x, "fp64")
+    big = torch.randn(4096, 4096, device=cuda)
+    for _ in range(3):
+        big = big @ big / 64.0
+    xs = torch.zeros((net.n_species, R), dtype=torch.int32, device=cuda)
+    xs += (big[0, 0] * 0).to(torch.int32)  # depends on the product
+    xs += torch.from_numpy(x).to(cuda, non_blocking=True)
+    a, c, _, _ = T.stage_propensities(m, xs, "fp64")
+    assert np.array_equal(_n(a), a_o) and np.array_equal(_n(c), c_o)
+
+
+def models_toy():
+    from tau_leaping_amd import models
+
+    return models.toy_network(32, 64, seed=7)
