@@ -179,7 +179,7 @@ def test_plain_form_one_pass_species_minimum(cuda, prec, R):
 def test_model_with_many_tiles_stores_a_in_hbm(cuda):
     # 6000 species x 300,000 channels: 19 propensity tiles, so the HBM-state
     # kernel keeps a[M] in global memory and sums the species rows in one
-    # pass; trajectories follow the oracle's
+    # pass; trajectories follow the oracle's (AUTO would run the plain form)
     import tl_oracle as O
     from helpers import toy_csc
 
@@ -188,7 +188,7 @@ def test_model_with_many_tiles_stores_a_in_hbm(cuda):
     om = O.OracleModel(6000, 300000, *csc)
     R, tf = 32, 1e-6
     track = np.arange(0, 6000, 100)
-    res = m.simulate(tf, R, seed=5, track=track)
+    res = m.simulate(tf, R, seed=5, track=track, mode="persistent")
     ora = om.simulate(R, tf, seed=5)
     assert res.block_size == 1024 and res.n_failed == 0 and (ora["status"] == 0).all()
     same = _compare(res.final_x, ora["x_final"][:, track], min_same=0.9)

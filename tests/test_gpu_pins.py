@@ -73,9 +73,12 @@ def test_grids_match_oracle(cuda, mode, bs):
     same = np.all(res.final_x == ora["x_final"][:, trk], axis=1) & (res.steps == ora["steps"])
     assert same.mean() >= 0.97
     g, go = res.grid[same], ora["grid"][same]
-    # the same path: step times agree to the rounding of the tau bound's sums
+    # the same path: step times agree to the rounding of the tau bound's
+    # species sums; the plain form sums them in the oracle's order, so its
+    # grids are the oracle's bit for bit
     np.testing.assert_allclose(g, go, rtol=1e-9, atol=1e-9)
-    assert np.mean(np.all(g == go, axis=(1, 2))) >= 0.5
+    if mode == "plain":
+        assert np.mean(np.all(g == go, axis=(1, 2))) >= 0.97
     np.testing.assert_allclose(res.grid_times, np.linspace(0, 0.1, n + 1), rtol=1e-15)
     # the reported moments are those of the returned grids
     np.testing.assert_allclose(res.mean, res.grid.mean(axis=0), rtol=1e-12)
