@@ -173,6 +173,24 @@ tl_status tl_simulate(const tl_model *m, const tl_sim_params *p, tl_result *r);
 tl_status tl_batch_simulate(const tl_model *const *models, int32_t n_models,
                             const tl_sim_params *params, tl_result *results, int32_t mode);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): tl_simulate's persistent form on one model, with the
+ * default launch shape, writing DEVICE outputs final_x [R][n_track], steps
+ * [R] and status [R]; it calls no hipMalloc / hipFree.  The kernel's other
+ * device memory (work item, counter, track list, per-realization event and
+ * rejection counts) is a 32 MiB array per GPU, held until the process ends:
+ * at most 2,097,000 realizations and 256 tracked species per call; no
+ * sensitivities, no time grid, no model whose state needs HBM (those return
+ * TL_ERR_INVALID_ARG / TL_ERR_TOO_LARGE).  The model's constants must
+ * already be on the device (any tl_simulate or tl_stage_* call with the
+ * model there uploads them; TL_ERR_INVALID_ARG otherwise).  stream (a
+ * hipStream_t, may be a capturing one): the work is enqueued on it and the
+ * call returns at once; NULL: the library's stream, and the call waits.
+ * Calls on one GPU share its array: overlapping calls race. */
+tl_status tl_simulate_noalloc(const tl_model *m, const tl_sim_params *p, int32_t *final_x, int64_t *steps,
+                              int32_t *status, void *stream);
+
 /* ---- stages on caller-provided DEVICE buffers (structure of arrays) ----
  * x [N][R] int32, a [M][R] (double or float per precision), crit [M][R]
  * uint8, per-realization scalars [R].  Synchronous with all device work: a

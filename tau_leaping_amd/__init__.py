@@ -94,6 +94,8 @@ _SIGS = {
     "tl_stage_update": ([C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int32),
     "tl_stage_ssa": ([C.c_void_p, C.c_int32, C.c_int64, C.c_uint64, C.c_uint32, C.c_int64, C.c_void_p,
                       C.c_void_p, C.c_void_p, C.c_double, C.c_int32, C.c_void_p], C.c_int32),
+    "tl_simulate_noalloc": ([C.c_void_p, C.POINTER(_Params), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+                            C.c_int32),
     "tl_last_error": ([], C.c_char_p),
     "tl_version": ([], C.c_char_p),
     "tl_device_count": ([C.POINTER(C.c_int32)], C.c_int32),

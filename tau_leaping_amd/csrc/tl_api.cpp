@@ -1062,6 +1062,40 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
 }
 
 }  // namespace
+
+// ------------------------------------------- helpers for other entry points --
+tl_status check_sim_params(const tl_model *m, const tl_sim_params *p) { return check_params(m, p); }
+
+bool sim_needs_ext(const tl_model *m, const tl_sim_params *p) { return needs_ext(m, p); }
+
+std::vector<int32_t> sim_track_list(const tl_model *m, const tl_sim_params *p) { return track_list(m, p); }
+
+tl_status default_shape(DeviceContext &dc, bool is64, bool sens, bool ext, int32_t n_max, int32_t m_max,
+                        PersistentConfig &cfg) {
+    const auto all = candidates(dc, is64, sens, n_max, m_max, ext);
+    if (all.empty()) return set_error(TL_ERR_TOO_LARGE, "model too large for any persistent launch shape");
+    cfg = heuristic(all, m_max);
+    return TL_OK;
+}
+
+PersistentLaunch launch_plan(DeviceContext &dc, const PersistentConfig &cfg, bool is64, bool sens, ModelItem *d_items,
+                             int32_t n_models, int64_t n_items, int32_t n_max, int32_t m_max,
+                             unsigned long long *d_counter) {
+    return make_launch(dc, cfg, is64, sens, d_items, n_models, n_items, n_max, m_max, d_counter, 0);
+}
+
+tl_status device_model_resident(const tl_model &m, int device, DevModel &out) {
+    std::lock_guard<std::mutex> g(m.dev_mu);
+    auto it = m.dev.find(device);
+    if (it == m.dev.end() || it->second->version != m.version)
+        return set_error(TL_ERR_INVALID_ARG,
+                         "the model's constants are not on device " + std::to_string(device) +
+                             " (or its rates / x0 changed since): one tl_simulate or tl_stage_* call with this model "
+                             "on the device uploads them");
+    out = it->second->dm;
+    return TL_OK;
+}
+
 }  // namespace tl
 
 using namespace tl;

@@ -177,6 +177,20 @@ size_t persistent_state_bytes(const PersistentConfig &c, bool is64, int32_t n_ma
 // Max resident blocks per CU for the config (0 if unsupported).
 int persistent_occupancy(const PersistentConfig &c, bool is64, bool sens, size_t lds_bytes);
 hipError_t launch_persistent(const PersistentConfig &c, bool is64, bool sens, const PersistentLaunch &L, hipStream_t s);
+
+// Host helpers shared by the entry points (tl_api.cpp).
+tl_status check_sim_params(const tl_model *m, const tl_sim_params *p);
+bool sim_needs_ext(const tl_model *m, const tl_sim_params *p);
+std::vector<int32_t> sim_track_list(const tl_model *m, const tl_sim_params *p);
+// The default persistent launch shape of a size class (a pure function of
+// the sizes, precision and kernel build); TL_ERR_TOO_LARGE if none fits.
+tl_status default_shape(DeviceContext &dc, bool is64, bool sens, bool ext, int32_t n_max, int32_t m_max,
+                        PersistentConfig &cfg);
+PersistentLaunch launch_plan(DeviceContext &dc, const PersistentConfig &cfg, bool is64, bool sens, ModelItem *d_items,
+                             int32_t n_models, int64_t n_items, int32_t n_max, int32_t m_max,
+                             unsigned long long *d_counter);
+// The model's device constants if they are resident and current (no upload).
+tl_status device_model_resident(const tl_model &m, int device, DevModel &out);
 void launch_grad_reduce(int64_t R, int32_t n_track, int32_t M, const int32_t *x_track, const double *score,
                         double *mean_scratch, double *dmean, double *dmean_se, hipStream_t s);
 
