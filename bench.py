@@ -41,6 +41,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "leaps/s (realizations×steps) batched vs plain at fixed CI; grad/fwd cost ratio"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_persistent_toy128x256_fp64_traffic.json")
+ISSUE_FILE = os.path.join(ROOT, "profiles", "k_persistent_toy128x256_fp64_issue.json")
 
 
 def log(msg):
@@ -65,6 +66,22 @@ def measured_traffic(workload):
     return float(t["fetch_kib"]) * 1024 * float(t["fetch_scale"]) + float(t["write_kib"]) * 1024 * float(t["write_scale"])
 
 
+def measured_issue(kernel_name):
+    """VALU issue and LDS bandwidth of the kernel from the committed counter
+    summary (SQ counters of one dispatch of the same kernel and model at
+    fewer realizations), if it covers this kernel."""
+    try:
+        with open(ISSUE_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel") != kernel_name:
+        return None
+    return {"valu_busy": t["valu_busy"], "valu_active_lanes": t["valu_active_lanes"], "lds_TBs": t["lds_tbs"],
+            "lds_peak_TBs": t["lds_peak_tbs"], "lds_frac": t["lds_frac"], "counter_workload": t["workload"],
+            "summary": t["summary"]}
+
+
 def cpu_baseline(net, R_sample, t_final, seed):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import tl_oracle
@@ -74,13 +91,94 @@ def cpu_baseline(net, R_sample, t_final, seed):
     out = om.simulate(R_sample, t_final, seed=seed, n_threads=0)
     dt = time.perf_counter() - t0
     steps = int(out["steps"].sum())
+    # one core on a smaller sample (the first realizations of the same run)
+    R1 = max(1, R_sample // 20)
+    t0 = time.perf_counter()
+    one = om.simulate(R1, t_final, seed=seed, n_threads=1)
+    dt1 = time.perf_counter() - t0
+    steps1 = int(one["steps"].sum())
     return dict(value=steps / dt, unit="realization-leaps/s", cores=int(out["threads"]), kind="port",
                 sample=f"{R_sample} realizations of the same workload (realization ids 0..{R_sample - 1}), "
-                       f"{steps} realization-steps in {dt:.1f} s on {out['threads']} threads")
+                       f"{steps} realization-steps in {dt:.1f} s on {out['threads']} threads",
+                single_core={"value": steps1 / dt1, "unit": "realization-leaps/s", "cores": 1,
+                             "sample": f"realization ids 0..{R1 - 1}, {steps1} realization-steps in {dt1:.1f} s"})
 
 
 def rate(res):
     return res.total_steps / (res.kernel_ms * 1e-3) if res.kernel_ms > 0 else float("nan")
+
+
+def hetero_run(T, models, np, tf=0.1):
+    """The 64-model heterogeneous batch: FP32, FP64 time, scores on for every
+    channel, per-model seeds 1234 + m and model ids m.  The full
+    configuration (all 161,120 realizations) in one persistent launch, then
+    the batched launch and the plain per-model loop on the same subset (every
+    4th model at 1/16 of its realizations, at least 16; same seeds, scores on
+    in both): the plain loop takes minutes per model at full size."""
+    nets, reals = models.hetero_batch(64)
+    ms = [T.Model.from_network(n) for n in nets]
+    kws = [dict(t_final=tf, n_realizations=r, seed=1234 + i, model_id=i, precision="fp32", sensitivities=True,
+                track=np.arange(4, dtype=np.int32)) for i, r in enumerate(reals)]
+    T.batch_simulate(ms, [dict(k, n_realizations=64) for k in kws], want_final=False)  # uploads, warm caches
+    t0 = time.perf_counter()
+    hb = T.batch_simulate(ms, kws, want_final=False)
+    dtb = time.perf_counter() - t0
+    steps_b = sum(r.total_steps for r in hb)
+    out = {"models": 64, "realizations": int(sum(reals)), "t_final": tf, "precision": "fp32", "sensitivities": True,
+           "species_range": [min(n.n_species for n in nets), max(n.n_species for n in nets)],
+           "channels_range": [min(n.n_channels for n in nets), max(n.n_channels for n in nets)],
+           "leaps": int(steps_b), "ssa_events": int(sum(r.total_ssa_events for r in hb)),
+           "persistent_leaps_per_s": steps_b / dtb, "persistent_seconds": dtb,
+           "persistent_kernel_ms": hb[0].kernel_ms, "failed": int(sum(r.n_failed for r in hb))}
+    pick = list(range(0, 64, 4))
+    sub = [dict(kws[i], n_realizations=max(16, reals[i] // 16)) for i in pick]
+    sub_ms = [ms[i] for i in pick]
+    log("hetero matched subset: batched")
+    t0 = time.perf_counter()
+    sb = T.batch_simulate(sub_ms, sub, mode="persistent", want_final=False)
+    dsb = time.perf_counter() - t0
+    log("hetero matched subset: plain loop")
+    t0 = time.perf_counter()
+    sp = T.batch_simulate(sub_ms, sub, mode="plain", want_final=False)
+    dsp = time.perf_counter() - t0
+    lb, lp = sum(r.total_steps for r in sb), sum(r.total_steps for r in sp)
+    out["matched_subset"] = {
+        "models": pick, "realizations": int(sum(k["n_realizations"] for k in sub)), "sensitivities": True,
+        "batched_leaps": int(lb), "plain_leaps": int(lp), "batched_seconds": dsb, "plain_seconds": dsp,
+        "batched_leaps_per_s": lb / dsb, "plain_leaps_per_s": lp / dsp, "batched_over_plain": (lb / dsb) / (lp / dsp),
+        "note": "same models, realizations, seeds and score setting in both forms; wall time of each call"}
+    return out
+
+
+def cyclic_lr_vs_fd(T, models, np, R=262144, n_fd=8):
+    """Likelihood-ratio dE[X_k]/dc_j of the 1024-species cyclic chain against
+    central finite differences (relative step 1e-3, common random numbers) on
+    the n_fd channels feeding each of the 4 tracked species, both over R
+    realizations (at 16,384 the finite differences alone carry ~10 % noise)."""
+    net = models.cyclic_chain(1024, seed=43)
+    m = T.Model.from_network(net)
+    track = np.array([0, 256, 512, 768], np.int32)
+    res = m.simulate(1.0, R, seed=1234, track=track, sensitivities=True)
+    c = np.ones(1024)
+    lr, lrse, fd, fdse = [], [], [], []
+    for q, k in enumerate(track):
+        for d in range(n_fd):
+            j = (int(k) - d) % 1024
+            outs = []
+            for sgn in (+1, -1):
+                cc = c.copy()
+                cc[j] += sgn * 1e-3
+                m.rates = cc
+                outs.append(m.simulate(1.0, R, seed=1234, track=[int(k)]).final_x[:, 0].astype(float))
+            diff = (outs[0] - outs[1]) / 2e-3
+            fd.append(diff.mean())
+            fdse.append(diff.std(ddof=1) / np.sqrt(R))
+            lr.append(res.dmean_dc[q, j])
+            lrse.append(res.dmean_dc_se[q, j])
+    lr, lrse, fd, fdse = map(np.array, (lr, lrse, fd, fdse))
+    return {"realizations": R, "entries": int(fd.size), "rel_err": float(np.linalg.norm(lr - fd) / np.linalg.norm(fd)),
+            "rel_noise": float(np.linalg.norm(np.hypot(lrse, fdse)) / np.linalg.norm(fd)), "bar": 0.05,
+            "max_abs_z": float((np.abs(lr - fd) / np.hypot(lrse, fdse)).max())}
 
 
 def extras_run(T, models, np, m_toy, kw_toy):
@@ -117,9 +215,8 @@ def extras_run(T, models, np, m_toy, kw_toy):
     ex["cyclic1024_grad"] = {
         "realizations": 16384, "fwd_seconds": fwd, "grad_seconds": grd, "grad_fwd_ratio": grd / fwd,
         "fwd_kernel_ms": min(r.kernel_ms for r in f), "grad_kernel_ms": min(r.kernel_ms for r in g),
-        "fwd_leaps_per_s": f[0].total_steps / fwd,
-        "dmean_dc_max_abs": float(np.abs(g[0].dmean_dc).max()),
-        "dmean_dc_median_rel_se": float(np.median(g[0].dmean_dc_se / (np.abs(g[0].dmean_dc) + 1e-300)))}
+        "fwd_leaps_per_s": f[0].total_steps / fwd, "grad_fwd_bar": 1.6,
+        "lr_vs_fd": cyclic_lr_vs_fd(T, models, np)}
 
     # colloid, 250 species, 31250 channels, 4096 realizations, t_final 0.05
     log("colloid")
@@ -142,35 +239,8 @@ def extras_run(T, models, np, m_toy, kw_toy):
 
     # heterogeneous batch: 64 toy-recipe models, FP32, sensitivities on; a
     # quarter of each model's realizations (the full batch takes minutes)
-    log("hetero batch")
-    nets, reals = models.hetero_batch(64)
-    ms = [T.Model.from_network(n) for n in nets]
-    kws = [dict(t_final=tf, n_realizations=max(64, r // 4), seed=1234 + i, model_id=i, precision="fp32",
-                sensitivities=True, track=np.arange(4, dtype=np.int32)) for i, r in enumerate(reals)]
-    T.batch_simulate(ms, [dict(k, n_realizations=max(64, k["n_realizations"] // 16)) for k in kws],
-                     want_final=False)
-    t0 = time.perf_counter()
-    hb = T.batch_simulate(ms, kws, want_final=False)
-    dtb = time.perf_counter() - t0
-    steps_b = sum(r.total_steps for r in hb)
-    # plain per-model loop on a sample: every 8th model, 1/16 of its
-    # realizations here (the plain form runs without sensitivities)
-    pick = list(range(0, 64, 8))
-    sub = [dict(kws[i], n_realizations=max(4, kws[i]["n_realizations"] // 16), sensitivities=False) for i in pick]
-    log("hetero plain sample")
-    t0 = time.perf_counter()
-    hp = T.batch_simulate([ms[i] for i in pick], sub, mode="plain", want_final=False)
-    dtp = time.perf_counter() - t0
-    steps_p = sum(r.total_steps for r in hp)
-    ex["hetero64"] = {
-        "models": 64, "realizations": int(sum(k["n_realizations"] for k in kws)),
-        "realizations_full_config": int(sum(reals)), "species_range": [min(n.n_species for n in nets),
-                                                                       max(n.n_species for n in nets)],
-        "persistent_leaps_per_s": steps_b / dtb, "persistent_seconds": dtb,
-        "persistent_kernel_ms": hb[0].kernel_ms, "persistent_sensitivities": True,
-        "plain_models": pick, "plain_realizations": int(sum(k["n_realizations"] for k in sub)),
-        "plain_leaps_per_s": steps_p / dtp,
-        "plain_sensitivities": False, "batched_over_plain": (steps_b / dtb) / (steps_p / dtp)}
+    log("hetero batch (full configuration)")
+    ex["hetero64"] = hetero_run(T, models, np)
 
     # large-model ingestion: toy_system_gen at 1e6 channels over 2e5 species
     # (about 500 MB of StochKit XML), the streaming reader, and the
@@ -313,7 +383,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(workload),
                      "kernel": "plain stage kernels" if plain_form else "k_persistent", "kernel_ms": avg_ms,
-                     "bytes_per_leap": B},
+                     "bytes_per_leap": B,
+                     "issue": measured_issue("k_persistent<double, 64, 64, true, false, 6, false, false>")
+                     if (args.workload == "toy" and fp64 and res[-1].block_size == 64) else None},
     }
     if not args.no_extras and args.workload == "toy":
         line["extras"] = extras_run(T, models, np, m, dict(n_realizations=R, kw=kw))

@@ -18,6 +18,8 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <type_traits>
+#include <cstdlib>
 #include <vector>
 
 #include "tl_internal.hpp"
@@ -717,53 +719,107 @@ PersistentLaunch make_launch(DeviceContext &dc, const PersistentConfig &cfg, boo
     return L;
 }
 
-// One launch per size class; with use_graph the counter resets and launches
-// of every class are captured into one hipGraph and replayed once.  The
-// classes run back to back on the library's stream (each fills the GPU with
-// its own work items).
+// One persistent launch per size class.  With use_graph the counter resets
+// and launches of every class are captured into one hipGraph: the classes
+// are independent branches on up to four streams (the library's and three
+// more, all default priority), so a class's launch fills the CUs that an
+// earlier class's tail leaves idle; the first class (the most expensive) is
+// dispatched first.  The executable graph is instantiated once per batch
+// shape (class configs, grids, LDS) and updated in place with the new
+// kernel arguments on later calls, then replayed.  TL_GRAPH=0 in the
+// environment runs the same launches on the library's stream without a
+// graph (for comparison).
 struct ClassLaunch {
     PersistentConfig cfg;
     PersistentLaunch L;
 };
+
+std::string graph_key(bool is64, bool sens, const std::vector<ClassLaunch> &cl) {
+    std::string k = std::to_string(is64) + "/" + std::to_string(sens);
+    for (const auto &c : cl)
+        k += "|" + std::to_string(c.cfg.block) + "," + std::to_string(c.cfg.group) + "," + std::to_string(c.cfg.a_lds) +
+             "," + std::to_string(c.cfg.wpe) + "," + std::to_string(c.cfg.tile_grains) + "," +
+             std::to_string(c.cfg.hbm) + "," + std::to_string(c.cfg.ext) + "," + std::to_string(c.L.grid) + "," +
+             std::to_string(c.L.lds_bytes);
+    return k;
+}
+
+bool graphs_enabled() {
+    const char *v = getenv("TL_GRAPH");
+    return !(v && v[0] == '0');
+}
+
 tl_status run_launches(DeviceContext &dc, bool is64, bool sens, const std::vector<ClassLaunch> &cl, bool use_graph,
                        float *ms_out) {
     hipStream_t s = dc.stream;
-    hipEvent_t e0, e1;
-    TL_HIP(hipEventCreate(&e0));
-    TL_HIP(hipEventCreate(&e1));
+    Events ev;
+    TL_TRY(ev.create());
+    use_graph = use_graph && graphs_enabled();
     if (use_graph) {
+        std::lock_guard<std::mutex> g(dc.graph_mu);
+        const int nb = std::min<int>((int)cl.size(), 1 + DeviceContext::kAux);
+        for (int b = 1; b < nb; ++b)
+            if (!dc.aux[b - 1]) {
+                TL_HIP(hipStreamCreateWithFlags(&dc.aux[b - 1], hipStreamNonBlocking));
+                TL_HIP(hipEventCreateWithFlags(&dc.join[b - 1], hipEventDisableTiming));
+            }
+        if (!dc.fork) TL_HIP(hipEventCreateWithFlags(&dc.fork, hipEventDisableTiming));
         hipGraph_t gph;
-        hipGraphExec_t gx;
         TL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         hipError_t le = hipSuccess;
-        for (const auto &c : cl) {
-            launch_fill_u64(c.L.counter, 0ull, 1, s);
-            const hipError_t e = launch_persistent(c.cfg, is64, sens, c.L, s);
+        auto keep = [&](hipError_t e) {
             if (le == hipSuccess) le = e;
+        };
+        keep(hipEventRecord(dc.fork, s));
+        for (int b = 1; b < nb; ++b) keep(hipStreamWaitEvent(dc.aux[b - 1], dc.fork, 0));
+        for (size_t i = 0; i < cl.size(); ++i) {
+            const int b = (int)(i % nb);
+            hipStream_t sb = b == 0 ? s : dc.aux[b - 1];
+            launch_fill_u64(cl[i].L.counter, 0ull, 1, sb);
+            keep(launch_persistent(cl[i].cfg, is64, sens, cl[i].L, sb));
+        }
+        for (int b = 1; b < nb; ++b) {
+            keep(hipEventRecord(dc.join[b - 1], dc.aux[b - 1]));
+            keep(hipStreamWaitEvent(s, dc.join[b - 1], 0));
         }
         TL_HIP(hipStreamEndCapture(s, &gph));
+        std::unique_ptr<std::remove_pointer<hipGraph_t>::type, hipError_t (*)(hipGraph_t)> gguard(gph,
+                                                                                                 hipGraphDestroy);
         TL_HIP(le);
-        TL_HIP(hipGraphInstantiate(&gx, gph, nullptr, nullptr, 0));
-        TL_HIP(hipEventRecord(e0, s));
+        const std::string key = graph_key(is64, sens, cl);
+        auto it = dc.graphs.find(key);
+        hipGraphExec_t gx = nullptr;
+        if (it != dc.graphs.end()) {
+            hipGraphNode_t err_node = nullptr;
+            hipGraphExecUpdateResult ur;
+            if (hipGraphExecUpdate(it->second, gph, &err_node, &ur) == hipSuccess && ur == hipGraphExecUpdateSuccess) {
+                gx = it->second;
+            } else {
+                (void)hipGetLastError();
+                hipGraphExecDestroy(it->second);
+                dc.graphs.erase(it);
+            }
+        }
+        if (!gx) {
+            TL_HIP(hipGraphInstantiate(&gx, gph, nullptr, nullptr, 0));
+            dc.graphs[key] = gx;
+        }
+        TL_HIP(hipEventRecord(ev.e0, s));
         TL_HIP(hipGraphLaunch(gx, s));
-        TL_HIP(hipEventRecord(e1, s));
+        TL_HIP(hipEventRecord(ev.e1, s));
         TL_HIP(hipStreamSynchronize(s));
-        hipGraphExecDestroy(gx);
-        hipGraphDestroy(gph);
     } else {
-        TL_HIP(hipEventRecord(e0, s));
+        TL_HIP(hipEventRecord(ev.e0, s));
         for (const auto &c : cl) {
             launch_fill_u64(c.L.counter, 0ull, 1, s);
             TL_HIP(launch_persistent(c.cfg, is64, sens, c.L, s));
         }
-        TL_HIP(hipEventRecord(e1, s));
+        TL_HIP(hipEventRecord(ev.e1, s));
         TL_HIP(hipStreamSynchronize(s));
     }
     TL_HIP(hipGetLastError());
     float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    hipEventElapsedTime(&ms, ev.e0, ev.e1);
     if (ms_out) *ms_out = ms;
     return TL_OK;
 }
@@ -928,6 +984,7 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
         classes[cls].push_back(i);
     }
     std::vector<ClassLaunch> launches;
+    std::vector<double> class_cost;
     std::vector<PersistentConfig> entry_cfg(n_models);
     for (auto &kv : classes) {
         std::vector<int32_t> &members = kv.second;
@@ -1006,11 +1063,22 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
         c.L = make_launch(*dc, cfg, is64, sens, d_items, nm, n_items, n_max, m_max, d_counter, 0);
         if (cfg.hbm) TL_TRY(B.alloc(c.L.state, (size_t)c.L.grid * c.L.state_stride));
         launches.push_back(c);
+        double cc = 0.0;
+        for (int32_t k = 0; k < nm; ++k) cc += E[members[k]].cost;
+        class_cost.push_back(cc);
         for (int32_t k = 0; k < nm; ++k) entry_cfg[members[k]] = cfg;
     }
 
     const double t0 = now_s();
     float ms = 0.f;
+    {  // the most expensive class is dispatched first
+        std::vector<size_t> ord(launches.size());
+        for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return class_cost[a] > class_cost[b]; });
+        std::vector<ClassLaunch> sorted;
+        for (size_t i : ord) sorted.push_back(launches[i]);
+        launches.swap(sorted);
+    }
     TL_TRY(run_launches(*dc, is64, sens, launches, true, &ms));
     for (auto &e : E)
         if (sens && e.res && (e.res->dmean_dc || e.res->dmean_dc_se))

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -37,12 +39,19 @@ struct DeviceModelCache {
 // Upload (or reuse) the model's constants on the current device.
 tl_status device_model(const tl_model &m, int device, DevModel &out, double *upload_seconds);
 
-// Per-device context: one stream (default priority) and reusable scratch.
+// Per-device context: the library's stream (default priority), up to three
+// more for the branches of a batch graph, and the executable batch graphs
+// cached by batch shape.
 struct DeviceContext {
+    static constexpr int kAux = 3;
     int device = -1;
     hipStream_t stream = nullptr;
     int n_cu = 0;
     size_t lds_per_block = 0;
+    hipStream_t aux[kAux] = {};
+    hipEvent_t fork = nullptr, join[kAux] = {};
+    std::mutex graph_mu;
+    std::map<std::string, hipGraphExec_t> graphs;
 };
 tl_status device_context(int device, DeviceContext *&ctx);
 

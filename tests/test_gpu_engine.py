@@ -341,3 +341,36 @@ def test_status_codes(cuda):
     om = oracle_of(net)
     ora = om.simulate(64, 0.1, seed=1, max_iter=3)
     assert (ora["status"] == 8).all()
+
+
+def test_batch_graph_replays(cuda, monkeypatch):
+    # a batch of several size classes runs as one graph whose branches (one
+    # per class, up to four streams) run concurrently; the executable graph
+    # is kept per batch shape and replayed with new arguments: every replay,
+    # with other device work and synchronizations between them, equals the
+    # same launches run without a graph
+    import torch
+
+    nets = [models.toy_network(12, 30, seed=301), models.toy_network(400, 1400, seed=302),
+            models.toy_network(60, 150, seed=303), models.colloidal(30, seed=304), models.toy_network(150, 500, seed=305)]
+    ms = [T.Model.from_network(n) for n in nets]
+    kws = [dict(t_final=0.02, n_realizations=300 + 50 * i, seed=900 + i, model_id=i) for i in range(len(nets))]
+    monkeypatch.setenv("TL_GRAPH", "0")
+    eager = T.batch_simulate(ms, kws, mode="persistent")
+    monkeypatch.delenv("TL_GRAPH")
+    for rep in range(3):
+        junk = torch.randn(2048, 2048, device=cuda)
+        (junk @ junk).sum().item()
+        torch.cuda.synchronize()
+        got = T.batch_simulate(ms, kws, mode="persistent")
+        for e, g in zip(eager, got):
+            assert g.n_failed == 0
+            assert np.array_equal(e.final_x, g.final_x), rep
+            assert np.array_equal(e.steps, g.steps), rep
+    # another realization count (new arguments, same shape): still the eager result
+    kws2 = [dict(k, n_realizations=k["n_realizations"] // 2) for k in kws]
+    got = T.batch_simulate(ms, kws2, mode="persistent")
+    monkeypatch.setenv("TL_GRAPH", "0")
+    eager2 = T.batch_simulate(ms, kws2, mode="persistent")
+    for e, g in zip(eager2, got):
+        assert np.array_equal(e.final_x, g.final_x) and np.array_equal(e.steps, g.steps)

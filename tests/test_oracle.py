@@ -541,3 +541,16 @@ def test_iteration_cap_status():
     om = oracle_of(models.toy_network(16, 32, seed=2))
     res = om.simulate(20, 0.1, seed=1, max_iter=3)
     assert (res["status"] == 8).all() and (res["steps"] == 3).all()
+
+
+def test_ring_jacobian_quadrature_matches_frechet():
+    # the quadrature closed form used at the cyclic chain's full size agrees
+    # with the Frechet derivative of exp(TL) on a small ring (wrap-around
+    # included)
+    from helpers import cyclic_mean_jacobian, cyclic_ring_jacobian
+
+    N, T = 12, 1.3
+    x0 = np.random.default_rng(3).integers(100, 900, N)
+    ref = cyclic_mean_jacobian(np.ones(N), x0, T)
+    got = cyclic_ring_jacobian(x0, T, list(range(N)))
+    np.testing.assert_allclose(got, ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
