@@ -58,10 +58,13 @@ enum {
 
 enum { TL_FP64 = 0, TL_FP32 = 1 };
 
-/* Firing counts of non-critical channels with mean >= 80: the reference's
- * truncated normal approximation |N(m, m)| (model.cu:496-520), or an exact
- * Poisson sample (PTRS). */
-enum { TL_POISSON_NORMAL = 0, TL_POISSON_EXACT = 1 };
+/* Firing counts of non-critical channels (model.cu:477-520).  NORMAL
+ * (default): the reference's sampler, the Knuth product below mean 80 and the
+ * truncated normal |N(m, m)| from 80.  EXACT: PTRS (Hoermann 1993, an exact
+ * Poisson sample) from 80 instead.  FAST: PTRS from mean 10 to 80 instead of
+ * the Knuth product (the same Poisson law from a few uniforms instead of
+ * mean + 1), |N(m, m)| from 80.  FAST_EXACT: PTRS from mean 10. */
+enum { TL_POISSON_NORMAL = 0, TL_POISSON_EXACT = 1, TL_POISSON_FAST = 2, TL_POISSON_FAST_EXACT = 3 };
 
 /* Engine forms: PLAIN = per-model host loop of stage kernels over
  * structure-of-arrays state in HBM; PERSISTENT = one work-stealing kernel that
@@ -126,8 +129,7 @@ typedef struct tl_sim_params {
                                    TL_BLOCK_TUNE: time the shapes on a sample and keep the fastest */
     uint32_t model_id;          /* RNG stream id of the model within a batch */
     int64_t realization_offset; /* first realization id */
-    int32_t poisson;            /* TL_POISSON_NORMAL (the reference's approximation above mean 80) or
-                                   TL_POISSON_EXACT (PTRS, Hoermann 1993, above mean 80) */
+    int32_t poisson;            /* TL_POISSON_* sampler (see the enum) */
 } tl_sim_params;
 
 void tl_sim_params_default(tl_sim_params *p);

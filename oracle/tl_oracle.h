@@ -51,8 +51,10 @@
  *     are rounded once to the propensity type (ref: FP32 sums); v * a_j is
  *     exact in FP64, so the rounded sums do not depend on the order of the
  *     additions to within the last FP64 bit.
- *   - option: an exact Poisson sampler (PTRS, Hoermann 1993) in place of the
- *     normal approximation at mean >= 80 (tlo_params.poisson_exact).
+ *   - options: an exact Poisson sampler (PTRS, Hoermann 1993) in place of
+ *     the normal approximation at mean >= 80 (TLO_POISSON_EXACT), and in
+ *     place of the Knuth product for 10 <= mean < 80 (TLO_POISSON_FAST: the
+ *     same law from a few uniforms instead of mean + 1).
  *   - the first Poisson draw of channel j at a step is output (j & 3) of the
  *     Philox block shared by channels 4*(j>>2)..4*(j>>2)+3; later draws come
  *     from the channel's own counter stream (see tlo_head_draw).
@@ -85,6 +87,11 @@ enum {
 #define TLO_POISSON_SWITCH 80
 #define TLO_KNUTH_CAP 300
 #define TLO_MAX_RETRIES 32
+#define TLO_PTRS_FROM 10 /* PTRS from this mean (the method's range) with TLO_POISSON_FAST */
+
+/* sampler flags (tlo_params.poisson) */
+#define TLO_POISSON_EXACT 1 /* PTRS above mean 80 instead of the normal approximation */
+#define TLO_POISSON_FAST 2  /* PTRS from mean 10 instead of the Knuth product up to 80 */
 
 /* RNG stream kinds (ctr[3] low two bits) */
 #define TLO_KIND_POISSON 0u
@@ -176,9 +183,9 @@ void tlo_poisson_ptrs_many(double mean, uint64_t seed, uint32_t model_id, int64_
 
 /* Single Poisson draw from the channel stream (test hook for the sampler). */
 int32_t tlo_poisson_f64(double mean, uint64_t seed, uint32_t model_id, int64_t rid,
-                        uint32_t epoch, int32_t channel);
+                        uint32_t epoch, int32_t channel, int32_t flags);
 int32_t tlo_poisson_f32(float mean, uint64_t seed, uint32_t model_id, int64_t rid,
-                        uint32_t epoch, int32_t channel);
+                        uint32_t epoch, int32_t channel, int32_t flags);
 
 /* ---- full simulation ---- */
 typedef struct tlo_params {
@@ -193,7 +200,7 @@ typedef struct tlo_params {
     int32_t n_track;
     const int32_t *track;  /* [n_track] species indices */
     int32_t n_threads;     /* 0: all online cores */
-    int32_t poisson_exact; /* 1: PTRS instead of the normal approximation at mean >= 80 */
+    int32_t poisson;       /* TLO_POISSON_EXACT | TLO_POISSON_FAST */
 } tlo_params;
 
 typedef struct tlo_outputs {

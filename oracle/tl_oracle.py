@@ -42,12 +42,13 @@ class _Model(C.Structure):
 
 
 def _poisson(p):
-    """Sampler above mean 80: 'normal' (the reference's approximation) or 'exact' (PTRS)."""
-    if p in ("normal", 0):
-        return 0
-    if p in ("exact", "ptrs", 1):
-        return 1
-    raise ValueError(f"unknown Poisson sampler {p!r}")
+    """Sampler flags: 'normal' (the reference's: Knuth below mean 80, the
+    normal approximation from 80), 'exact' (PTRS from 80), 'fast' (PTRS from
+    10 to 80), 'fast-exact' (PTRS from 10)."""
+    v = {"normal": 0, "exact": 1, "ptrs": 1, "fast": 2, "fast-exact": 3}.get(p, p)
+    if v not in (0, 1, 2, 3):
+        raise ValueError(f"unknown Poisson sampler {p!r}")
+    return v
 
 
 class _Params(C.Structure):
@@ -55,7 +56,7 @@ class _Params(C.Structure):
         ("t_final", C.c_double), ("eps", C.c_double), ("seed", C.c_uint64),
         ("model_id", C.c_uint32), ("precision", C.c_int32), ("max_iter", C.c_int64),
         ("sensitivities", C.c_int32), ("n_intervals", C.c_int32), ("n_track", C.c_int32),
-        ("track", C.POINTER(C.c_int32)), ("n_threads", C.c_int32), ("poisson_exact", C.c_int32),
+        ("track", C.POINTER(C.c_int32)), ("n_threads", C.c_int32), ("poisson", C.c_int32),
     ]
 
 
@@ -92,9 +93,9 @@ def lib():
                                                      U32P, RP, U8P, F64P, I32P, I32P]
             getattr(L, "tlo_ssa_" + suf).argtypes = [C.POINTER(_Model), C.c_int64, C.c_uint64, C.c_uint32, C.c_int64,
                                                     I32P, F64P, U32P, C.c_double, C.c_int32, I32P]
-        L.tlo_poisson_f64.argtypes = [C.c_double, C.c_uint64, C.c_uint32, C.c_int64, C.c_uint32, C.c_int32]
+        L.tlo_poisson_f64.argtypes = [C.c_double, C.c_uint64, C.c_uint32, C.c_int64, C.c_uint32, C.c_int32, C.c_int32]
         L.tlo_poisson_f64.restype = C.c_int32
-        L.tlo_poisson_f32.argtypes = [C.c_float, C.c_uint64, C.c_uint32, C.c_int64, C.c_uint32, C.c_int32]
+        L.tlo_poisson_f32.argtypes = [C.c_float, C.c_uint64, C.c_uint32, C.c_int64, C.c_uint32, C.c_int32, C.c_int32]
         L.tlo_poisson_f32.restype = C.c_int32
         L.tlo_poisson_ptrs.argtypes = [C.c_double, C.c_uint64, C.c_uint32, C.c_int64, C.c_uint32, C.c_int32]
         L.tlo_poisson_ptrs.restype = C.c_int32
@@ -244,6 +245,6 @@ class OracleModel:
                                     out)
         return out
 
-    def poisson(self, mean, seed, rid, epoch, channel, precision="fp64", model_id=0):
+    def poisson(self, mean, seed, rid, epoch, channel, precision="fp64", model_id=0, poisson="normal"):
         f = lib().tlo_poisson_f64 if _rtype(precision) == np.float64 else lib().tlo_poisson_f32
-        return f(mean, int(seed), int(model_id), int(rid), int(epoch), int(channel))
+        return f(mean, int(seed), int(model_id), int(rid), int(epoch), int(channel), _poisson(poisson))

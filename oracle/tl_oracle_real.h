@@ -134,14 +134,18 @@ static double FN(tau_one)(const tlo_model *m, REAL eps, const int32_t *x, ptrdif
     return (double)best;
 }
 
-/* Knuth product (mean < 80, cap 300) or |polar Box-Muller| truncated
- * (model.cu:477-520, functions.h:37-54); with `exact`, PTRS (tlo_ptrs)
- * instead of the normal approximation at mean >= 80.  Draw 1 of the channel is `head`
- * (see head_draw); draws 2, 3, ... come from the channel's own stream, output
- * ((n-2) & 3) of Philox block (n-2) >> 2 at counter (block, channel, epoch,
- * ctr3). */
+/* Firing count of a non-critical channel (model.cu:477-520, functions.h:37-54),
+ * flags TLO_POISSON_EXACT / TLO_POISSON_FAST:
+ *   mean < 80 (< 10 with FAST): the Knuth product, lam = 1/exp(mean), cap 300;
+ *   10 <= mean < 80 with FAST, and above 80 with EXACT: PTRS (tlo_ptrs), an
+ *     exact Poisson sample from a few uniforms (the Knuth product needs
+ *     mean + 1);
+ *   otherwise: |polar Box-Muller(mean, var = mean)| truncated.
+ * Draw 1 of the channel is `head` (see head_draw); draws 2, 3, ... come from
+ * the channel's own stream, output ((n-2) & 3) of Philox block (n-2) >> 2 at
+ * counter (block, channel, epoch, ctr3). */
 static int32_t FN(poisson_stream)(REAL mean, uint32_t head, const uint32_t key[2], uint32_t channel,
-                                  uint32_t epoch, uint32_t ctr3, int exact) {
+                                  uint32_t epoch, uint32_t ctr3, int flags) {
     uint32_t blk[4];
     uint32_t ctr[4] = {0u, channel, epoch, ctr3};
     int have = 0, pos = 0, first = 1;
@@ -160,7 +164,8 @@ static int32_t FN(poisson_stream)(REAL mean, uint32_t head, const uint32_t key[2
         }                                             \
         (dst) = FN(unif)(blk[pos++]);                 \
     } while (0)
-    if (mean < (REAL)TLO_POISSON_SWITCH) {
+    const REAL knuth_below = (flags & TLO_POISSON_FAST) ? (REAL)TLO_PTRS_FROM : (REAL)TLO_POISSON_SWITCH;
+    if (mean < knuth_below) {
         REAL lam = (REAL)1 / R_EXP(mean);
         REAL val = 1;
         for (int32_t n = 1; n < TLO_KNUTH_CAP; ++n) {
@@ -170,7 +175,7 @@ static int32_t FN(poisson_stream)(REAL mean, uint32_t head, const uint32_t key[2
             if (val < lam) return n - 1;
         }
         return TLO_KNUTH_CAP;
-    } else if (exact) {
+    } else if (mean < (REAL)TLO_POISSON_SWITCH || (flags & TLO_POISSON_EXACT)) {
         return tlo_ptrs((double)mean, head, key, channel, epoch, ctr3);
     } else {
         REAL s, u, v;
@@ -190,11 +195,11 @@ static int32_t FN(poisson_stream)(REAL mean, uint32_t head, const uint32_t key[2
 }
 
 int32_t FN(tlo_poisson)(REAL mean, uint64_t seed, uint32_t model_id, int64_t rid, uint32_t epoch,
-                        int32_t channel) {
+                        int32_t channel, int32_t flags) {
     uint32_t key[2];
     tlo_key(seed, rid, key);
     uint32_t head = tlo_head_draw(key, model_id << 2, epoch, channel);
-    return FN(poisson_stream)(mean, head, key, (uint32_t)channel, epoch, (model_id << 2) | TLO_KIND_POISSON, 0);
+    return FN(poisson_stream)(mean, head, key, (uint32_t)channel, epoch, (model_id << 2) | TLO_KIND_POISSON, flags);
 }
 
 /* Critical pick (GetReaction2Fire, model.cu:555-598): upper_bound over the
@@ -236,7 +241,7 @@ static void FN(select_one)(const tlo_model *m, const uint32_t key[2], uint32_t c
 /* k_j for all channels (NumberOfFiringFn, model.cu:470-474). */
 static void FN(fire_one)(const tlo_model *m, const uint32_t key[2], uint32_t ctr3, uint32_t epoch,
                          const REAL *a, const uint8_t *crit, ptrdiff_t as, double tau, int32_t jc,
-                         int32_t *k, ptrdiff_t ks, int exact) {
+                         int32_t *k, ptrdiff_t ks, int flags) {
     REAL tr = (REAL)tau;
     for (int32_t j = 0; j < m->M; ++j) {
         int32_t kj;
@@ -244,7 +249,7 @@ static void FN(fire_one)(const tlo_model *m, const uint32_t key[2], uint32_t ctr
             kj = (j == jc) ? 1 : 0;
         else
             kj = FN(poisson_stream)(a[(ptrdiff_t)j * as] * tr, tlo_head_draw(key, ctr3, epoch, j), key, (uint32_t)j,
-                                    epoch, ctr3 | TLO_KIND_POISSON, exact);
+                                    epoch, ctr3 | TLO_KIND_POISSON, flags);
         k[(ptrdiff_t)j * ks] = kj;
     }
 }
@@ -496,7 +501,7 @@ static void FN(simulate_one)(const tlo_model *m, const tlo_params *p, int64_t ri
             int32_t jc;
             uint8_t clamped;
             FN(select_one)(m, key, ctr3, epoch, w->a, w->crit, 1, a0c, tau1, tf - t, &tau, &jc, &clamped);
-            FN(fire_one)(m, key, ctr3, epoch, w->a, w->crit, 1, tau, jc, w->k, 1, p->poisson_exact);
+            FN(fire_one)(m, key, ctr3, epoch, w->a, w->crit, 1, tau, jc, w->k, 1, p->poisson);
             epoch++;
             int32_t *xn = w->xn;
             memcpy(xn, x, sizeof(int32_t) * (size_t)N);
@@ -518,7 +523,7 @@ static void FN(simulate_one)(const tlo_model *m, const tlo_params *p, int64_t ri
             }
             if (score)
                 FN(score_leap)(m, w->a, w->crit, w->k, tau, (jc < 0 && !clamped), clamped, istar, which, score,
-                               w->dtime, p->poisson_exact);
+                               w->dtime, (p->poisson & TLO_POISSON_EXACT) != 0);
             /* swap: x <- xn */
             double tn = clamped ? tf : t + tau;
             if (!(tn > t)) {

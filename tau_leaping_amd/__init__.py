@@ -25,11 +25,11 @@ from . import models  # noqa: F401  (reaction networks and generators)
 __all__ = ["Model", "SimResult", "simulate", "batch_simulate", "mean_final_state", "lib", "TLError",
            "stage_propensities", "stage_tau", "stage_select", "stage_fire", "stage_update", "stage_ssa",
            "FP64", "FP32", "MODE_AUTO", "MODE_PLAIN", "MODE_PERSISTENT", "POISSON_NORMAL", "POISSON_EXACT",
-           "models"]
+           "POISSON_FAST", "POISSON_FAST_EXACT", "models"]
 
 FP64, FP32 = 0, 1
 MODE_AUTO, MODE_PLAIN, MODE_PERSISTENT = 0, 1, 2
-POISSON_NORMAL, POISSON_EXACT = 0, 1
+POISSON_NORMAL, POISSON_EXACT, POISSON_FAST, POISSON_FAST_EXACT = 0, 1, 2, 3
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtau_leaping_amd.so")
@@ -136,10 +136,13 @@ def _prec(p):
 
 
 def _poisson(p):
-    """Firing counts above mean 80: 'normal' (the reference's |N(m, m)|) or
-    'exact' (PTRS)."""
-    v = {"normal": POISSON_NORMAL, "exact": POISSON_EXACT}.get(p, p)
-    if v not in (POISSON_NORMAL, POISSON_EXACT):
+    """Firing-count sampler: 'normal' (default, the reference's: the Knuth
+    product below mean 80, |N(m, m)| from 80), 'exact' (PTRS from 80), 'fast'
+    (PTRS from 10 to 80: the same law from fewer uniforms), 'fast-exact'
+    (PTRS from 10)."""
+    v = {"normal": POISSON_NORMAL, "exact": POISSON_EXACT, "fast": POISSON_FAST,
+         "fast-exact": POISSON_FAST_EXACT}.get(p, p)
+    if v not in (POISSON_NORMAL, POISSON_EXACT, POISSON_FAST, POISSON_FAST_EXACT):
         raise ValueError(f"unknown Poisson sampler {p!r}")
     return v
 

@@ -7,8 +7,8 @@
 //   -i <logging intervals, 1>
 // and writes mean_<log>.txt / stddev_<log>.txt with lines "t v1 v2 ..."
 // (StatToFile, statistics.cpp:89-111).  Extensions: --precision fp64|fp32,
-// --seed N, --poisson normal|exact (exact: PTRS instead of the normal
-// approximation above mean 80), --mode auto|persistent|plain (auto: persistent,
+// --seed N, --poisson normal|exact|fast|fast-exact (the firing-count
+// sampler, see TL_POISSON_* in tl_api.h), --mode auto|persistent|plain (auto: persistent,
 // plain where it runs a model too large for LDS faster), --block-size N, --sensitivities (writes
 // dmean_<log>.txt: one line per tracked species, dE[X(t_final)]/dc_j),
 // --json (one JSON record of the run on stdout).
@@ -60,7 +60,7 @@ struct Args {
     std::cerr << "usage: tau_leaping -f model.xml -t t_end [-n realizations] [-r names...|all] [-e eps]\n"
                  "                   [-a max_iterations] [-d device] [-b log_name] [-i intervals]\n"
                  "                   [--precision fp64|fp32] [--seed N] [--mode auto|persistent|plain]\n"
-                 "                   [--poisson normal|exact] [--block-size N] [--sensitivities] [--json]\n";
+                 "                   [--poisson normal|exact|fast|fast-exact] [--block-size N] [--sensitivities] [--json]\n";
     std::exit(EXIT_FAILURE);
 }
 
@@ -111,7 +111,9 @@ Args parse(int argc, char **argv) {
             const std::string v = need(argc, argv, a);
             if (v == "normal") A.poisson = TL_POISSON_NORMAL;
             else if (v == "exact") A.poisson = TL_POISSON_EXACT;
-            else usage("poisson must be normal or exact");
+            else if (v == "fast") A.poisson = TL_POISSON_FAST;
+            else if (v == "fast-exact") A.poisson = TL_POISSON_FAST_EXACT;
+            else usage("poisson must be normal, exact, fast or fast-exact");
         } else if (k == "--block-size") A.block_size = atoi(need(argc, argv, a));
         else if (k == "--sensitivities") A.sens = true;
         else if (k == "--json") A.json = true;

@@ -292,8 +292,8 @@ tl_status check_params(const tl_model *m, const tl_sim_params *p) {
         if (p->track[q] < 0 || p->track[q] >= m->N) return set_error(TL_ERR_INVALID_ARG, "tracked species out of range");
     if (p->n_intervals < 0) return set_error(TL_ERR_INVALID_ARG, "n_intervals must be >= 0");
     if (p->model_id >= (1u << 30)) return set_error(TL_ERR_INVALID_ARG, "model_id must be < 2^30");
-    if (p->poisson != TL_POISSON_NORMAL && p->poisson != TL_POISSON_EXACT)
-        return set_error(TL_ERR_INVALID_ARG, "poisson must be TL_POISSON_NORMAL or TL_POISSON_EXACT");
+    if (p->poisson < TL_POISSON_NORMAL || p->poisson > TL_POISSON_FAST_EXACT)
+        return set_error(TL_ERR_INVALID_ARG, "poisson must be one of TL_POISSON_NORMAL, _EXACT, _FAST, _FAST_EXACT");
     return TL_OK;
 }
 
@@ -301,7 +301,7 @@ tl_status check_params(const tl_model *m, const tl_sim_params *p) {
 // the exact sampler compiled in (s2 is present exactly when the model has a
 // source or a three-reactant channel).
 bool needs_ext(const tl_model *m, const tl_sim_params *p) {
-    return !m->s2.empty() || p->poisson == TL_POISSON_EXACT;
+    return !m->s2.empty() || p->poisson != TL_POISSON_NORMAL;
 }
 
 std::vector<int32_t> track_list(const tl_model *m, const tl_sim_params *p) {
@@ -496,7 +496,8 @@ tl_status run_plain(const tl_model *m, const tl_sim_params *p, tl_result *res) {
     std::unique_ptr<unsigned long long, hipError_t (*)(void *)> done_guard(h_done, hipHostFree);
     *h_done = 0;
     const uint64_t seed = p->seed;
-    const bool exact = p->poisson == TL_POISSON_EXACT;
+    const int pflags = p->poisson;
+    const bool exact = (pflags & kPoissonExact) != 0;
     for (int64_t it = 0;; ++it) {
         launch_prop(is64, dm, R, nchunk, x, a, crit, a0, a0c, part, partc, st.active, s);
         launch_tau(is64, dm, R, p->eps, x, a, crit, tau_i, which_i, tau1, argmin, which, st.active, s, nrc, pbest,
@@ -504,7 +505,7 @@ tl_status run_plain(const tl_model *m, const tl_sim_params *p, tl_result *res) {
         launch_plain_select(is64, dm, R, seed, p->model_id, p->realization_offset, p->t_final, p->max_iter, x, a, crit,
                             a0, a0c, tau1, st, nchunk, partc, s);
         launch_fire(is64, dm, R, seed, p->model_id, p->realization_offset, st.epoch, a, crit, st.tau, st.jc, k, st.mode,
-                    exact, s);
+                    pflags, s);
         launch_update(dm, R, x, k, xn, st.neg, st.mode, s);
         if (grid) launch_plain_grid(dm, R, x, xn, st, p->t_final, s);
         launch_plain_commit(dm, R, x, xn, st, p->t_final, s);
@@ -627,7 +628,8 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
                     lds = persistent_lds_bytes(c, is64, n_max, m_max);
                 }
                 if (lds > dc.lds_per_block) continue;
-                if (persistent_occupancy(c, is64, sens, lds) <= 0) continue;
+                c.occ = persistent_occupancy(c, is64, sens, lds);
+                if (c.occ <= 0) continue;
                 out.push_back(c);
             }
         }
@@ -665,25 +667,28 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
 }
 
 // Default shape without timing: one wavefront per realization up to 1024
-// channels, then 256-lane blocks, and 1024 lanes from 8192 channels;
-// propensities in LDS when they fit.  Measured: a 128 x 256 toy network runs
-// 3x faster on one wavefront than on blocks; a 960 x 3800 one 2x faster on
-// 256-lane blocks than on one wavefront (2.6x on 512-lane blocks two per CU,
-// but those lose on batches of such models with many direct-method events);
-// the 250-species colloid (31,250 channels) 1.8x faster on 1024-lane blocks
-// than on 512.
+// channels while the LDS (or registers) still hold at least 8 such groups per
+// CU, then 256-lane blocks, and 1024 lanes from 8192 channels; propensities
+// in LDS when they fit.  Measured: a 128 x 256 toy network runs 5x faster on
+// one wavefront than on 256-lane blocks, a 300 x 900 one 1.4x; the
+// 1024-species cyclic chain (37 KB of LDS per realization: 4 one-wavefront
+// groups per CU) 1.9x faster on 256-lane blocks; a 960 x 3000 toy network 1.8x
+// faster on 256-lane blocks than on one wavefront (2.6x on 512-lane blocks two
+// per CU, but those lose on batches of such models with many direct-method
+// events); the 250-species colloid (31,250 channels) 1.8x faster on 1024-lane
+// blocks than on 512.
 PersistentConfig heuristic(const std::vector<PersistentConfig> &cands, int32_t m_max) {
-    auto pick = [&](int b, int g) -> const PersistentConfig * {
-        for (auto &c : cands)
-            if (c.block == b && c.group == g) return &c;  // preferred variants come first
+    auto pick = [&](int b, int g, int min_waves) -> const PersistentConfig * {
+        for (auto &c : cands)  // preferred variants come first
+            if (c.block == b && c.group == g && c.occ * (b / 64) >= min_waves) return &c;
         return nullptr;
     };
     const PersistentConfig *c = nullptr;
-    if (m_max <= 1024) c = pick(64, 64);
-    if (!c && m_max <= 1024) c = pick(256, 64);
-    if (!c && m_max <= 8192) c = pick(256, 256);
-    if (!c) c = pick(1024, 1024);
-    if (!c) c = pick(512, 512);
+    if (m_max <= 1024) c = pick(64, 64, 8);
+    if (!c && m_max <= 1024) c = pick(256, 64, 8);
+    if (!c && m_max <= 8192) c = pick(256, 256, 0);
+    if (!c) c = pick(1024, 1024, 0);
+    if (!c) c = pick(512, 512, 0);
     if (!c && !cands.empty()) c = &cands.front();
     return c ? *c : PersistentConfig{0, 0, false, 1};
 }
@@ -1022,7 +1027,7 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
             mi.tlast = e.d_tlast;
             mi.n_intervals = e.p->n_intervals;
             mi.grid = e.d_grid;
-            mi.poisson_exact = e.p->poisson == TL_POISSON_EXACT ? 1 : 0;
+            mi.poisson = e.p->poisson;
             base += mi.R;
         }
         const int64_t n_items = base;
@@ -1336,7 +1341,7 @@ tl_status tl_stage_fire(const tl_model *m, int32_t precision, int64_t R, uint64_
     DevModel dm;
     TL_TRY(stage_setup(m, precision, R, dc, dm));
     if (!epoch || !a || !crit || !tau || !jc || !k) return set_error(TL_ERR_INVALID_ARG, "null buffer");
-    launch_fire(precision == TL_FP64, dm, R, seed, model_id, realization_offset, epoch, a, crit, tau, jc, k, nullptr, false,
+    launch_fire(precision == TL_FP64, dm, R, seed, model_id, realization_offset, epoch, a, crit, tau, jc, k, nullptr, 0,
                 dc->stream);
     return stage_finish(dc);
 }

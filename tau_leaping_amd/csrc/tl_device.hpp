@@ -236,67 +236,94 @@ __device__ __forceinline__ int32_t poisson_ptrs(double mean, uint32_t head, uint
     return (int32_t)mean;
 }
 
-// Firing count for a non-critical channel: Knuth product below 80 (capped at
-// 300), else |mean + sqrt(mean) * z| truncated, z from the polar method, or
-// with `exact` the PTRS sample above.
-// Draw 1 of the channel is `head`; draw n >= 2 is output ((n-2) & 3) of the
-// channel's own Philox block (n-2) >> 2 at counter (block, channel, epoch,
-// ctr3).  The Knuth loop returns 0 on its first draw without evaluating exp
-// when head < 1 - mean - margin, a bound below the rounded 1/exp(mean) (the
-// result is the one the full loop returns).
+// Sampler flags (tl_sim_params.poisson): bit 0 = PTRS above mean 80 instead
+// of the reference's normal approximation; bit 1 = PTRS from mean 10 instead
+// of the reference's Knuth product up to 80 (same law, a few uniforms instead
+// of mean + 1).
+constexpr int kPoissonExact = 1;
+constexpr int kPoissonFast = 2;
+constexpr float kPtrsFrom = 10.f;  // PTRS from this mean (the method's range) with kPoissonFast
+
+// Mean below which the Knuth product draws a non-critical channel's count.
 template <class R>
-__device__ __forceinline__ int32_t poisson_draw(R mean, uint32_t head, uint2 key, uint32_t channel,
-                                                uint32_t epoch, uint32_t ctr3, bool exact) {
+__device__ __forceinline__ R knuth_below(bool fast) {
+    return fast ? (R)kPtrsFrom : (R)kPoissonSwitch;
+}
+
+// The Knuth product (mean below knuth_below), capped at 300.  Draw 1 of the
+// channel is `head`; draw n >= 2 is output ((n-2) & 3) of the channel's own
+// Philox block (n-2) >> 2 at counter (block, channel, epoch, ctr3).  The loop
+// returns 0 on its first draw without evaluating exp when head < 1 - mean -
+// margin, a bound below the rounded 1/exp(mean) (the result is the one the
+// full loop returns).
+template <class R>
+__device__ __forceinline__ int32_t poisson_light(R mean, uint32_t head, uint2 key, uint32_t channel,
+                                                 uint32_t epoch, uint32_t ctr3) {
     uint32_t blk = 0;
     const R u1 = Real<R>::unif(head);
-    if (mean < (R)kPoissonSwitch) {
-        const R margin = sizeof(R) == 8 ? (R)9.094947017729282e-13 : (R)9.5367431640625e-07;  // 2^-40 / 2^-20
-        if (mean < (R)1 && u1 < (R)1 - mean - margin) return 0;
-        const R lam = (R)1 / Real<R>::exp_(mean);
-        R val = u1;
-        if (val < lam) return 0;
-        int32_t n = 2;
-        for (;;) {
-            const uint4 o = philox4x32_10(make_uint4(blk++, channel, epoch, ctr3), key);
-            val *= Real<R>::unif(o.x);
-            if (val < lam) return n - 1;
-            if (++n == kKnuthCap) return kKnuthCap;
-            val *= Real<R>::unif(o.y);
-            if (val < lam) return n - 1;
-            if (++n == kKnuthCap) return kKnuthCap;
-            val *= Real<R>::unif(o.z);
-            if (val < lam) return n - 1;
-            if (++n == kKnuthCap) return kKnuthCap;
-            val *= Real<R>::unif(o.w);
-            if (val < lam) return n - 1;
-            if (++n == kKnuthCap) return kKnuthCap;
-        }
-    } else if (exact) {
-        return poisson_ptrs((double)mean, head, key, channel, epoch, ctr3);
-    } else {
-        // pairs (head, t0), (t1, t2), (t3, t4), ... of the channel's draws
-        R u = 2 * u1 - 1;
-        uint4 o = philox4x32_10(make_uint4(blk++, channel, epoch, ctr3), key);
-        R v = 2 * Real<R>::unif(o.x) - 1;
-        R s = u * u + v * v;
-        if (s >= 1) {
-            for (;;) {
-                u = 2 * Real<R>::unif(o.y) - 1;
-                v = 2 * Real<R>::unif(o.z) - 1;
-                s = u * u + v * v;
-                if (s < 1) break;
-                const uint32_t w = o.w;
-                o = philox4x32_10(make_uint4(blk++, channel, epoch, ctr3), key);
-                u = 2 * Real<R>::unif(w) - 1;
-                v = 2 * Real<R>::unif(o.x) - 1;
-                s = u * u + v * v;
-                if (s < 1) break;
-            }
-        }
-        const R z = u * Real<R>::sqrt_(-2 * Real<R>::log_(s) / s);
-        const R y = mean + Real<R>::sqrt_(mean) * z;
-        return (int32_t)Real<R>::fabs_(y);
+    const R margin = sizeof(R) == 8 ? (R)9.094947017729282e-13 : (R)9.5367431640625e-07;  // 2^-40 / 2^-20
+    if (mean < (R)1 && u1 < (R)1 - mean - margin) return 0;
+    const R lam = (R)1 / Real<R>::exp_(mean);
+    R val = u1;
+    if (val < lam) return 0;
+    int32_t n = 2;
+    for (;;) {
+        const uint4 o = philox4x32_10(make_uint4(blk++, channel, epoch, ctr3), key);
+        val *= Real<R>::unif(o.x);
+        if (val < lam) return n - 1;
+        if (++n == kKnuthCap) return kKnuthCap;
+        val *= Real<R>::unif(o.y);
+        if (val < lam) return n - 1;
+        if (++n == kKnuthCap) return kKnuthCap;
+        val *= Real<R>::unif(o.z);
+        if (val < lam) return n - 1;
+        if (++n == kKnuthCap) return kKnuthCap;
+        val *= Real<R>::unif(o.w);
+        if (val < lam) return n - 1;
+        if (++n == kKnuthCap) return kKnuthCap;
     }
+}
+
+// The draw of a channel whose mean is at least knuth_below: PTRS below 80
+// (with `fast`; and with `exact` above 80), else |mean + sqrt(mean) * z|
+// truncated, z from the polar method on the pairs (head, t0), (t1, t2),
+// (t3, t4), ... of the channel's draws.
+template <class R>
+__device__ __forceinline__ int32_t poisson_heavy(R mean, uint32_t head, uint2 key, uint32_t channel, uint32_t epoch,
+                                                 uint32_t ctr3, bool exact) {
+    if (mean < (R)kPoissonSwitch || exact) return poisson_ptrs((double)mean, head, key, channel, epoch, ctr3);
+    uint32_t blk = 0;
+    R u = 2 * Real<R>::unif(head) - 1;
+    uint4 o = philox4x32_10(make_uint4(blk++, channel, epoch, ctr3), key);
+    R v = 2 * Real<R>::unif(o.x) - 1;
+    R s = u * u + v * v;
+    if (s >= 1) {
+        for (;;) {
+            u = 2 * Real<R>::unif(o.y) - 1;
+            v = 2 * Real<R>::unif(o.z) - 1;
+            s = u * u + v * v;
+            if (s < 1) break;
+            const uint32_t w = o.w;
+            o = philox4x32_10(make_uint4(blk++, channel, epoch, ctr3), key);
+            u = 2 * Real<R>::unif(w) - 1;
+            v = 2 * Real<R>::unif(o.x) - 1;
+            s = u * u + v * v;
+            if (s < 1) break;
+        }
+    }
+    const R z = u * Real<R>::sqrt_(-2 * Real<R>::log_(s) / s);
+    const R y = mean + Real<R>::sqrt_(mean) * z;
+    return (int32_t)Real<R>::fabs_(y);
+}
+
+// Firing count of a non-critical channel: the Knuth product below 80 (below
+// 10 with `fast`, PTRS from there to 80), then the reference's normal
+// approximation, or PTRS with `exact`.
+template <class R>
+__device__ __forceinline__ int32_t poisson_draw(R mean, uint32_t head, uint2 key, uint32_t channel,
+                                                uint32_t epoch, uint32_t ctr3, bool exact, bool fast = false) {
+    if (mean < knuth_below<R>(fast)) return poisson_light<R>(mean, head, key, channel, epoch, ctr3);
+    return poisson_heavy<R>(mean, head, key, channel, epoch, ctr3, exact);
 }
 
 // Leap / direct-event scalars r1 (pick) and r2 (waiting time), FP64.

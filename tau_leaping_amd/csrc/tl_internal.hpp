@@ -78,7 +78,7 @@ void launch_select(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint3
                    hipStream_t s);
 void launch_fire(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase,
                  const uint32_t *epoch, const void *a, const uint8_t *crit, const double *tau, const int32_t *jc,
-                 int32_t *k, const uint8_t *mode, bool exact, hipStream_t s);
+                 int32_t *k, const uint8_t *mode, int pflags, hipStream_t s);
 void launch_update(const DevModel &m, int64_t R, const int32_t *x, const int32_t *k, int32_t *x_out, uint8_t *neg,
                    const uint8_t *mode, hipStream_t s);
 void launch_ssa(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase,
@@ -152,7 +152,7 @@ struct ModelItem {
     double *tlast;         // [R][M] time up to which S_j holds its direct-event exposure (sensitivities) or null
     int32_t n_intervals;   // > 0: record tracked species on the grid k*t_final/n_intervals
     double *grid;          // [R][n_intervals+1][n_track]
-    int32_t poisson_exact; // PTRS instead of the normal approximation at mean >= 80
+    int32_t poisson;       // sampler flags kPoissonExact | kPoissonKnuth (tl_sim_params.poisson)
 };
 
 struct PersistentConfig {
@@ -163,6 +163,7 @@ struct PersistentConfig {
     int tile_grains = 1;  // block groups without a in LDS: propensity tile of tile_grains * 4096 channels
     bool hbm = false;     // group state in global memory (models whose state does not fit LDS)
     bool ext = false;     // kernels with three-reactant channels and the exact sampler compiled in
+    int occ = 0;          // resident blocks per CU (filled by the candidate list)
 };
 
 struct PersistentLaunch {

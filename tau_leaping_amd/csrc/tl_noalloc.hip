@@ -126,7 +126,7 @@ extern "C" tl_status tl_simulate_noalloc(const tl_model *m, const tl_sim_params 
     mi.out_ssa = d_ssa;
     mi.out_rej = d_rej;
     mi.out_status = status;
-    mi.poisson_exact = p->poisson == TL_POISSON_EXACT ? 1 : 0;
+    mi.poisson = p->poisson;
     TrackList tlist;
     tlist.n = (int32_t)trk.size();
     for (int32_t q = 0; q < tlist.n; ++q) tlist.idx[q] = trk[q];

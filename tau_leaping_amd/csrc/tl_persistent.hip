@@ -911,11 +911,14 @@ __device__ __forceinline__ void prop_update(const GModel &m, gptr<const R> c, co
 // caller synchronizes before reading what f wrote.
 // CRIT_A: f needs a_j of critical channels too (else they get 0 and a
 // recomputed propensity is not evaluated for them).
-// EXACT: the exact sampler above mean 80 (a separate instance, so that the
-// default pass carries none of its registers).
-template <class R, int GS, bool CRIT_A, bool EXACT, class P, class F>
+// pflags: kPoissonExact / kPoissonFast, honoured in the EXT build only (the
+// default build carries none of PTRS's registers).
+template <class R, int GS, bool CRIT_A, bool EXT, class P, class F>
 __device__ __forceinline__ void for_each_firing(const GModel &m, const Grp<GS> &G, const P &pa, const uint64_t *critq,
-                                                uint2 key, uint32_t epoch, uint32_t c3, R tr, int32_t jfire, F &&f) {
+                                                uint2 key, uint32_t epoch, uint32_t c3, R tr, int32_t jfire,
+                                                int pflags, F &&f) {
+    const bool exact = EXT && (pflags & kPoissonExact) != 0;
+    const bool fast = EXT && (pflags & kPoissonFast) != 0;
     const uint32_t ck = c3 | kKindPoisson;
     for (int32_t c0 = 0; c0 < m.M; c0 += 4 * GS) {
         const int32_t jb = c0 + 4 * G.lane;
@@ -941,7 +944,7 @@ __device__ __forceinline__ void for_each_firing(const GModel &m, const Grp<GS> &
                     aj = i == 0 ? a4[0] : (i == 1 ? a4[1] : (i == 2 ? a4[2] : a4[3]));
                 const int32_t kj =
                     cr ? (j == jfire ? 1 : 0)
-                       : poisson_draw<R>(aj * tr, quad_pick(h, i), key, (uint32_t)j, epoch, ck, EXACT);
+                       : poisson_draw<R>(aj * tr, quad_pick(h, i), key, (uint32_t)j, epoch, ck, exact, fast);
                 f(j, cr, aj, kj);
             }
         }
@@ -993,7 +996,8 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
     const uint32_t c3 = mi.model_id << 2;
     const double tf = mi.t_final;
     const R eps = (R)mi.eps;
-    const bool exact = EXT && mi.poisson_exact != 0;
+    const int pflags = EXT ? mi.poisson : 0;
+    const bool exact = (pflags & kPoissonExact) != 0;
     // Likelihood-ratio score S_j = dlog p(path)/dc_j, and D_j = dt/dc_j of the
     // current time (nonzero through tau1-limited leaps; used by the clamped
     // last leap and the truncated last direct interval).
@@ -1219,10 +1223,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                                            if (c[j] != (R)0) S[j] += term / (double)c[j];
                                        }
                                    };
-            if (exact)
-                for_each_firing<R, GS, SENS, true>(m, G, pa, critq, key, epoch, c3, tr, jfire, fire_one);
-            else
-                for_each_firing<R, GS, SENS, false>(m, G, pa, critq, key, epoch, c3, tr, jfire, fire_one);
+            for_each_firing<R, GS, SENS, EXT>(m, G, pa, critq, key, epoch, c3, tr, jfire, pflags, fire_one);
             sync_();  // dx complete
             bool neg = false;
             for (int32_t i = lane; i < m.N; i += GS) neg |= (xs[i] + dx[i]) < 0;
@@ -1233,10 +1234,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                     auto undo = [&](int32_t j, bool cr, R aj, int32_t kj) {
                         if (c[j] != (R)0) S[j] -= score_term<R>(cr, aj, tr, tau, kj, exact) / (double)c[j];
                     };
-                    if (exact)
-                        for_each_firing<R, GS, SENS, true>(m, G, pa, critq, key, epoch, c3, tr, jfire, undo);
-                    else
-                        for_each_firing<R, GS, SENS, false>(m, G, pa, critq, key, epoch, c3, tr, jfire, undo);
+                    for_each_firing<R, GS, SENS, EXT>(m, G, pa, critq, key, epoch, c3, tr, jfire, pflags, undo);
                 }
                 epoch++;
                 nrej++;

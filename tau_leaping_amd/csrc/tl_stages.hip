@@ -487,7 +487,7 @@ __global__ __launch_bounds__(kT, 5) void k_fire(DevModel m, int64_t Rn, uint64_t
                                                 const uint32_t *__restrict__ epoch, const R *__restrict__ a,
                                                 const uint8_t *__restrict__ crit, const double *__restrict__ tau,
                                                 const int32_t *__restrict__ jc, int32_t *__restrict__ k,
-                                                const uint8_t *__restrict__ mode, int exact) {
+                                                const uint8_t *__restrict__ mode, int pflags) {
     // a thread takes channels 4q..4q+3 of one realization: their first draws
     // share one Philox block (head_block), computed once while the channels'
     // propensities and criticality flags load.  blockIdx.x = realization
@@ -517,7 +517,8 @@ __global__ __launch_bounds__(kT, 5) void k_fire(DevModel m, int64_t Rn, uint64_t
         const int32_t j = 4 * q + i;
         k[(int64_t)j * Rn + r] = cv[i] ? ((jcr == j) ? 1 : 0)
                                        : poisson_draw<R>(av[i] * tr, quad_pick(h, i), key, (uint32_t)j, ep,
-                                                         (model_id << 2) | kKindPoisson, exact != 0);
+                                                         (model_id << 2) | kKindPoisson,
+                                                         (pflags & kPoissonExact) != 0, (pflags & kPoissonFast) != 0);
     }
 }
 
@@ -1083,15 +1084,15 @@ void launch_select(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint3
 
 void launch_fire(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase,
                  const uint32_t *epoch, const void *a, const uint8_t *crit, const double *tau, const int32_t *jc,
-                 int32_t *k, const uint8_t *mode, bool exact, hipStream_t s) {
+                 int32_t *k, const uint8_t *mode, int pflags, hipStream_t s) {
     const int32_t nquad = (m.M + 3) / 4;
     const dim3 g(blocks_for(R), row_grid(nquad), (unsigned)((nquad + 65534) / 65535));
     if (is64)
         k_fire<double><<<g, kT, 0, s>>>(m, R, seed, model_id, rbase, epoch, (const double *)a, crit, tau, jc, k, mode,
-                                        exact ? 1 : 0);
+                                        pflags);
     else
         k_fire<float><<<g, kT, 0, s>>>(m, R, seed, model_id, rbase, epoch, (const float *)a, crit, tau, jc, k, mode,
-                                       exact ? 1 : 0);
+                                       pflags);
 }
 
 void launch_update(const DevModel &m, int64_t R, const int32_t *x, const int32_t *k, int32_t *x_out, uint8_t *neg,

@@ -283,6 +283,22 @@ def test_exact_sampler_matches_oracle(cuda, prec, mode, bs):
     assert not np.array_equal(normal.final_x, res.final_x)  # the option changes the draws
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("mode,bs", [("persistent", 64), ("persistent", 256), ("persistent", 1024), ("plain", 0)])
+def test_fast_sampler_matches_oracle(cuda, prec, mode, bs):
+    # the cyclic chain draws means of 30-80 per leap: the fast option samples
+    # them by PTRS in every form, as the oracle does
+    m, om = pair(models.cyclic_chain(256, seed=9))
+    R, tf = 1024, 0.5
+    res = m.simulate(tf, R, seed=66, precision=prec, mode=mode, block_size=bs, poisson="fast")
+    ora = om.simulate(R, tf, seed=66, precision=prec, poisson="fast")
+    assert res.n_failed == 0 and (ora["status"] == 0).all()
+    same = _compare(res.final_x, ora["x_final"])
+    assert np.array_equal(res.steps[same], ora["steps"][same])
+    ref = m.simulate(tf, R, seed=66, precision=prec, mode=mode, block_size=bs)
+    assert not np.array_equal(ref.final_x, res.final_x)  # the option changes the draws
+
+
 def test_exact_sampler_law(cuda):
     # a source at rate 1e4 from x0 = 1e6: tau1 = eps*x0/c = 3 > t_final, so the
     # run is one leap clamped to t_final and x - x0 is one firing count of

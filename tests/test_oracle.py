@@ -554,3 +554,21 @@ def test_ring_jacobian_quadrature_matches_frechet():
     ref = cyclic_mean_jacobian(np.ones(N), x0, T)
     got = cyclic_ring_jacobian(x0, T, list(range(N)))
     np.testing.assert_allclose(got, ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
+
+
+def test_fast_sampler_is_ptrs_from_10():
+    # the 'fast' option draws means in [10, 80) by PTRS on the channel's
+    # stream (PTRS is pinned by test_ptrs_is_poisson), and keeps the Knuth
+    # product below 10 and the normal approximation from 80
+    om = oracle_of(models.dimer_decay(100))
+    for mean in (0.7, 4.0, 9.99):
+        for rid in range(50):
+            assert om.poisson(mean, 5, rid, 3, 2, poisson="fast") == om.poisson(mean, 5, rid, 3, 2)
+    for mean in (10.0, 25.0, 79.9):
+        got = [om.poisson(mean, 5, rid, 3, 2, poisson="fast") for rid in range(200)]
+        assert got == [om.poisson_ptrs(mean, 5, rid, 3, 2) for rid in range(200)]
+        assert got != [om.poisson(mean, 5, rid, 3, 2) for rid in range(200)]
+    for mean in (80.0, 300.0):
+        for rid in range(50):
+            assert om.poisson(mean, 5, rid, 3, 2, poisson="fast") == om.poisson(mean, 5, rid, 3, 2)
+            assert om.poisson(mean, 5, rid, 3, 2, poisson="fast-exact") == om.poisson_ptrs(mean, 5, rid, 3, 2)
