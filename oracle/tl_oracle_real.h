@@ -23,8 +23,9 @@ static inline REAL FN(unif)(uint32_t b) {
 #define R_SQRT sqrt
 #define R_FABS fabs
 #else
-#define R_EXP expf
-#define R_LOG logf
+/* FP32 exp and log in FP64, rounded once (as the engine evaluates them) */
+#define R_EXP(v) ((float)exp((double)(v)))
+#define R_LOG(v) ((float)log((double)(v)))
 #define R_SQRT sqrtf
 #define R_FABS fabsf
 #endif

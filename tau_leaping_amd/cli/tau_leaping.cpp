@@ -182,7 +182,7 @@ int main(int argc, char **argv) {
     std::vector<int32_t> sim_track = track.empty() ? std::vector<int32_t>{0} : track;
     p.n_track = (int32_t)sim_track.size();
     p.track = sim_track.data();
-    p.n_intervals = (A.mode == TL_MODE_PLAIN) ? 0 : A.intervals;
+    p.n_intervals = A.intervals;
     p.sensitivities = A.sens ? 1 : 0;
     p.device = A.device;
     p.mode = A.mode;
@@ -208,19 +208,6 @@ int main(int argc, char **argv) {
     if (st != TL_OK) {
         tl_model_destroy(m);
         return fail(st);
-    }
-    if (p.n_intervals == 0 && A.intervals > 0) {  // plain form: t = 0 row from x0
-        std::vector<int32_t> x0(N);
-        tl_model_get_x0(m, x0.data());
-        std::vector<double> m2((size_t)2 * T), s2((size_t)2 * T, 0.0), t2{0.0, A.t_end};
-        for (int q = 0; q < T; ++q) {
-            m2[q] = x0[sim_track[q]];
-            m2[T + q] = mean[q];
-            s2[T + q] = sd[q];
-        }
-        mean.swap(m2);
-        sd.swap(s2);
-        times.swap(t2);
     }
     const int rows = (int)times.size();
     const int Tout = (int)track.size();

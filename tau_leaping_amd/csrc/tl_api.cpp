@@ -313,25 +313,6 @@ std::vector<int32_t> track_list(const tl_model *m, const tl_sim_params *p) {
     return t;
 }
 
-// mean and sample standard deviation (n-1) of [R][T] samples, in FP64
-// (StepStat2StdDev, statistics.cpp:67-87).
-template <class T>
-void moments(const T *x, int64_t R, int32_t nT, double *mean, double *sd) {
-    std::vector<double> s(nT, 0.0), s2(nT, 0.0);
-    for (int64_t r = 0; r < R; ++r)
-        for (int32_t q = 0; q < nT; ++q) s[q] += (double)x[r * nT + q];
-    for (int32_t q = 0; q < nT; ++q) s[q] /= (double)R;
-    for (int64_t r = 0; r < R; ++r)
-        for (int32_t q = 0; q < nT; ++q) {
-            const double d = (double)x[r * nT + q] - s[q];
-            s2[q] += d * d;
-        }
-    for (int32_t q = 0; q < nT; ++q) {
-        if (mean) mean[q] = s[q];
-        if (sd) sd[q] = R > 1 ? std::sqrt(s2[q] / (double)(R - 1)) : 0.0;
-    }
-}
-
 void fill_summary(tl_result *res, const std::vector<int64_t> &steps, const std::vector<int64_t> &ssa,
                   const std::vector<int64_t> &rej, const std::vector<int32_t> &status) {
     res->total_steps = 0;
@@ -353,18 +334,33 @@ void fill_summary(tl_result *res, const std::vector<int64_t> &steps, const std::
     if (res->status) std::copy(status.begin(), status.end(), res->status);
 }
 
-// Moments over realizations of [R][n_grid][T] grid values, per grid point,
-// into res->mean / stddev [n_grid][T] (and the grid times).
-void grid_moments(const std::vector<double> &hg, int64_t R, int32_t n_grid, int32_t T, double t_final,
-                  tl_result *res) {
-    std::vector<double> slice((size_t)R * T);
-    for (int32_t g = 0; g < n_grid; ++g) {
-        for (int64_t r = 0; r < R; ++r)
-            for (int32_t q = 0; q < T; ++q) slice[r * T + q] = hg[((size_t)r * n_grid + g) * T + q];
-        moments(slice.data(), R, T, res->mean ? res->mean + (size_t)g * T : nullptr,
-                res->stddev ? res->stddev + (size_t)g * T : nullptr);
-        if (res->grid_times) res->grid_times[g] = t_final / (n_grid - 1) * g;
+// The moments of a run on the device (mean and sample standard deviation of
+// each tracked species, at t_final or per grid point: StepStat2StdDev,
+// statistics.cpp:67-87), then the copies the caller asked for: the moments,
+// final_x [R][T] and grid [R][n_grid][T].  Asynchronous on s.
+tl_status finish_outputs(DevBufs &B, hipStream_t s, int64_t R, int32_t T, int32_t n_grid, double t_final,
+                         const int32_t *d_outx, const double *d_grid, tl_result *res) {
+    if (!res) return TL_OK;
+    const int64_t C = (int64_t)n_grid * T;
+    if (res->mean || res->stddev) {
+        double *mom;
+        TL_TRY(B.alloc(mom, (size_t)2 * C));
+        if (d_grid)
+            launch_moments_f64(R, C, d_grid, mom, mom + C, s);
+        else
+            launch_moments_i32(R, T, d_outx, mom, mom + C, s);
+        if (res->mean) TL_HIP(hipMemcpyAsync(res->mean, mom, C * 8, hipMemcpyDeviceToHost, s));
+        if (res->stddev) TL_HIP(hipMemcpyAsync(res->stddev, mom + C, C * 8, hipMemcpyDeviceToHost, s));
     }
+    if (res->final_x) TL_HIP(hipMemcpyAsync(res->final_x, d_outx, (size_t)R * T * 4, hipMemcpyDeviceToHost, s));
+    if (res->grid && d_grid) TL_HIP(hipMemcpyAsync(res->grid, d_grid, (size_t)R * C * 8, hipMemcpyDeviceToHost, s));
+    if (res->grid_times) {
+        if (d_grid)
+            for (int32_t g = 0; g < n_grid; ++g) res->grid_times[g] = t_final / (n_grid - 1) * g;
+        else
+            res->grid_times[0] = t_final;
+    }
+    return TL_OK;
 }
 
 // ------------------------------------------------------------ plain loop --
@@ -525,21 +521,20 @@ tl_status run_plain(const tl_model *m, const tl_sim_params *p, tl_result *res) {
     if (sens && res && (res->score || want_grad)) {
         TL_TRY(B.alloc(d_scoreT, (size_t)R * m->M));
         launch_transpose_f64(R, m->M, st.S, d_scoreT, s);
-        if (want_grad) launch_grad_reduce(R, T, m->M, d_outx, d_scoreT, d_mean, d_dmean, d_dse, s);
+        if (want_grad) {
+            const int32_t RS = grad_slices(R, T, m->M);
+            double *gpart;
+            TL_TRY(B.alloc(gpart, (size_t)2 * RS * T * m->M));
+            launch_grad_reduce(R, T, m->M, d_outx, d_scoreT, d_mean, gpart, RS, d_dmean, d_dse, s);
+        }
     }
-    TL_HIP(hipGetLastError());
-    std::vector<int32_t> hx((size_t)T * R), hst(R);
+    TL_TRY(finish_outputs(B, s, R, T, n_grid, p->t_final, d_outx, d_grid, res));
+    std::vector<int32_t> hst(R);
     std::vector<int64_t> hsteps(R), hssa(R), hrej(R);
-    TL_HIP(hipMemcpyAsync(hx.data(), d_outx, hx.size() * 4, hipMemcpyDeviceToHost, s));
     TL_HIP(hipMemcpyAsync(hsteps.data(), st.steps, R * 8, hipMemcpyDeviceToHost, s));
     TL_HIP(hipMemcpyAsync(hssa.data(), st.ssa, R * 8, hipMemcpyDeviceToHost, s));
     TL_HIP(hipMemcpyAsync(hrej.data(), st.rej, R * 8, hipMemcpyDeviceToHost, s));
     TL_HIP(hipMemcpyAsync(hst.data(), st.status, R * 4, hipMemcpyDeviceToHost, s));
-    std::vector<double> hg;
-    if (grid) {
-        hg.resize((size_t)R * n_grid * T);
-        TL_HIP(hipMemcpyAsync(hg.data(), d_grid, hg.size() * 8, hipMemcpyDeviceToHost, s));
-    }
     if (res && sens) {
         if (res->score) TL_HIP(hipMemcpyAsync(res->score, d_scoreT, (size_t)R * m->M * 8, hipMemcpyDeviceToHost, s));
         if (res->dmean_dc)
@@ -555,14 +550,6 @@ tl_status run_plain(const tl_model *m, const tl_sim_params *p, tl_result *res) {
         res->upload_seconds = up;
         res->block_size = 256;
         res->form = TL_MODE_PLAIN;
-        if (res->final_x) std::copy(hx.begin(), hx.end(), res->final_x);
-        if (grid) {
-            if (res->grid) std::copy(hg.begin(), hg.end(), res->grid);
-            grid_moments(hg, R, n_grid, T, p->t_final, res);
-        } else {
-            moments(hx.data(), R, T, res->mean, res->stddev);
-            if (res->grid_times) res->grid_times[0] = p->t_final;
-        }
         fill_summary(res, hsteps, hssa, hrej, hst);
         res->sim_seconds = now_s() - t0;
     }
@@ -1086,17 +1073,27 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
     }
     TL_TRY(run_launches(*dc, is64, sens, launches, true, &ms));
     for (auto &e : E)
-        if (sens && e.res && (e.res->dmean_dc || e.res->dmean_dc_se))
-            launch_grad_reduce(e.p->n_realizations, (int32_t)e.track.size(), e.m->M, e.d_outx, e.d_score, e.d_mean,
-                               e.d_dmean, e.d_dse, s);
-    TL_HIP(hipStreamSynchronize(s));
+        if (sens && e.res && (e.res->dmean_dc || e.res->dmean_dc_se)) {
+            const int64_t R = e.p->n_realizations;
+            const int32_t T = (int32_t)e.track.size();
+            const int32_t RS = grad_slices(R, T, e.m->M);
+            double *gpart;
+            TL_TRY(B.alloc(gpart, (size_t)2 * RS * T * e.m->M));
+            launch_grad_reduce(R, T, e.m->M, e.d_outx, e.d_score, e.d_mean, gpart, RS, e.d_dmean, e.d_dse, s);
+        }
     for (auto &e : E) {
         if (!e.res) continue;
         const int64_t R = e.p->n_realizations;
         const int32_t T = (int32_t)e.track.size();
-        std::vector<int32_t> hx((size_t)R * T), hst(R);
+        const int32_t n_grid = e.p->n_intervals > 0 ? e.p->n_intervals + 1 : 1;
+        TL_TRY(finish_outputs(B, s, R, T, n_grid, e.p->t_final, e.d_outx, e.d_grid, e.res));
+    }
+    for (auto &e : E) {
+        if (!e.res) continue;
+        const int64_t R = e.p->n_realizations;
+        const int32_t T = (int32_t)e.track.size();
+        std::vector<int32_t> hst(R);
         std::vector<int64_t> hsteps(R), hssa(R), hrej(R);
-        TL_HIP(hipMemcpyAsync(hx.data(), e.d_outx, hx.size() * 4, hipMemcpyDeviceToHost, s));
         TL_HIP(hipMemcpyAsync(hsteps.data(), e.d_steps, R * 8, hipMemcpyDeviceToHost, s));
         TL_HIP(hipMemcpyAsync(hssa.data(), e.d_ssa, R * 8, hipMemcpyDeviceToHost, s));
         TL_HIP(hipMemcpyAsync(hrej.data(), e.d_rej, R * 8, hipMemcpyDeviceToHost, s));
@@ -1107,22 +1104,8 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
             TL_HIP(hipMemcpyAsync(e.res->dmean_dc_se, e.d_dse, (size_t)T * e.m->M * 8, hipMemcpyDeviceToHost, s));
         if (sens && e.res->score)
             TL_HIP(hipMemcpyAsync(e.res->score, e.d_score, (size_t)R * e.m->M * 8, hipMemcpyDeviceToHost, s));
-        std::vector<double> hg;
-        const int32_t n_grid = e.p->n_intervals > 0 ? e.p->n_intervals + 1 : 1;
-        if (e.p->n_intervals > 0) {
-            hg.resize((size_t)R * n_grid * T);
-            TL_HIP(hipMemcpyAsync(hg.data(), e.d_grid, hg.size() * 8, hipMemcpyDeviceToHost, s));
-        }
         TL_HIP(hipStreamSynchronize(s));
         tl_result *res = e.res;
-        if (res->final_x) std::copy(hx.begin(), hx.end(), res->final_x);
-        if (e.p->n_intervals > 0) {
-            if (res->grid) std::copy(hg.begin(), hg.end(), res->grid);
-            grid_moments(hg, R, n_grid, T, e.p->t_final, res);
-        } else {
-            moments(hx.data(), R, T, res->mean, res->stddev);
-            if (res->grid_times) res->grid_times[0] = e.p->t_final;
-        }
         fill_summary(res, hsteps, hssa, hrej, hst);
         res->kernel_ms = ms;
         res->upload_seconds = up_total;

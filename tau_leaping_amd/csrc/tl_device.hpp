@@ -100,10 +100,15 @@ template <> struct Real<double> {
     static __device__ __forceinline__ double fabs_(double v) { return fabs(v); }
     static __device__ __forceinline__ const double *rates(const DevModel &m) { return m.c64; }
 };
+// FP32 exp and log are evaluated in FP64 and rounded once: the device's and
+// the host library's FP64 functions differ by at most an ulp of the double,
+// which the rounding to float hides except within an ulp of a float rounding
+// boundary, so the oracle's draws and the GPU's agree (FP32 library
+// functions differ between the two by an ulp of the float).
 template <> struct Real<float> {
     static __device__ __forceinline__ float unif(uint32_t b) { return unif32(b); }
-    static __device__ __forceinline__ float exp_(float v) { return expf(v); }
-    static __device__ __forceinline__ float log_(float v) { return logf(v); }
+    static __device__ __forceinline__ float exp_(float v) { return (float)exp((double)v); }
+    static __device__ __forceinline__ float log_(float v) { return (float)log((double)v); }
     static __device__ __forceinline__ float sqrt_(float v) { return sqrtf(v); }
     static __device__ __forceinline__ float fabs_(float v) { return fabsf(v); }
     static __device__ __forceinline__ const float *rates(const DevModel &m) { return m.c32; }

@@ -201,7 +201,17 @@ PersistentLaunch launch_plan(DeviceContext &dc, const PersistentConfig &cfg, boo
                              unsigned long long *d_counter);
 // The model's device constants if they are resident and current (no upload).
 tl_status device_model_resident(const tl_model &m, int device, DevModel &out);
+// ---------------------------------------------------------- reductions --
+// mean / sample standard deviation (n-1) of every column of v [R][C] (either
+// output may be null)
+void launch_moments_i32(int64_t R, int64_t C, const int32_t *v, double *mean, double *sd, hipStream_t s);
+void launch_moments_f64(int64_t R, int64_t C, const double *v, double *mean, double *sd, hipStream_t s);
+// dmean[q][j] = mean_r (X_q - Xbar_q) S_j and its standard error from
+// x_track [R][n_track] and score [R][M]; mean_scratch [n_track]; part
+// 2 * RS * n_track * M doubles, RS = grad_slices(R, n_track, M)
+int32_t grad_slices(int64_t R, int32_t n_track, int32_t M);
 void launch_grad_reduce(int64_t R, int32_t n_track, int32_t M, const int32_t *x_track, const double *score,
-                        double *mean_scratch, double *dmean, double *dmean_se, hipStream_t s);
+                        double *mean_scratch, double *part, int32_t RS, double *dmean, double *dmean_se,
+                        hipStream_t s);
 
 }  // namespace tl

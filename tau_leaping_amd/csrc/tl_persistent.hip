@@ -1407,46 +1407,6 @@ const void *kernel_for_tu(const PersistentConfig &c, bool is64, bool sens) {
                 : kernel_ptr<float, false>(c.block, c.group, c.a_lds, c.wpe, c.hbm);
 }
 
-#if !TL_PERSISTENT_EXT
-// ------------------------------------------------------- gradient reduce --
-__global__ __launch_bounds__(256) void k_track_mean(int64_t Rn, int32_t T, const int32_t *__restrict__ xt,
-                                                    double *__restrict__ mean) {
-    __shared__ double sh[256];
-    const int q = blockIdx.x;
-    double s = 0.0;
-    for (int64_t r = threadIdx.x; r < Rn; r += 256) s += (double)xt[r * T + q];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) mean[q] = sh[0] / (double)Rn;
-}
-
-// dmean[q][j] = mean_r (X_q^r - Xbar_q) S_j^r, and its standard error.
-__global__ __launch_bounds__(256) void k_grad(int64_t Rn, int32_t T, int32_t M, const int32_t *__restrict__ xt,
-                                              const double *__restrict__ S, const double *__restrict__ mean,
-                                              double *__restrict__ dmean, double *__restrict__ se) {
-    const int32_t j = blockIdx.x * 256 + threadIdx.x;
-    const int32_t q = blockIdx.y;
-    if (j >= M) return;
-    const double mq = mean[q];
-    double acc = 0.0, acc2 = 0.0;
-    for (int64_t r = 0; r < Rn; ++r) {
-        const double d = ((double)xt[r * T + q] - mq) * S[r * M + j];
-        acc += d;
-        acc2 += d * d;
-    }
-    const double mu = acc / (double)Rn;
-    dmean[(int64_t)q * M + j] = mu;
-    if (se) {
-        const double var = (Rn > 1) ? (acc2 / (double)Rn - mu * mu) * (double)Rn / (double)(Rn - 1) : 0.0;
-        se[(int64_t)q * M + j] = sqrt(fmax(var, 0.0) / (double)Rn);
-    }
-}
-#endif
-
 }  // namespace
 
 #if TL_PERSISTENT_EXT
@@ -1483,13 +1443,6 @@ hipError_t launch_persistent(const PersistentConfig &c, bool is64, bool sens, co
     if (!k) return hipErrorInvalidDeviceFunction;
     void *args[] = {const_cast<PersistentLaunch *>(&L)};
     return hipLaunchKernel(k, dim3((unsigned)L.grid), dim3((unsigned)c.block), args, L.lds_bytes, s);
-}
-
-void launch_grad_reduce(int64_t R, int32_t n_track, int32_t M, const int32_t *x_track, const double *score,
-                        double *mean_scratch, double *dmean, double *dmean_se, hipStream_t s) {
-    k_track_mean<<<(unsigned)n_track, 256, 0, s>>>(R, n_track, x_track, mean_scratch);
-    dim3 g((unsigned)((M + 255) / 256), (unsigned)n_track);
-    k_grad<<<g, 256, 0, s>>>(R, n_track, M, x_track, score, mean_scratch, dmean, dmean_se);
 }
 
 #endif  // !TL_PERSISTENT_EXT

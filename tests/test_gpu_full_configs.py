@@ -125,17 +125,18 @@ def test_hetero64_batch(cuda):
     kws = [dict(t_final=0.1, n_realizations=min(r, 512), seed=1234 + i, model_id=i, precision="fp32",
                 sensitivities=True, return_scores=True, track=np.arange(4, dtype=np.int32)) for i, r in enumerate(reals)]
     out = T.batch_simulate(ms, kws, mode="persistent")
-    same_all = []
+    ok_all = []
     for i, (net, res) in enumerate(zip(nets, out)):
         assert res.n_failed == 0
         x, steps, proj = fx[f"x_{i}"], fx[f"steps_{i}"], fx[f"proj_{i}"]
         n = steps.size
-        same = np.all(res.final_x[:n] == x, axis=1) & (res.steps[:n] == steps)
-        same_all.append(same)
-        assert same.mean() >= 0.9, (i, same.mean())
+        # the fixture tracks species 0-3, so the score projections are what
+        # shows a path that differs in the other species
         W = np.random.default_rng(7000 + i).standard_normal((net.n_channels, 4))
         got = res.score[:n] @ W
-        scale = np.abs(proj).max(axis=1, keepdims=True) + 1e-300
-        err = np.abs(got[same] - proj[same]) / scale[same]
-        assert np.quantile(err.max(axis=1), 0.99) < 1e-7, (i, np.quantile(err.max(axis=1), 0.99))
-    assert np.concatenate(same_all).mean() >= 0.97
+        scale = np.abs(proj).max(axis=1) + 1e-300
+        ok = (np.all(res.final_x[:n] == x, axis=1) & (res.steps[:n] == steps) &
+              (np.abs(got - proj).max(axis=1) < 1e-8 * scale))
+        ok_all.append(ok)
+        assert ok.mean() >= 0.9, (i, ok.mean())
+    assert np.concatenate(ok_all).mean() >= 0.97, np.concatenate(ok_all).mean()
