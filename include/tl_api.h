@@ -24,7 +24,8 @@
  * Conventions: every call returns tl_status (0 = OK) and never throws or
  * exits; tl_last_error() returns a thread-local message for the last failure.
  * Model handles are library-owned and immutable once built except through
- * tl_model_set_rates / tl_model_set_x0.  Result buffers are caller-owned.
+ * tl_model_set_rates / tl_model_set_x0 / tl_model_set_net_stoichiometry.
+ * Result buffers are caller-owned.
  * All simulation entry points require an AMD GPU; there is no CPU fallback.
  */
 #ifndef TL_API_H
@@ -104,8 +105,23 @@ tl_status tl_model_get_rates(const tl_model *m, double *rates);
 tl_status tl_model_set_rates(tl_model *m, const double *rates);
 tl_status tl_model_get_x0(const tl_model *m, int32_t *x0);
 tl_status tl_model_set_x0(tl_model *m, const int32_t *x0);
+/* tau bound from net stoichiometry (on != 0): the entries of one species in
+ * one channel (a catalyst's reactant and product entries) count once, with
+ * their sum, in the species sums mu_i = sum_j v_ij a_j and
+ * sigma^2_i = sum_j v_ij^2 a_j; off (the default, as
+ * tau_leaping/CSCHostMatrixData.cpp:74-103 builds the entries) each entry
+ * counts, so a catalyst adds to sigma^2 with no change of its population.
+ * Reaction orders, criticality and the state update are the same either way.
+ * The reference has no such switch (the tau bound of
+ * tau_leaping/CSRMatrixPacked.cu:311-358 always takes the entries as read).
+ * TL_ERR_STOICH_RANGE if a net coefficient leaves [-2, 5]. */
+tl_status tl_model_set_net_stoichiometry(tl_model *m, int32_t on);
+int32_t tl_model_net_stoichiometry(const tl_model *m);
+/* Entries of the packed layouts: tl_model_nnz, or fewer with net stoichiometry. */
+int32_t tl_model_layout_nnz(const tl_model *m);
 /* Derived structure (any pointer may be NULL): order[M], s0[M], s1[M],
- * species-major row_ptr[N+1] and packed entries csr[nnz] = (channel<<3)|((v+2)&7). */
+ * species-major row_ptr[N+1] and packed entries
+ * csr[tl_model_layout_nnz] = (channel<<3)|((v+2)&7). */
 tl_status tl_model_structure(const tl_model *m, int8_t *order, int32_t *s0, int32_t *s1,
                              int32_t *row_ptr, int32_t *csr);
 

@@ -262,6 +262,32 @@ int tlo_model_create(int32_t N, int32_t M, const int32_t *col_ptr, const int32_t
     return TLO_OK;
 }
 
+/* Net stoichiometry for the tau bound (the product's
+ * tl_model_set_net_stoichiometry; the reference has no such option, its tau
+ * bound takes every entry as read, tau_leaping/CSRMatrixPacked.cu:311-358).
+ * A species row lists its entries in channel order, so the two entries of a
+ * catalyst are adjacent: they merge in place into their sum, and a zero sum
+ * leaves the row.  The state update keeps the entries as read (same net
+ * change).  Idempotent; on TLO_ERR_STOICH_RANGE the model is unchanged. */
+int tlo_model_set_net_stoichiometry(tlo_model *m) {
+    for (int pass = 0; pass < 2; ++pass) {
+        int32_t w = 0;
+        for (int32_t i = 0; i < m->N; ++i) {
+            const int32_t e0 = m->row_ptr[i], e1 = m->row_ptr[i + 1];
+            if (pass) m->row_ptr[i] = w;
+            for (int32_t e = e0; e < e1;) {
+                const int32_t j = m->csr_packed[e] >> 3;
+                int32_t v = 0;
+                for (; e < e1 && (m->csr_packed[e] >> 3) == j; ++e) v += (m->csr_packed[e] & 7) - 2;
+                if (v < -2 || v > 5) return TLO_ERR_STOICH_RANGE; /* only in the checking pass */
+                if (pass && v != 0) m->csr_packed[w++] = (j << 3) + ((v + 2) & 7);
+            }
+        }
+        if (pass) m->row_ptr[m->N] = w;
+    }
+    return TLO_OK;
+}
+
 /* ---- precision-generic part ---- */
 #define REAL double
 #define SUF f64

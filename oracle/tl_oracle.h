@@ -125,6 +125,11 @@ int tlo_model_create(int32_t N, int32_t M, const int32_t *col_ptr, const int32_t
                      const int8_t *stoich, const double *rates, const int32_t *x0,
                      tlo_model **out);
 void tlo_model_destroy(tlo_model *m);
+/* tau bound over net stoichiometry: each species row keeps one entry per
+ * channel holding the sum of that channel's entries for the species, zero
+ * sums dropped (row_ptr / csr_packed rewritten; nnz keeps the entries as
+ * read). */
+int tlo_model_set_net_stoichiometry(tlo_model *m);
 
 /* ---- stage functions over structure-of-arrays buffers ----
  * x is [N][R] int32, a is [M][R] real, crit is [M][R] uint8, per-realization

@@ -84,6 +84,8 @@ def lib():
                                        C.POINTER(C.POINTER(_Model))]
         L.tlo_model_create.restype = C.c_int
         L.tlo_model_destroy.argtypes = [C.POINTER(_Model)]
+        L.tlo_model_set_net_stoichiometry.argtypes = [C.POINTER(_Model)]
+        L.tlo_model_set_net_stoichiometry.restype = C.c_int
         for suf, RP in (("f64", F64P), ("f32", F32P)):
             getattr(L, "tlo_propensities_" + suf).argtypes = [C.POINTER(_Model), C.c_int64, I32P, RP, U8P, F64P, F64P]
             getattr(L, "tlo_tau_" + suf).argtypes = [C.POINTER(_Model), C.c_int64, C.c_double, I32P, RP, U8P, F64P, I32P, I32P]
@@ -126,7 +128,7 @@ class OracleModel:
     """Model built from channel-major (CSC) entries, as the product's
     tl_model_from_csc takes them."""
 
-    def __init__(self, N, M, col_ptr, row_ind, stoich, rates, x0):
+    def __init__(self, N, M, col_ptr, row_ind, stoich, rates, x0, net_stoichiometry=False):
         self.N, self.M = int(N), int(M)
         ptr = C.POINTER(_Model)()
         st = lib().tlo_model_create(self.N, self.M, np.ascontiguousarray(col_ptr, np.int32),
@@ -137,13 +139,18 @@ class OracleModel:
         if st != 0:
             raise ValueError(f"tlo_model_create failed with status {st}")
         self._p = ptr
+        if net_stoichiometry:
+            st = lib().tlo_model_set_net_stoichiometry(ptr)
+            if st != 0:
+                raise ValueError(f"tlo_model_set_net_stoichiometry failed with status {st}")
         m = ptr.contents
         self.nnz = m.nnz
         self.order = np.ctypeslib.as_array(m.order, (self.M,)).copy()
         self.s0 = np.ctypeslib.as_array(m.s0, (self.M,)).copy()
         self.s1 = np.ctypeslib.as_array(m.s1, (self.M,)).copy()
         self.row_ptr = np.ctypeslib.as_array(m.row_ptr, (self.N + 1,)).copy()
-        self.csr_packed = np.ctypeslib.as_array(m.csr_packed, (max(self.nnz, 1),))[: self.nnz].copy()
+        nz = int(self.row_ptr[-1])
+        self.csr_packed = np.ctypeslib.as_array(m.csr_packed, (max(self.nnz, 1),))[:nz].copy()
         self.x0 = np.ascontiguousarray(x0, np.int32).copy()
 
     def __del__(self):

@@ -78,6 +78,9 @@ _SIGS = {
     "tl_model_get_x0": ([C.c_void_p, C.c_void_p], C.c_int32),
     "tl_model_set_x0": ([C.c_void_p, C.c_void_p], C.c_int32),
     "tl_model_structure": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int32),
+    "tl_model_set_net_stoichiometry": ([C.c_void_p, C.c_int32], C.c_int32),
+    "tl_model_net_stoichiometry": ([C.c_void_p], C.c_int32),
+    "tl_model_layout_nnz": ([C.c_void_p], C.c_int32),
     "tl_sim_params_default": ([C.POINTER(_Params)], None),
     "tl_simulate": ([C.c_void_p, C.POINTER(_Params), C.POINTER(_Result)], C.c_int32),
     "tl_batch_simulate": ([C.POINTER(C.c_void_p), C.c_int32, C.POINTER(_Params), C.POINTER(_Result), C.c_int32],
@@ -267,9 +270,24 @@ class Model:
             raise ValueError("x0 must have one entry per species")
         _check(lib().tl_model_set_x0(self._h, _ptr(v)))
 
+    @property
+    def net_stoichiometry(self) -> bool:
+        """Whether the tau bound takes a catalyst's entries as one net entry
+        (off: each entry as read, as the reference builds them)."""
+        return bool(lib().tl_model_net_stoichiometry(self._h))
+
+    @net_stoichiometry.setter
+    def net_stoichiometry(self, on):
+        _check(lib().tl_model_set_net_stoichiometry(self._h, 1 if on else 0))
+
+    @property
+    def layout_nnz(self) -> int:
+        """Entries of the packed layouts (fewer than nnz with net stoichiometry)."""
+        return lib().tl_model_layout_nnz(self._h)
+
     def structure(self):
         """(order, s0, s1, row_ptr, csr_packed) as the engine derived them."""
-        M, N, Z = self.n_channels, self.n_species, self.nnz
+        M, N, Z = self.n_channels, self.n_species, self.layout_nnz
         order = np.zeros(M, np.int8)
         s0 = np.zeros(M, np.int32)
         s1 = np.zeros(M, np.int32)

@@ -8,7 +8,8 @@
 // and writes mean_<log>.txt / stddev_<log>.txt with lines "t v1 v2 ..."
 // (StatToFile, statistics.cpp:89-111).  Extensions: --precision fp64|fp32,
 // --seed N, --poisson normal|exact|fast|fast-exact (the firing-count
-// sampler, see TL_POISSON_* in tl_api.h), --mode auto|persistent|plain (auto: persistent,
+// sampler, see TL_POISSON_* in tl_api.h), --net-stoichiometry (a catalyst
+// counts once, with its net coefficient, in the tau bound), --mode auto|persistent|plain (auto: persistent,
 // plain where it runs a model too large for LDS faster), --block-size N, --sensitivities (writes
 // dmean_<log>.txt: one line per tracked species, dE[X(t_final)]/dc_j),
 // --json (one JSON record of the run on stdout).
@@ -52,6 +53,7 @@ struct Args {
     int poisson = TL_POISSON_NORMAL;
     int block_size = 0;
     bool sens = false;
+    bool net = false;
     bool json = false;
 };
 
@@ -60,7 +62,8 @@ struct Args {
     std::cerr << "usage: tau_leaping -f model.xml -t t_end [-n realizations] [-r names...|all] [-e eps]\n"
                  "                   [-a max_iterations] [-d device] [-b log_name] [-i intervals]\n"
                  "                   [--precision fp64|fp32] [--seed N] [--mode auto|persistent|plain]\n"
-                 "                   [--poisson normal|exact|fast|fast-exact] [--block-size N] [--sensitivities] [--json]\n";
+                 "                   [--poisson normal|exact|fast|fast-exact] [--net-stoichiometry]\n"
+                 "                   [--block-size N] [--sensitivities] [--json]\n";
     std::exit(EXIT_FAILURE);
 }
 
@@ -116,6 +119,7 @@ Args parse(int argc, char **argv) {
             else usage("poisson must be normal, exact, fast or fast-exact");
         } else if (k == "--block-size") A.block_size = atoi(need(argc, argv, a));
         else if (k == "--sensitivities") A.sens = true;
+        else if (k == "--net-stoichiometry") A.net = true;
         else if (k == "--json") A.json = true;
         else if (k == "-h" || k == "--help") usage(nullptr);
         else usage(("unknown option " + k).c_str());
@@ -141,6 +145,10 @@ int main(int argc, char **argv) {
     tl_model *m = nullptr;
     tl_status st = tl_model_from_stochkit(A.file.c_str(), &m);
     if (st != TL_OK) return fail(st);
+    if (A.net && (st = tl_model_set_net_stoichiometry(m, 1)) != TL_OK) {
+        tl_model_destroy(m);
+        return fail(st);
+    }
     const int N = tl_model_n_species(m), M = tl_model_n_channels(m);
 
     std::vector<int32_t> track;

@@ -116,7 +116,7 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     const double t0 = now_s();
     // one allocation: c64 | c32 | s0 | s1 | row_ptr | csr | col_ptr | csc | x0 | order | g_order | rs |
     // dep_ptr | dep | colq | aff_ptr | aff | tile_ptr | aff_rs | aff_c64 | aff_c32 | s2
-    const size_t M = (size_t)m.M, N = (size_t)m.N, Z = (size_t)std::max(m.nnz, 1);
+    const size_t M = (size_t)m.M, N = (size_t)m.N, Z = (size_t)std::max(m.lnnz, 1);
     std::vector<size_t> off;
     size_t total = 0;
     auto add = [&](size_t bytes) {
@@ -155,9 +155,9 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     memcpy(&host[off[2]], m.s0.data(), M * 4);
     memcpy(&host[off[3]], m.s1.data(), M * 4);
     memcpy(&host[off[4]], m.row_ptr.data(), (N + 1) * 4);
-    if (m.nnz) memcpy(&host[off[5]], m.csr.data(), (size_t)m.nnz * 4);
-    memcpy(&host[off[6]], m.col_ptr.data(), (M + 1) * 4);
-    if (m.nnz) memcpy(&host[off[7]], m.csc.data(), (size_t)m.nnz * 4);
+    if (m.lnnz) memcpy(&host[off[5]], m.csr.data(), (size_t)m.lnnz * 4);
+    memcpy(&host[off[6]], m.lcol_ptr.data(), (M + 1) * 4);
+    if (m.lnnz) memcpy(&host[off[7]], m.csc.data(), (size_t)m.lnnz * 4);
     memcpy(&host[off[8]], m.x0.data(), N * 4);
     memcpy(&host[off[9]], m.order.data(), M);
     memcpy(&host[off[10]], m.g_order.data(), N);
@@ -208,7 +208,7 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     DevModel &d = c->dm;
     d.N = m.N;
     d.M = m.M;
-    d.nnz = m.nnz;
+    d.nnz = m.lnnz;
     d.c64 = (const double *)(b + off[0]);
     d.c32 = (const float *)(b + off[1]);
     d.s0 = (const int32_t *)(b + off[2]);
@@ -932,7 +932,7 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
         TL_TRY(device_model(*e.m, device, e.dm, &up));
         up_total += up;
         e.track = track_list(e.m, e.p);
-        e.cost = (double)e.p->n_realizations * (double)(e.m->M + e.m->nnz);
+        e.cost = (double)e.p->n_realizations * (double)(e.m->M + e.m->lnnz);
         const int64_t R = e.p->n_realizations;
         const int32_t T = (int32_t)e.track.size();
         TL_TRY(B.alloc(e.d_track, T));

@@ -137,18 +137,60 @@ tl_status build_model(tl_model &m, std::string &msg) {
     // extended formulas)
     if (ext) m.s2.swap(third);
     else m.s2.clear();
+    // layout entries: as read, or one entry per (channel, species) holding
+    // the net stoichiometry, in species order, zero sums dropped
+    std::vector<int32_t> net_row;
+    std::vector<int8_t> net_v;
+    const int32_t *lrow = m.row_ind.data();
+    const int8_t *lv = m.stoich.data();
+    if (m.net_stoich) {
+        m.lcol_ptr.assign(m.M + 1, 0);
+        net_row.reserve(m.nnz);
+        net_v.reserve(m.nnz);
+        std::vector<std::pair<int32_t, int32_t>> col;
+        for (int32_t j = 0; j < m.M; ++j) {
+            col.clear();
+            for (int32_t e = m.col_ptr[j]; e < m.col_ptr[j + 1]; ++e) col.emplace_back(m.row_ind[e], m.stoich[e]);
+            std::stable_sort(col.begin(), col.end(),
+                             [](const std::pair<int32_t, int32_t> &x, const std::pair<int32_t, int32_t> &y) {
+                                 return x.first < y.first;
+                             });
+            for (size_t k = 0; k < col.size();) {
+                int32_t v = 0;
+                size_t k1 = k;
+                for (; k1 < col.size() && col[k1].first == col[k].first; ++k1) v += col[k1].second;
+                if (v < -2 || v > 5) {
+                    msg = "net stoichiometry " + std::to_string(v) + " of species " + std::to_string(col[k].first) +
+                          " in channel " + std::to_string(j) + " outside [-2, 5]";
+                    return TL_ERR_STOICH_RANGE;
+                }
+                if (v != 0) {
+                    net_row.push_back(col[k].first);
+                    net_v.push_back((int8_t)v);
+                }
+                k = k1;
+            }
+            m.lcol_ptr[j + 1] = (int32_t)net_row.size();
+        }
+        lrow = net_row.data();
+        lv = net_v.data();
+    } else {
+        m.lcol_ptr.assign(m.col_ptr.begin(), m.col_ptr.begin() + m.M + 1);
+    }
+    m.lnnz = m.lcol_ptr[m.M];
+    const std::vector<int32_t> &lcol = m.lcol_ptr;
     // species-major transpose, rows in channel order
     m.row_ptr.assign(m.N + 1, 0);
-    for (int32_t e = 0; e < m.nnz; ++e) m.row_ptr[m.row_ind[e] + 1]++;
+    for (int32_t e = 0; e < m.lnnz; ++e) m.row_ptr[lrow[e] + 1]++;
     for (int32_t i = 0; i < m.N; ++i) m.row_ptr[i + 1] += m.row_ptr[i];
     std::vector<int32_t> fill(m.row_ptr.begin(), m.row_ptr.end() - 1);
-    m.csr.assign(m.nnz, 0);
-    m.csc.assign(m.nnz, 0);
+    m.csr.assign(m.lnnz, 0);
+    m.csc.assign(m.lnnz, 0);
     for (int32_t j = 0; j < m.M; ++j)
-        for (int32_t e = m.col_ptr[j]; e < m.col_ptr[j + 1]; ++e) {
-            const int32_t v3 = (m.stoich[e] + 2) & 7;
-            m.csr[fill[m.row_ind[e]]++] = (j << 3) | v3;
-            m.csc[e] = (m.row_ind[e] << 3) | v3;
+        for (int32_t e = lcol[j]; e < lcol[j + 1]; ++e) {
+            const int32_t v3 = (lv[e] + 2) & 7;
+            m.csr[fill[lrow[e]]++] = (j << 3) | v3;
+            m.csc[e] = (lrow[e] << 3) | v3;
         }
     m.g_order.assign(m.N, 0);
     for (int32_t i = 0; i < m.N && i < m.M; ++i) m.g_order[i] = m.order[i];
@@ -173,12 +215,12 @@ tl_status build_model(tl_model &m, std::string &msg) {
     }
     // fixed-width columns
     int32_t max_col = 0;
-    for (int32_t j = 0; j < m.M; ++j) max_col = std::max(max_col, m.col_ptr[j + 1] - m.col_ptr[j]);
+    for (int32_t j = 0; j < m.M; ++j) max_col = std::max(max_col, lcol[j + 1] - lcol[j]);
     m.colq.clear();
     if (max_col <= 4) {
         m.colq.assign((size_t)m.M * 4, 2);
         for (int32_t j = 0; j < m.M; ++j)
-            for (int32_t e = m.col_ptr[j]; e < m.col_ptr[j + 1]; ++e) m.colq[(size_t)j * 4 + (e - m.col_ptr[j])] = m.csc[e];
+            for (int32_t e = lcol[j]; e < lcol[j + 1]; ++e) m.colq[(size_t)j * 4 + (e - lcol[j])] = m.csc[e];
     }
     // affected-channel lists (union of the reactant-of lists of the column)
     {
@@ -188,8 +230,8 @@ tl_status build_model(tl_model &m, std::string &msg) {
         bool ok = true;
         for (int32_t j = 0; j < m.M && ok; ++j) {
             const size_t start = list.size();
-            for (int32_t e = m.col_ptr[j]; e < m.col_ptr[j + 1]; ++e) {
-                const int32_t i = m.row_ind[e];
+            for (int32_t e = lcol[j]; e < lcol[j + 1]; ++e) {
+                const int32_t i = lrow[e];
                 for (int32_t d = m.dep_ptr[i]; d < m.dep_ptr[i + 1]; ++d)
                     if (stamp[m.dep[d]] != j) {
                         stamp[m.dep[d]] = j;
@@ -327,6 +369,28 @@ tl_status tl_model_set_x0(tl_model *m, const int32_t *x0) {
     m->version++;
     return TL_OK;
 }
+
+tl_status tl_model_set_net_stoichiometry(tl_model *m, int32_t on) {
+    if (!m) return set_error(TL_ERR_INVALID_ARG, "null model");
+    std::lock_guard<std::mutex> g(m->dev_mu);
+    const bool prev = m->net_stoich;
+    if (prev == (on != 0)) return TL_OK;
+    m->net_stoich = on != 0;
+    std::string msg;
+    const tl_status st = build_model(*m, msg);
+    if (st != TL_OK) {
+        m->net_stoich = prev;
+        std::string again;
+        build_model(*m, again);
+        return set_error(st, msg);
+    }
+    m->version++;
+    return TL_OK;
+}
+
+int32_t tl_model_net_stoichiometry(const tl_model *m) { return m && m->net_stoich ? 1 : 0; }
+
+int32_t tl_model_layout_nnz(const tl_model *m) { return m ? m->lnnz : 0; }
 
 tl_status tl_model_structure(const tl_model *m, int8_t *order, int32_t *s0, int32_t *s1, int32_t *row_ptr,
                              int32_t *csr) {

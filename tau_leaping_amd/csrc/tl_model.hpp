@@ -33,7 +33,16 @@ struct tl_model {
     std::vector<std::string> names;
     std::unordered_map<std::string, int32_t> name_index;
 
+    // Net stoichiometry in the layouts below: the entries of one species in
+    // one channel (a catalyst: a reactant and a product entry) merge into one
+    // entry holding their sum, dropped when the sum is 0.  Off: the entries
+    // as read, so a catalyst adds v^2 a_j twice to sigma^2.  Reaction orders
+    // come from the entries as read either way.
+    bool net_stoich = false;
+
     // derived
+    std::vector<int32_t> lcol_ptr;  // [M+1] column offsets of csc (col_ptr unless net_stoich)
+    int32_t lnnz = 0;               // entries of csr / csc
     std::vector<int8_t> order;
     std::vector<int32_t> s0, s1;
     std::vector<int32_t> s2;  // third reactant of order-4 channels; empty unless an order 0, 4 or 5 exists

@@ -96,6 +96,20 @@ def dimer_decay(s1: int = 100000) -> Network:
     return Network(3, rx, np.array([s1, 0, 0], np.int32), ["S1", "S2", "S3"], "dimer_decay")
 
 
+def gene_expression(genes: int = 50) -> Network:
+    """A network with catalysts (species that are reactant and product of one
+    channel): transcription G -> G + M, translation M -> M + P, decays,
+    dimerisation 2P <-> D, repression D + G -> D + Gi with release Gi -> G,
+    and 2P -> P (net coefficient -1 of P)."""
+    rx = [Reaction({0: 1}, {0: 1, 1: 1}, 5.0), Reaction({1: 1}, {1: 1, 2: 1}, 2.0),
+          Reaction({1: 1}, {}, 1.0), Reaction({2: 1}, {}, 0.05),
+          Reaction({2: 2}, {3: 1}, 1e-5), Reaction({3: 1}, {2: 2}, 0.1),
+          Reaction({0: 1, 3: 1}, {3: 1, 4: 1}, 1e-5), Reaction({4: 1}, {0: 1}, 0.5),
+          Reaction({2: 2}, {2: 1}, 1e-6)]
+    return Network(5, rx, np.array([genes, 250, 5000, 1000, 0], np.int32), ["G", "M", "P", "D", "Gi"],
+                   "gene_expression")
+
+
 def _log_uniform(rng, lo, hi, size=None):
     return np.exp(rng.uniform(np.log(lo), np.log(hi), size))
 

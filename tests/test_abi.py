@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import tau_leaping_amd as T
+import tl_oracle as O
 from helpers import oracle_of
 from tau_leaping_amd import models
 from tau_leaping_amd.models import Network, Reaction
@@ -51,6 +52,40 @@ def test_model_structure_matches_oracle():
         assert m.n_species == net.n_species and m.n_channels == net.n_channels
         np.testing.assert_array_equal(m.rates, [r.rate for r in net.reactions])
         np.testing.assert_array_equal(m.x0, net.x0)
+
+
+def test_net_stoichiometry_layout_matches_oracle():
+    # the engine merges per column, the oracle per species row; both must
+    # give the same packed rows, and switching back restores the entries as read
+    for net in (models.gene_expression(), models.dimer_decay(), models.toy_network(40, 90, seed=1)):
+        m = T.Model.from_network(net)
+        lit = m.structure()
+        assert not m.net_stoichiometry and m.layout_nnz == m.nnz
+        m.net_stoichiometry = True
+        assert m.net_stoichiometry
+        om = O.OracleModel(net.n_species, net.n_channels, *net.to_csc(), net_stoichiometry=True)
+        order, s0, s1, rp, csr = m.structure()
+        # reaction orders come from the entries as read
+        assert np.array_equal(order, lit[0]) and np.array_equal(s0, lit[1]) and np.array_equal(s1, lit[2])
+        assert np.array_equal(rp, om.row_ptr) and np.array_equal(csr, om.csr_packed)
+        assert m.layout_nnz == rp[-1] and m.nnz == lit[3][-1]
+        m.net_stoichiometry = False
+        back = m.structure()
+        assert all(np.array_equal(u, v) for u, v in zip(back, lit))
+    ge = T.Model.from_network(models.gene_expression())
+    ge.net_stoichiometry = True
+    # G (catalyst of transcription) and the dimer D (catalyst of repression)
+    # lose those entries; 2P -> P keeps one entry of -1
+    assert ge.nnz - ge.layout_nnz == 7
+
+
+def test_net_stoichiometry_range_error():
+    # two product entries of one species: 5 + 5 leaves the 3-bit range once merged
+    m = T.Model.from_csc(2, [0, 3], [0, 1, 1], [-1, 5, 5], [1.0], [10, 0])
+    with pytest.raises(T.TLError) as e:
+        m.net_stoichiometry = True
+    assert e.value.status == 4
+    assert not m.net_stoichiometry and m.layout_nnz == 3
 
 
 def test_colloid_size():

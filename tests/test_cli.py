@@ -111,3 +111,28 @@ def test_simulator_end_to_end(tmp_path, cuda):
     ora = oracle_of(net).simulate(R, tf, seed=1234)["x_final"][:, [0, 2]].astype(float)
     se = np.sqrt(sd[-1, 1:] ** 2 / R + ora.var(0, ddof=1) / R)
     assert np.all(np.abs(mean[-1, 1:] - ora.mean(0)) < 3.3 * se)
+
+
+@pytest.mark.gpu
+def test_simulator_net_stoichiometry(tmp_path, cuda):
+    """--net-stoichiometry runs the model with catalysts counted once in the
+    tau bound: the same realization-steps as the binding with the switch on,
+    fewer than without it."""
+    import json
+
+    net = models.gene_expression()
+    path = tmp_path / "gene.xml"
+    path.write_text(net.to_stochkit_xml())
+    R, tf = 2000, 1.0
+    recs = {}
+    for flag in ([], ["--net-stoichiometry"]):
+        r = subprocess.run([SIM, "-f", str(path), "-t", str(tf), "-n", str(R), "-r", "all", "--seed", "5",
+                            "-b", "g", "--json"] + flag, capture_output=True, text=True, timeout=300,
+                           cwd=str(tmp_path))
+        assert r.returncode == 0, r.stderr
+        recs[bool(flag)] = json.loads(r.stdout.strip().splitlines()[-1])
+    m = T.Model.from_stochkit(str(path))
+    m.net_stoichiometry = True
+    res = m.simulate(tf, R, seed=5)
+    assert recs[True]["total_steps"] == res.total_steps
+    assert recs[False]["total_steps"] > 1.3 * recs[True]["total_steps"]

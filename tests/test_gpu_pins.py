@@ -10,12 +10,17 @@ closed forms of tests/test_oracle_pins.py.
   * grids: each realization's grid equals the oracle's on identical paths,
     and the one-molecule decay's grid is 1 - g/T1 exactly;
   * normal-branch score: per-realization scores of one clamped leap at mean
-    500 are the exact discrete-law score to within the approximation's 0.03.
+    500 are the exact discrete-law score to within the approximation's 0.03;
+  * net stoichiometry (catalysts counted once in the tau bound): a network
+    with catalysts, switched to net entries after a run with the entries as
+    read, follows the oracle's net-stoichiometry paths and scores in every
+    form; the entries as read take more, shorter leaps.
 """
 import numpy as np
 import pytest
 
 import tau_leaping_amd as T
+import tl_oracle as O
 from helpers import pair
 from tau_leaping_amd import models
 from test_oracle_pins import (REJ_EPS, REJ_TF, check_against_law, exact_normal_term, grid_closed_form,
@@ -109,3 +114,25 @@ def test_normal_branch_score_on_gpu(cuda, prec, bs):
     same = res.final_x[:, 0] == ora["x_final"][:, 0]
     assert same.mean() >= 0.99
     np.testing.assert_allclose(res.score[same, 0], ora["score"][same, 0], rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("mode,bs", SHAPES)
+def test_net_stoichiometry_matches_oracle(cuda, prec, mode, bs):
+    net = models.gene_expression()
+    m = T.Model.from_network(net)
+    R, tf = 2048, 1.0
+    lit = m.simulate(tf, R, seed=77, precision=prec, mode=mode, block_size=bs)
+    m.net_stoichiometry = True  # the device copy of the model is rebuilt
+    res = m.simulate(tf, R, seed=77, precision=prec, mode=mode, block_size=bs, sensitivities=True,
+                     return_scores=True)
+    om = O.OracleModel(net.n_species, net.n_channels, *net.to_csc(), net_stoichiometry=True)
+    ora = om.simulate(R, tf, seed=77, precision=prec, sensitivities=True)
+    assert res.form == mode and res.n_failed == 0 and lit.n_failed == 0 and (ora["status"] == 0).all()
+    same = np.all(res.final_x == ora["x_final"], axis=1) & (res.steps == ora["steps"])
+    assert same.mean() >= 0.97, same.mean()
+    S, So = res.score[same], ora["score"][same]
+    scale = np.abs(So).max(axis=1, keepdims=True) + 1e-300
+    assert np.quantile((np.abs(S - So) / scale).max(axis=1), 0.99) < 1e-8
+    # the double-counted catalysts shrink tau: more leaps per realization
+    assert lit.steps.mean() > 1.3 * res.steps.mean(), (lit.steps.mean(), res.steps.mean())

@@ -28,6 +28,12 @@ restatement of the oracle's code:
     approximation of m d/dm log P_m(k); the exact value follows from normal
     CDFs and differs from the approximation by less than 0.02 at m = 500
     (0.7 when the variance correction is dropped).
+  * catalysts in the tau bound: S0 -> S0 + S1, S1 -> 0, 2 S2 -> S2.  With
+    the entries as read the catalyst S0 adds 2 a0 to sigma0^2 (its -1 and +1
+    entries) and 2 S2 -> S2 adds (4 + 1) a2; with net stoichiometry S0 drops
+    out of the bound and S2 adds a2.  tau1, its species and its branch are
+    fixed in closed form for 256 states; planted mistakes in the merge (no
+    merge, summed magnitudes, last entry only) each break the net pin.
 """
 import math
 import os
@@ -141,23 +147,30 @@ PLANTED = {
 }
 
 
-@pytest.mark.parametrize("mistake", sorted(PLANTED))
-def test_rejection_law_catches_planted_mistakes(mistake, tmp_path):
+def run_planted(tmp_path, fname, pat, rep, test_id):
+    """Run one test of this file against a copy of the oracle with one
+    planted mistake (regex pat -> rep in fname, found exactly once); returns
+    the pytest run."""
     src = os.path.join(ROOT, "oracle")
     for f in ("tl_oracle.c", "tl_oracle.h", "tl_oracle_real.h"):
         shutil.copy(os.path.join(src, f), tmp_path / f)
-    real = (tmp_path / "tl_oracle_real.h").read_text()
-    pat, rep = PLANTED[mistake]
-    planted, n = re.subn(pat, rep, real)
-    assert n == 1, f"pattern for {mistake} found {n} times"
-    (tmp_path / "tl_oracle_real.h").write_text(planted)
+    text = (tmp_path / fname).read_text()
+    planted, n = re.subn(pat, rep, text)
+    assert n == 1, f"pattern {pat!r} found {n} times"
+    (tmp_path / fname).write_text(planted)
     so = tmp_path / "libtl_oracle.so"
     subprocess.run(["gcc", "-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-shared", "-o", str(so),
                     str(tmp_path / "tl_oracle.c"), "-lm", "-pthread"], check=True)
     env = dict(os.environ, TLO_ORACLE_LIB=str(so))
-    r = subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "-p", "no:cacheprovider",
-                        os.path.join(ROOT, "tests", "test_oracle_pins.py") + "::test_rejection_law[fp64]"],
-                       env=env, cwd=ROOT, capture_output=True, text=True, timeout=300)
+    return subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "-p", "no:cacheprovider",
+                           os.path.join(ROOT, "tests", "test_oracle_pins.py") + "::" + test_id],
+                          env=env, cwd=ROOT, capture_output=True, text=True, timeout=300)
+
+
+@pytest.mark.parametrize("mistake", sorted(PLANTED))
+def test_rejection_law_catches_planted_mistakes(mistake, tmp_path):
+    pat, rep = PLANTED[mistake]
+    r = run_planted(tmp_path, "tl_oracle_real.h", pat, rep, "test_rejection_law[fp64]")
     assert r.returncode != 0, f"the law did not catch {mistake}:\n{r.stdout[-2000:]}"
     assert "1 failed" in r.stdout, r.stdout[-2000:]
 
@@ -259,3 +272,98 @@ def test_normal_branch_dmean():
     m = c * tf
     exact = tf * (mean_floor_abs(m * (1 + 1e-5)) - mean_floor_abs(m * (1 - 1e-5))) / (2e-5 * m)
     assert abs(lr - exact) < 4.5 * se, (lr, exact, se)
+
+
+# ------------------------------------------------ net-stoichiometry catalysts --
+CAT_C = (50.0, 4.0, 1.0)
+
+
+def catalyst_net():
+    """S0 -> S0 + S1 (S0 a catalyst: net coefficient 0), S1 -> 0, and
+    2 S2 -> S2 (reactant 2, product 1: net -1)."""
+    rx = [Reaction({0: 1}, {0: 1, 1: 1}, CAT_C[0]), Reaction({1: 1}, {}, CAT_C[1]),
+          Reaction({2: 2}, {2: 1}, CAT_C[2])]
+    return Network(3, rx, np.array([100, 1000, 201], np.int32))
+
+
+def catalyst_tau_closed_form(x, eps, net):
+    """(tau1, argmin, which) of the catalyst network at states x [3][R], no
+    channel critical.  Species i < M takes g from channel i's order
+    (tau_leaping/CSRMatrixPacked.cu:311-358): g0 = g1 = 1, g2 = 2 + 1/(x2 - 1).
+    Entries as read: mu0 = 0, sigma0^2 = 2 a0 (the catalyst's -1 and +1 each
+    count); mu1 = a0 - a1, sigma1^2 = a0 + a1; mu2 = -a2, sigma2^2 = (4 + 1) a2.
+    Net: species 0 has no entry (tau0 infinite), sigma2^2 = a2."""
+    x = np.asarray(x, np.float64)
+    a0, a1 = CAT_C[0] * x[0], CAT_C[1] * x[1]
+    a2 = CAT_C[2] * x[2] * (x[2] - 1) / 2
+    q0 = np.maximum(eps * x[0], 1)
+    q1 = np.maximum(eps * x[1], 1)
+    q2 = np.maximum(eps * x[2] / (2 + 1 / (x[2] - 1)), 1)
+    inf = np.full_like(a0, np.inf)
+    cands = [  # (tau, species, which)
+        (inf if net else q0 ** 2 / (2 * a0), 0, 1),
+        (q1 / np.abs(a0 - a1), 1, 0), (q1 ** 2 / (a0 + a1), 1, 1),
+        (q2 / a2, 2, 0), (q2 ** 2 / (a2 if net else 5 * a2), 2, 1)]
+    taus = np.stack([c[0] for c in cands])
+    k = np.argmin(taus, axis=0)  # first minimum: species order, then mean before variance
+    return (taus[k, np.arange(taus.shape[1])], np.array([cands[i][1] for i in k]),
+            np.array([cands[i][2] for i in k]))
+
+
+def catalyst_states():
+    # hand-picked columns: the literal bound is set by the double-counted
+    # catalyst (species 0) or by sigma2^2 = 5 a2; the net bound by species 1 or
+    # by the mean of species 2; then random states
+    hand = np.array([[100, 1000, 201], [100, 1000, 3], [1000, 10000, 3], [60, 2000, 500]], np.int32).T
+    rng = np.random.default_rng(11)
+    rnd = np.stack([rng.integers(50, 2001, 252), rng.integers(50, 20001, 252), rng.integers(3, 501, 252)])
+    return np.ascontiguousarray(np.concatenate([hand, rnd.astype(np.int32)], axis=1))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("net", [False, True], ids=["as_read", "net"])
+def test_catalyst_tau_closed_form(prec, net):
+    eps = 0.03
+    net_model = catalyst_net()
+    om = O.OracleModel(3, 3, *net_model.to_csc(), net_stoichiometry=net)
+    x = catalyst_states()
+    rt = np.float64 if prec == "fp64" else np.float32
+    a = np.stack([CAT_C[0] * x[0], CAT_C[1] * x[1], (CAT_C[2] * x[2]) * (x[2] - 1.0) * 0.5]).astype(rt)
+    crit = np.zeros(a.shape, np.uint8)
+    tau, am, wh = om.tau(x, a, crit, eps, precision=prec)
+    ex, eam, ewh = catalyst_tau_closed_form(x, eps, net)
+    np.testing.assert_allclose(tau, ex, rtol=1e-12 if prec == "fp64" else 2e-6)
+    assert np.array_equal(am, eam) and np.array_equal(wh, ewh)
+    # the pin is not vacuous: each side's distinguishing bound sets tau somewhere
+    won = set(zip(eam.tolist(), ewh.tolist()))
+    assert ({(2, 0), (1, 0)} <= won) if net else ({(0, 1), (2, 1)} <= won), won
+    if net:
+        assert 0 not in eam
+
+
+def test_catalyst_net_layout_idempotent():
+    # merging the merged rows changes nothing
+    csc = catalyst_net().to_csc()
+    om = O.OracleModel(3, 3, *csc, net_stoichiometry=True)
+    assert O.lib().tlo_model_set_net_stoichiometry(om._p) == 0
+    m = om._p.contents
+    assert np.array_equal(np.ctypeslib.as_array(m.row_ptr, (4,)), om.row_ptr)
+    assert np.array_equal(np.ctypeslib.as_array(m.csr_packed, (int(om.row_ptr[-1]),)), om.csr_packed)
+    # species 0 has no entry; species 2 one entry of -1 in channel 2
+    assert list(om.row_ptr) == [0, 0, 2, 3]
+    assert om.csr_packed[-1] == (2 << 3) | (-1 + 2)
+
+
+PLANTED_NET = {
+    "no_merge": (r"(int tlo_model_set_net_stoichiometry\(tlo_model \*m\) \{)", r"\1\n    return TLO_OK;"),
+    "magnitudes": (r"v \+= \(m->csr_packed\[e\] & 7\) - 2;", r"v += abs((m->csr_packed[e] & 7) - 2);"),
+    "last_entry_only": (r"v \+= \(m->csr_packed\[e\] & 7\) - 2;", r"v = (m->csr_packed[e] & 7) - 2;"),
+}
+
+
+@pytest.mark.parametrize("mistake", sorted(PLANTED_NET))
+def test_catalyst_pin_catches_planted_mistakes(mistake, tmp_path):
+    pat, rep = PLANTED_NET[mistake]
+    r = run_planted(tmp_path, "tl_oracle.c", pat, rep, "test_catalyst_tau_closed_form[net-fp64]")
+    assert r.returncode != 0, f"the closed form did not catch {mistake}:\n{r.stdout[-2000:]}"
+    assert "1 failed" in r.stdout, r.stdout[-2000:]
