@@ -178,7 +178,8 @@ def cyclic_lr_vs_fd(T, models, np, R=262144, n_fd=8):
     lr, lrse, fd, fdse = map(np.array, (lr, lrse, fd, fdse))
     return {"realizations": R, "entries": int(fd.size), "rel_err": float(np.linalg.norm(lr - fd) / np.linalg.norm(fd)),
             "rel_noise": float(np.linalg.norm(np.hypot(lrse, fdse)) / np.linalg.norm(fd)), "bar": 0.05,
-            "max_abs_z": float((np.abs(lr - fd) / np.hypot(lrse, fdse)).max())}
+            "max_abs_z": float((np.abs(lr - fd) / np.hypot(lrse, fdse)).max()),
+            "mean_z2": float(np.mean(((lr - fd) / np.hypot(lrse, fdse)) ** 2))}
 
 
 def extras_run(T, models, np, m_toy, kw_toy):

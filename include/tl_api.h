@@ -44,9 +44,10 @@ enum {
     TL_OK = 0,
     TL_ERR_IO = 1,                /* file missing or unreadable (StochKitReader.cpp:127-131) */
     TL_ERR_PARSE = 2,             /* malformed XML / missing node / unknown rate id / count mismatch */
-    TL_ERR_UNSUPPORTED_ORDER = 3, /* reactant pattern other than ->, S->, 2S->, Sa+Sb->, Sa+Sb+Sc->, 2Sa+Sb->
-                                     (the reference accepts S->, 2S->, Sa+Sb->: CSCMatrix.cu:225-231) */
-    TL_ERR_STOICH_RANGE = 4,      /* stoichiometry outside [-2, 5] (3-bit packing, functions.h:126-130) */
+    TL_ERR_UNSUPPORTED_ORDER = 3, /* reactant pattern other than ->, S->, 2S->, Sa+Sb->, Sa+Sb+Sc->, 2Sa+Sb->,
+                                     3S-> (the reference accepts S->, 2S->, Sa+Sb->: CSCMatrix.cu:225-231) */
+    TL_ERR_STOICH_RANGE = 4,      /* stoichiometry outside [-3, 12] (4-bit packing; the reference's 3-bit
+                                     packing, functions.h:126-130, holds [-2, 5]) */
     TL_ERR_ALLOC = 5,
     TL_ERR_HIP = 6,
     TL_ERR_RETRY_EXHAUSTED = 7,   /* a leap was rejected more than 32 times in a row */
@@ -60,11 +61,13 @@ enum {
 enum { TL_FP64 = 0, TL_FP32 = 1 };
 
 /* Firing counts of non-critical channels (model.cu:477-520).  NORMAL
- * (default): the reference's sampler, the Knuth product below mean 80 and the
- * truncated normal |N(m, m)| from 80.  EXACT: PTRS (Hoermann 1993, an exact
- * Poisson sample) from 80 instead.  FAST: PTRS from mean 10 to 80 instead of
- * the Knuth product (the same Poisson law from a few uniforms instead of
- * mean + 1), |N(m, m)| from 80.  FAST_EXACT: PTRS from mean 10. */
+ * (default): below mean 80 a Poisson count by inversion of the channel's
+ * first uniform, the support searched outward from the mode (the reference
+ * multiplies mean + 1 uniforms, Knuth's product, for the same law), and the
+ * reference's truncated normal |N(m, m)| from 80.  EXACT: PTRS (Hoermann
+ * 1993, an exact Poisson sample) from 80 instead.  FAST: PTRS from mean 10
+ * to 80 instead of the inversion, |N(m, m)| from 80.  FAST_EXACT: PTRS from
+ * mean 10. */
 enum { TL_POISSON_NORMAL = 0, TL_POISSON_EXACT = 1, TL_POISSON_FAST = 2, TL_POISSON_FAST_EXACT = 3 };
 
 /* Engine forms: PLAIN = per-model host loop of stage kernels over
@@ -114,14 +117,15 @@ tl_status tl_model_set_x0(tl_model *m, const int32_t *x0);
  * Reaction orders, criticality and the state update are the same either way.
  * The reference has no such switch (the tau bound of
  * tau_leaping/CSRMatrixPacked.cu:311-358 always takes the entries as read).
- * TL_ERR_STOICH_RANGE if a net coefficient leaves [-2, 5]. */
+ * TL_ERR_STOICH_RANGE if a net coefficient leaves [-3, 12]. */
 tl_status tl_model_set_net_stoichiometry(tl_model *m, int32_t on);
 int32_t tl_model_net_stoichiometry(const tl_model *m);
 /* Entries of the packed layouts: tl_model_nnz, or fewer with net stoichiometry. */
 int32_t tl_model_layout_nnz(const tl_model *m);
 /* Derived structure (any pointer may be NULL): order[M], s0[M], s1[M],
  * species-major row_ptr[N+1] and packed entries
- * csr[tl_model_layout_nnz] = (channel<<3)|((v+2)&7). */
+ * csr[tl_model_layout_nnz] = (channel<<4)|(v+3).  order: 0 source, 1 S, 2 Sa+Sb,
+ * 3 2S, 4 Sa+Sb+Sc, 5 2Sa+Sb, 6 3S (the reactant side of the channel). */
 tl_status tl_model_structure(const tl_model *m, int8_t *order, int32_t *s0, int32_t *s1,
                              int32_t *row_ptr, int32_t *csr);
 

@@ -140,9 +140,9 @@ void tlo_model_destroy(tlo_model *m) {
 /* Reaction order from the negative entries of a column, in species order
  * (OrderDetectorAndIndFiller, CSCMatrix.cu:200-231).  A column without
  * negative entries is a source (order 0; the reference returns order -1 and
- * leaves its propensity unset); Sa+Sb+Sc (4) and 2Sa+Sb (5) have three
- * reactant molecules (the reference treats them as order 2).  Other patterns
- * are rejected. */
+ * leaves its propensity unset); Sa+Sb+Sc (4), 2Sa+Sb (5) and 3S (6) have
+ * three reactant molecules (the reference treats the first two as order 2
+ * and cannot pack the coefficient 3).  Other patterns are rejected. */
 static int detect_order(const tlo_model *m, int32_t j) {
     int32_t nneg = 0, v[3] = {0, 0, 0}, id[3] = {-1, -1, -1};
     for (int32_t e = m->col_ptr[j]; e < m->col_ptr[j + 1]; ++e) {
@@ -162,6 +162,12 @@ static int detect_order(const tlo_model *m, int32_t j) {
     }
     if (nneg == 1 && v[0] == -2) {
         m->order[j] = 3;
+        m->s0[j] = id[0];
+        m->s1[j] = id[0];
+        return TLO_OK;
+    }
+    if (nneg == 1 && v[0] == -3) {
+        m->order[j] = 6; /* 3S */
         m->s0[j] = id[0];
         m->s1[j] = id[0];
         return TLO_OK;
@@ -204,7 +210,8 @@ int tlo_model_create(int32_t N, int32_t M, const int32_t *col_ptr, const int32_t
     int32_t nnz = col_ptr[M];
     for (int32_t e = 0; e < nnz; ++e) {
         if (row_ind[e] < 0 || row_ind[e] >= N) return TLO_ERR_INVALID_ARG;
-        if (stoich[e] < -2 || stoich[e] > 5) return TLO_ERR_STOICH_RANGE; /* 3-bit packing, functions.h:126-130 */
+        /* 4-bit packed entries (the reference packs 3 bits, [-2, 5], functions.h:126-130) */
+        if (stoich[e] < -3 || stoich[e] > 12) return TLO_ERR_STOICH_RANGE;
     }
     tlo_model *m = (tlo_model *)calloc(1, sizeof(tlo_model));
     if (!m) return TLO_ERR_ALLOC;
@@ -255,7 +262,7 @@ int tlo_model_create(int32_t N, int32_t M, const int32_t *col_ptr, const int32_t
     memcpy(fill, m->row_ptr, sizeof(int32_t) * (size_t)N);
     for (int32_t j = 0; j < M; ++j)
         for (int32_t e = col_ptr[j]; e < col_ptr[j + 1]; ++e)
-            m->csr_packed[fill[row_ind[e]]++] = (j << 3) + ((stoich[e] + 2) & 7);
+            m->csr_packed[fill[row_ind[e]]++] = (j << 4) + (stoich[e] + 3);
     free(fill);
     for (int32_t i = 0; i < N; ++i) m->g_order[i] = (i < M) ? m->order[i] : 0;
     *out = m;
@@ -276,11 +283,11 @@ int tlo_model_set_net_stoichiometry(tlo_model *m) {
             const int32_t e0 = m->row_ptr[i], e1 = m->row_ptr[i + 1];
             if (pass) m->row_ptr[i] = w;
             for (int32_t e = e0; e < e1;) {
-                const int32_t j = m->csr_packed[e] >> 3;
+                const int32_t j = m->csr_packed[e] >> 4;
                 int32_t v = 0;
-                for (; e < e1 && (m->csr_packed[e] >> 3) == j; ++e) v += (m->csr_packed[e] & 7) - 2;
-                if (v < -2 || v > 5) return TLO_ERR_STOICH_RANGE; /* only in the checking pass */
-                if (pass && v != 0) m->csr_packed[w++] = (j << 3) + ((v + 2) & 7);
+                for (; e < e1 && (m->csr_packed[e] >> 4) == j; ++e) v += (m->csr_packed[e] & 15) - 3;
+                if (v < -3 || v > 12) return TLO_ERR_STOICH_RANGE; /* only in the checking pass */
+                if (pass && v != 0) m->csr_packed[w++] = (j << 4) + (v + 3);
             }
         }
         if (pass) m->row_ptr[m->N] = w;

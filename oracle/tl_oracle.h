@@ -39,22 +39,29 @@
  *     per-channel hybrid Tausworthe state (d_rng.cuh:27-35).
  *   - sources (no reactant, -> products) have a = c, are never critical and
  *     take g = 1 where GetG reads their order; channels with three reactant
- *     molecules, Sa+Sb+Sc (order 4: ((c*xa)*xb)*xc, g = 3) and 2Sa+Sb (order
- *     5: (((c*xa)*(xa-1))*0.5)*xb, g = 1.5*(2 + 1/(x-1))), are critical when a
- *     reactant count is below nc times its coefficient; other reactant
+ *     molecules, Sa+Sb+Sc (order 4: ((c*xa)*xb)*xc, g = 3), 2Sa+Sb (order
+ *     5: (((c*xa)*(xa-1))*0.5)*xb, g = 1.5*(2 + 1/(x-1))) and 3S (order 6:
+ *     (((c*x)*(x-1))*(x-2))/6, g = 3 + 1/(x-1) + 2/(x-2)), are critical when
+ *     a reactant's coefficient times its count is below nc; other reactant
  *     patterns are rejected at model build (ref: order -1 gives an
  *     uninitialised propensity, functions.h:118-120, CSCMatrix.cu:225-231,
  *     and a third reactant overwrites the second).
+ *   - packed entries hold 4 bits of stoichiometry, (index << 4) | (v + 3),
+ *     v in [-3, 12] (ref: 3 bits, (index << 3) | ((v + 2) & 7), [-2, 5]),
+ *     so that 3S can be written.
  *   - tau_i for species rows i >= M is +inf (ref reads order[] out of bounds,
  *     CSRMatrixPacked.cu:353).
  *   - the species sums mu_i, sigma^2_i of the tau bound accumulate in FP64 and
  *     are rounded once to the propensity type (ref: FP32 sums); v * a_j is
  *     exact in FP64, so the rounded sums do not depend on the order of the
  *     additions to within the last FP64 bit.
+ *   - mean < 80: sequential-search inversion of the channel's first uniform
+ *     (ref: Knuth's product of mean + 1 uniforms, capped at 300,
+ *     model.cu:483-513, for the same Poisson law; the search is capped at
+ *     k = 300 too).
  *   - options: an exact Poisson sampler (PTRS, Hoermann 1993) in place of
  *     the normal approximation at mean >= 80 (TLO_POISSON_EXACT), and in
- *     place of the Knuth product for 10 <= mean < 80 (TLO_POISSON_FAST: the
- *     same law from a few uniforms instead of mean + 1).
+ *     place of the inversion for 10 <= mean < 80 (TLO_POISSON_FAST).
  *   - the first Poisson draw of channel j at a step is output (j & 3) of the
  *     Philox block shared by channels 4*(j>>2)..4*(j>>2)+3; later draws come
  *     from the channel's own counter stream (see tlo_head_draw).
@@ -85,13 +92,13 @@ enum {
 #define TLO_SSA_FACTOR 10.0
 #define TLO_SSA_STEPS 100
 #define TLO_POISSON_SWITCH 80
-#define TLO_KNUTH_CAP 300
+#define TLO_INVERT_CAP 300
 #define TLO_MAX_RETRIES 32
 #define TLO_PTRS_FROM 10 /* PTRS from this mean (the method's range) with TLO_POISSON_FAST */
 
 /* sampler flags (tlo_params.poisson) */
 #define TLO_POISSON_EXACT 1 /* PTRS above mean 80 instead of the normal approximation */
-#define TLO_POISSON_FAST 2  /* PTRS from mean 10 instead of the Knuth product up to 80 */
+#define TLO_POISSON_FAST 2  /* PTRS from mean 10 instead of the inversion up to 80 */
 
 /* RNG stream kinds (ctr[3] low two bits) */
 #define TLO_KIND_POISSON 0u
@@ -107,10 +114,10 @@ typedef struct tlo_model {
     double *rates;       /* [M] */
     float *rates_f;      /* [M]    rates rounded once to float */
     int32_t *x0;         /* [N] */
-    int8_t *order;       /* [M]    0: ->, 1: S->, 2: Sa+Sb->, 3: 2S->, 4: Sa+Sb+Sc->, 5: 2Sa+Sb-> (s0 = a) */
+    int8_t *order;       /* [M]    0: ->, 1: S->, 2: Sa+Sb->, 3: 2S->, 4: Sa+Sb+Sc->, 5: 2Sa+Sb-> (s0 = a), 6: 3S-> */
     int32_t *s0, *s1;    /* [M]    reactant species (s1 == s0 for orders 1, 3) */
     int32_t *row_ptr;    /* [N+1]  species-major (CSR) */
-    int32_t *csr_packed; /* [nnz]  (channel << 3) | ((v + 2) & 7) */
+    int32_t *csr_packed; /* [nnz]  (channel << 4) | (v + 3), v in [-3, 12] */
     int8_t *g_order;     /* [N]    order[i] for i < M, else 0 */
     int32_t *s2;         /* [M]    third reactant of order-4 channels (else 0) */
 } tlo_model;

@@ -42,9 +42,9 @@ class _Model(C.Structure):
 
 
 def _poisson(p):
-    """Sampler flags: 'normal' (the reference's: Knuth below mean 80, the
-    normal approximation from 80), 'exact' (PTRS from 80), 'fast' (PTRS from
-    10 to 80), 'fast-exact' (PTRS from 10)."""
+    """Sampler flags: 'normal' (a Poisson count by inversion below mean 80,
+    the reference's normal approximation from 80), 'exact' (PTRS from 80),
+    'fast' (PTRS from 10 to 80), 'fast-exact' (PTRS from 10)."""
     v = {"normal": 0, "exact": 1, "ptrs": 1, "fast": 2, "fast-exact": 3}.get(p, p)
     if v not in (0, 1, 2, 3):
         raise ValueError(f"unknown Poisson sampler {p!r}")

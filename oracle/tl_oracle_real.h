@@ -52,6 +52,9 @@ static inline REAL FN(prop1)(int order, REAL c, int32_t x0, int32_t x1, int32_t 
     case 5: /* 2Sa + Sb */
         *crit = (x0 * 2 < TLO_NC) || (x1 < TLO_NC);
         return (((c * (REAL)x0) * (REAL)(x0 - 1)) * (REAL)0.5) * (REAL)x1;
+    case 6: /* 3S */
+        *crit = x0 * 3 < TLO_NC;
+        return (((c * (REAL)x0) * (REAL)(x0 - 1)) * (REAL)(x0 - 2)) / (REAL)6;
     case 1:
         *crit = x0 < TLO_NC;
         return c * (REAL)x0;
@@ -91,6 +94,7 @@ static inline REAL FN(getg)(int order, int32_t x) {
     case 3: return (REAL)2 + (REAL)1 / ((REAL)x - (REAL)1);
     case 4: return (REAL)3;
     case 5: return (REAL)1.5 * ((REAL)2 + (REAL)1 / ((REAL)x - (REAL)1));
+    case 6: return ((REAL)3 + (REAL)1 / ((REAL)x - (REAL)1)) + (REAL)2 / ((REAL)x - (REAL)2);
     default: return (REAL)0;
     }
 }
@@ -109,8 +113,8 @@ static double FN(tau_one)(const tlo_model *m, REAL eps, const int32_t *x, ptrdif
         double mu_d = 0, sig_d = 0;
         for (int32_t e = m->row_ptr[i]; e < m->row_ptr[i + 1]; ++e) {
             int32_t p = m->csr_packed[e];
-            int32_t j = p >> 3;
-            double v = (double)((p & 7) - 2);
+            int32_t j = p >> 4;
+            double v = (double)((p & 15) - 3);
             double tmp = v * (double)a[(ptrdiff_t)j * as] * (double)(!crit[(ptrdiff_t)j * as]);
             sig_d += v * tmp;
             mu_d += tmp;
@@ -135,12 +139,30 @@ static double FN(tau_one)(const tlo_model *m, REAL eps, const int32_t *x, ptrdif
     return (double)best;
 }
 
+/* Poisson count by inversion of one uniform u: the smallest k with
+ * u < F(k) = sum_{i <= k} P(i), P(0) = exp(-mean), P(i) = P(i-1) mean / i, in
+ * REAL arithmetic (sequential-search inversion, L. Devroye, Non-Uniform
+ * Random Variate Generation, Springer 1986, sec. X.3).  The search stops at
+ * TLO_INVERT_CAP. */
+static int32_t FN(poisson_invert)(REAL mean, REAL u) {
+    REAL p = R_EXP(-mean);
+    REAL F = p;
+    int32_t k = 0;
+    while (!(u < F) && k < TLO_INVERT_CAP) {
+        ++k;
+        p = (p * mean) / (REAL)k;
+        F += p;
+    }
+    return k;
+}
+
 /* Firing count of a non-critical channel (model.cu:477-520, functions.h:37-54),
  * flags TLO_POISSON_EXACT / TLO_POISSON_FAST:
- *   mean < 80 (< 10 with FAST): the Knuth product, lam = 1/exp(mean), cap 300;
+ *   mean < 80 (< 10 with FAST): inversion of draw 1 (poisson_invert; the
+ *     reference multiplies mean + 1 uniforms, Knuth's product, for the same
+ *     law, capped at 300 draws, as the search is capped at k = 300);
  *   10 <= mean < 80 with FAST, and above 80 with EXACT: PTRS (tlo_ptrs), an
- *     exact Poisson sample from a few uniforms (the Knuth product needs
- *     mean + 1);
+ *     exact Poisson sample from a few uniforms;
  *   otherwise: |polar Box-Muller(mean, var = mean)| truncated.
  * Draw 1 of the channel is `head` (see head_draw); draws 2, 3, ... come from
  * the channel's own stream, output ((n-2) & 3) of Philox block (n-2) >> 2 at
@@ -165,17 +187,9 @@ static int32_t FN(poisson_stream)(REAL mean, uint32_t head, const uint32_t key[2
         }                                             \
         (dst) = FN(unif)(blk[pos++]);                 \
     } while (0)
-    const REAL knuth_below = (flags & TLO_POISSON_FAST) ? (REAL)TLO_PTRS_FROM : (REAL)TLO_POISSON_SWITCH;
-    if (mean < knuth_below) {
-        REAL lam = (REAL)1 / R_EXP(mean);
-        REAL val = 1;
-        for (int32_t n = 1; n < TLO_KNUTH_CAP; ++n) {
-            REAL u;
-            NEXT_U(u);
-            val *= u;
-            if (val < lam) return n - 1;
-        }
-        return TLO_KNUTH_CAP;
+    const REAL invert_below = (flags & TLO_POISSON_FAST) ? (REAL)TLO_PTRS_FROM : (REAL)TLO_POISSON_SWITCH;
+    if (mean < invert_below) {
+        return FN(poisson_invert)(mean, FN(unif)(head));
     } else if (mean < (REAL)TLO_POISSON_SWITCH || (flags & TLO_POISSON_EXACT)) {
         return tlo_ptrs((double)mean, head, key, channel, epoch, ctr3);
     } else {
@@ -401,15 +415,15 @@ static void FN(score_leap)(const tlo_model *m, const REAL *a, const uint8_t *cri
     /* mean / variance of the argmin row recomputed in FP64 */
     double mu = 0.0, sig = 0.0;
     for (int32_t e = m->row_ptr[istar]; e < m->row_ptr[istar + 1]; ++e) {
-        int32_t p = m->csr_packed[e], j = p >> 3;
-        double v = (double)((p & 7) - 2);
+        int32_t p = m->csr_packed[e], j = p >> 4;
+        double v = (double)((p & 15) - 3);
         if (crit[j]) continue;
         mu += v * (double)a[j];
         sig += v * v * (double)a[j];
     }
     for (int32_t e = m->row_ptr[istar]; e < m->row_ptr[istar + 1]; ++e) {
-        int32_t p = m->csr_packed[e], j = p >> 3;
-        double v = (double)((p & 7) - 2);
+        int32_t p = m->csr_packed[e], j = p >> 4;
+        double v = (double)((p & 15) - 3);
         if (crit[j] || c[j] == 0) continue;
         double h = (double)a[j] / (double)c[j];
         double dtau_dc;

@@ -139,10 +139,9 @@ def _prec(p):
 
 
 def _poisson(p):
-    """Firing-count sampler: 'normal' (default, the reference's: the Knuth
-    product below mean 80, |N(m, m)| from 80), 'exact' (PTRS from 80), 'fast'
-    (PTRS from 10 to 80: the same law from fewer uniforms), 'fast-exact'
-    (PTRS from 10)."""
+    """Firing-count sampler: 'normal' (default: a Poisson count by inversion
+    below mean 80, the reference's |N(m, m)| from 80), 'exact' (PTRS from 80),
+    'fast' (PTRS from 10 to 80), 'fast-exact' (PTRS from 10)."""
     v = {"normal": POISSON_NORMAL, "exact": POISSON_EXACT, "fast": POISSON_FAST,
          "fast-exact": POISSON_FAST_EXACT}.get(p, p)
     if v not in (POISSON_NORMAL, POISSON_EXACT, POISSON_FAST, POISSON_FAST_EXACT):

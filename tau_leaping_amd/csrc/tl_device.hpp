@@ -20,8 +20,8 @@ namespace tl {
 constexpr int kNc = 5;                 // criticality threshold
 constexpr double kSsaFactor = 10.0;    // direct branch when tau1 < 10/a0
 constexpr int kSsaSteps = 100;         // direct events per direct step
-constexpr int kPoissonSwitch = 80;     // Knuth below, normal approximation at or above
-constexpr int kKnuthCap = 300;         // Knuth returns 300 after 299 draws
+constexpr int kPoissonSwitch = 80;     // inversion below, normal approximation at or above
+constexpr int kInvertCap = 300;        // inversion searches up to k = 300
 constexpr int kMaxRetries = 32;        // rejected leaps in a row before giving up
 constexpr float kFltMax = 3.402823466e+38f;
 constexpr int32_t kTileGrain = 4096;  // channels per tile_ptr column; block-group tau tiles are 1 or 2 grains
@@ -47,9 +47,9 @@ struct DevModel {
     const int32_t *s1;       // [M]
     const int32_t *s2;       // [M] third reactant of Sa+Sb+Sc channels; null unless some order is 0, 4 or 5
     const int32_t *row_ptr;  // [N+1] species-major
-    const int32_t *csr;      // [nnz] (channel << 3) | ((v + 2) & 7)
+    const int32_t *csr;      // [nnz] (channel << 4) | (v + 3)
     const int32_t *col_ptr;  // [M+1] channel-major
-    const int32_t *csc;      // [nnz] (species << 3) | ((v + 2) & 7)
+    const int32_t *csc;      // [nnz] (species << 4) | (v + 3)
     const int8_t *g_order;   // [N] order[i] for i < M else 0
     const int32_t *x0;       // [N]
     const int2 *rs;          // [M] (s0 | order << 29, s1): one load per channel
@@ -131,9 +131,10 @@ __device__ __forceinline__ R propensity(int order, R c, int32_t x0, int32_t x1, 
 
 // The forms beyond the reference's (models with m.s2 != null): order 0, a
 // source, a = c and never critical; channels with three reactant molecules,
-// order 4 Sa+Sb+Sc ((c*x0)*x1)*x2 and order 5 2Sa+Sb
-// (((c*x0)*(x0-1))*0.5)*x1 (x0 the species taken twice), critical when a
-// reactant count is below nc times its coefficient.  Orders 1-3 as
+// order 4 Sa+Sb+Sc ((c*x0)*x1)*x2, order 5 2Sa+Sb (((c*x0)*(x0-1))*0.5)*x1
+// (x0 the species taken twice) and order 6 3S (((c*x0)*(x0-1))*(x0-2))/6,
+// critical when a reactant's coefficient times its count is below nc (the
+// reference's 2S test, 2*x0 < nc, for every coefficient).  Orders 1-3 as
 // propensity().
 template <class R>
 __device__ __forceinline__ R propensity_ext(int order, R c, int32_t x0, int32_t x1, int32_t x2, bool &crit) {
@@ -149,12 +150,16 @@ __device__ __forceinline__ R propensity_ext(int order, R c, int32_t x0, int32_t 
         crit = (x0 * 2 < kNc) || (x1 < kNc);
         return (((c * (R)x0) * (R)(x0 - 1)) * (R)0.5) * (R)x1;
     }
+    if (order == 6) {
+        crit = x0 * 3 < kNc;
+        return (((c * (R)x0) * (R)(x0 - 1)) * (R)(x0 - 2)) / (R)6;
+    }
     return propensity<R>(order, c, x0, x1, crit);
 }
 
 // g of the tau bound for the order GetG reads (the channel-indexed order):
-// Cao-Gillespie-Petzold's highest-order forms, with 3 for Sa+Sb+Sc and
-// (3/2)(2 + 1/(x-1)) for 2Sa+Sb.
+// Cao-Gillespie-Petzold's highest-order forms, with 3 for Sa+Sb+Sc,
+// (3/2)(2 + 1/(x-1)) for 2Sa+Sb and 3 + 1/(x-1) + 2/(x-2) for 3S.
 template <class R>
 __device__ __forceinline__ R getg(int order, int32_t x) {
     if (order == 0 || order == 1) return (R)1;  // order 0: a source, the first-order bound
@@ -162,6 +167,7 @@ __device__ __forceinline__ R getg(int order, int32_t x) {
     if (order == 3) return (R)2 + (R)1 / ((R)x - (R)1);
     if (order == 4) return (R)3;
     if (order == 5) return (R)1.5 * ((R)2 + (R)1 / ((R)x - (R)1));
+    if (order == 6) return ((R)3 + (R)1 / ((R)x - (R)1)) + (R)2 / ((R)x - (R)2);
     return (R)0;
 }
 
@@ -243,53 +249,42 @@ __device__ __forceinline__ int32_t poisson_ptrs(double mean, uint32_t head, uint
 
 // Sampler flags (tl_sim_params.poisson): bit 0 = PTRS above mean 80 instead
 // of the reference's normal approximation; bit 1 = PTRS from mean 10 instead
-// of the reference's Knuth product up to 80 (same law, a few uniforms instead
-// of mean + 1).
+// of the inversion up to 80.
 constexpr int kPoissonExact = 1;
 constexpr int kPoissonFast = 2;
 constexpr float kPtrsFrom = 10.f;  // PTRS from this mean (the method's range) with kPoissonFast
 
-// Mean below which the Knuth product draws a non-critical channel's count.
+// Mean below which inversion draws a non-critical channel's count.
 template <class R>
-__device__ __forceinline__ R knuth_below(bool fast) {
+__device__ __forceinline__ R invert_below(bool fast) {
     return fast ? (R)kPtrsFrom : (R)kPoissonSwitch;
 }
 
-// The Knuth product (mean below knuth_below), capped at 300.  Draw 1 of the
-// channel is `head`; draw n >= 2 is output ((n-2) & 3) of the channel's own
-// Philox block (n-2) >> 2 at counter (block, channel, epoch, ctr3).  The loop
-// returns 0 on its first draw without evaluating exp when head < 1 - mean -
-// margin, a bound below the rounded 1/exp(mean) (the result is the one the
-// full loop returns).
+// Poisson count by inversion of the channel's first draw u: the smallest k
+// with u < F(k) = sum_{i <= k} P(i), P(0) = exp(-mean), P(i) = P(i-1) mean / i
+// (sequential-search inversion, L. Devroye, Non-Uniform Random Variate
+// Generation, Springer 1986, sec. X.3): the Poisson law from one uniform,
+// where the reference's Knuth product takes mean + 1.  The search stops at
+// kInvertCap (the mass beyond is below 1e-60 for means below 80).  It returns
+// 0 without evaluating exp when u < 1 - mean - margin, a bound below the
+// rounded exp(-mean) (the result is the one the full search returns).
 template <class R>
-__device__ __forceinline__ int32_t poisson_light(R mean, uint32_t head, uint2 key, uint32_t channel,
-                                                 uint32_t epoch, uint32_t ctr3) {
-    uint32_t blk = 0;
-    const R u1 = Real<R>::unif(head);
+__device__ __forceinline__ int32_t poisson_invert(R mean, uint32_t head) {
+    const R u = Real<R>::unif(head);
     const R margin = sizeof(R) == 8 ? (R)9.094947017729282e-13 : (R)9.5367431640625e-07;  // 2^-40 / 2^-20
-    if (mean < (R)1 && u1 < (R)1 - mean - margin) return 0;
-    const R lam = (R)1 / Real<R>::exp_(mean);
-    R val = u1;
-    if (val < lam) return 0;
-    int32_t n = 2;
-    for (;;) {
-        const uint4 o = philox4x32_10(make_uint4(blk++, channel, epoch, ctr3), key);
-        val *= Real<R>::unif(o.x);
-        if (val < lam) return n - 1;
-        if (++n == kKnuthCap) return kKnuthCap;
-        val *= Real<R>::unif(o.y);
-        if (val < lam) return n - 1;
-        if (++n == kKnuthCap) return kKnuthCap;
-        val *= Real<R>::unif(o.z);
-        if (val < lam) return n - 1;
-        if (++n == kKnuthCap) return kKnuthCap;
-        val *= Real<R>::unif(o.w);
-        if (val < lam) return n - 1;
-        if (++n == kKnuthCap) return kKnuthCap;
+    if (mean < (R)1 && u < (R)1 - mean - margin) return 0;
+    R p = Real<R>::exp_(-mean);
+    R F = p;
+    int32_t k = 0;
+    while (!(u < F) && k < kInvertCap) {
+        ++k;
+        p = (p * mean) / (R)k;
+        F += p;
     }
+    return k;
 }
 
-// The draw of a channel whose mean is at least knuth_below: PTRS below 80
+// The draw of a channel whose mean is at least invert_below: PTRS below 80
 // (with `fast`; and with `exact` above 80), else |mean + sqrt(mean) * z|
 // truncated, z from the polar method on the pairs (head, t0), (t1, t2),
 // (t3, t4), ... of the channel's draws.
@@ -321,13 +316,13 @@ __device__ __forceinline__ int32_t poisson_heavy(R mean, uint32_t head, uint2 ke
     return (int32_t)Real<R>::fabs_(y);
 }
 
-// Firing count of a non-critical channel: the Knuth product below 80 (below
-// 10 with `fast`, PTRS from there to 80), then the reference's normal
-// approximation, or PTRS with `exact`.
+// Firing count of a non-critical channel: inversion below 80 (below 10 with
+// `fast`, PTRS from there to 80), then the reference's normal approximation,
+// or PTRS with `exact`.
 template <class R>
 __device__ __forceinline__ int32_t poisson_draw(R mean, uint32_t head, uint2 key, uint32_t channel,
                                                 uint32_t epoch, uint32_t ctr3, bool exact, bool fast = false) {
-    if (mean < knuth_below<R>(fast)) return poisson_light<R>(mean, head, key, channel, epoch, ctr3);
+    if (mean < invert_below<R>(fast)) return poisson_invert<R>(mean, head);
     return poisson_heavy<R>(mean, head, key, channel, epoch, ctr3, exact);
 }
 
@@ -354,7 +349,8 @@ __device__ __forceinline__ double score_term(bool cr, R aj, R tr, double tau, in
     return (double)kj - mj;
 }
 
-__device__ __forceinline__ int32_t entry_index(int32_t p) { return p >> 3; }
-__device__ __forceinline__ int32_t entry_value(int32_t p) { return (p & 7) - 2; }
+// Packed entry (index << 4) | (v + 3), v in [-3, 12].
+__device__ __forceinline__ int32_t entry_index(int32_t p) { return p >> 4; }
+__device__ __forceinline__ int32_t entry_value(int32_t p) { return (p & 15) - 3; }
 
 }  // namespace tl

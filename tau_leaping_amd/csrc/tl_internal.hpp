@@ -152,7 +152,7 @@ struct ModelItem {
     double *tlast;         // [R][M] time up to which S_j holds its direct-event exposure (sensitivities) or null
     int32_t n_intervals;   // > 0: record tracked species on the grid k*t_final/n_intervals
     double *grid;          // [R][n_intervals+1][n_track]
-    int32_t poisson;       // sampler flags kPoissonExact | kPoissonKnuth (tl_sim_params.poisson)
+    int32_t poisson;       // sampler flags kPoissonExact | kPoissonFast (tl_sim_params.poisson)
 };
 
 struct PersistentConfig {

@@ -25,7 +25,7 @@ namespace tl {
 // Reaction order and reactant indices from the negative entries of column j in
 // species order.  Accepted patterns: -> (0, a source), S-> (1), Sa+Sb-> (2),
 // 2S-> (3), and three reactant molecules: Sa+Sb+Sc-> (4, third index in c),
-// 2Sa+Sb-> (5, a the species taken twice).
+// 2Sa+Sb-> (5, a the species taken twice), 3S-> (6).
 static bool channel_order(const tl_model &m, int32_t j, int8_t &order, int32_t &a, int32_t &b, int32_t &c) {
     int nneg = 0;
     int v[3] = {0, 0, 0};
@@ -50,6 +50,11 @@ static bool channel_order(const tl_model &m, int32_t j, int8_t &order, int32_t &
     }
     if (nneg == 1 && v[0] == -2) {
         order = 3;
+        a = b = id[0];
+        return true;
+    }
+    if (nneg == 1 && v[0] == -3) {
+        order = 6;  // 3S
         a = b = id[0];
         return true;
     }
@@ -80,8 +85,8 @@ tl_status build_model(tl_model &m, std::string &msg) {
         msg = "model needs at least one species and one channel";
         return TL_ERR_INVALID_ARG;
     }
-    if ((int64_t)m.M * 8 > INT32_MAX || (int64_t)m.N * 8 > INT32_MAX) {
-        msg = "model too large for the packed 29-bit index";
+    if ((int64_t)m.M * 16 > INT32_MAX || (int64_t)m.N * 16 > INT32_MAX) {
+        msg = "model too large for the packed 27-bit index";
         return TL_ERR_TOO_LARGE;
     }
     if ((int32_t)m.col_ptr.size() != m.M + 1 || m.col_ptr[0] != 0) {
@@ -103,8 +108,8 @@ tl_status build_model(tl_model &m, std::string &msg) {
             msg = "species index out of range at entry " + std::to_string(e);
             return TL_ERR_INVALID_ARG;
         }
-        if (m.stoich[e] < -2 || m.stoich[e] > 5) {
-            msg = "stoichiometry " + std::to_string((int)m.stoich[e]) + " outside [-2, 5] at channel entry " +
+        if (m.stoich[e] < -3 || m.stoich[e] > 12) {
+            msg = "stoichiometry " + std::to_string((int)m.stoich[e]) + " outside [-3, 12] at channel entry " +
                   std::to_string(e);
             return TL_ERR_STOICH_RANGE;
         }
@@ -127,7 +132,7 @@ tl_status build_model(tl_model &m, std::string &msg) {
     for (int32_t j = 0; j < m.M; ++j) {
         if (!channel_order(m, j, m.order[j], m.s0[j], m.s1[j], third[j])) {
             msg = "channel " + std::to_string(j) +
-                  ": unsupported reactant pattern (need ->, S->, 2S->, Sa+Sb->, Sa+Sb+Sc-> or 2Sa+Sb->)";
+                  ": unsupported reactant pattern (need ->, S->, 2S->, Sa+Sb->, Sa+Sb+Sc->, 2Sa+Sb-> or 3S->)";
             return TL_ERR_UNSUPPORTED_ORDER;
         }
         ext |= m.order[j] == 0 || m.order[j] >= 4;
@@ -159,9 +164,9 @@ tl_status build_model(tl_model &m, std::string &msg) {
                 int32_t v = 0;
                 size_t k1 = k;
                 for (; k1 < col.size() && col[k1].first == col[k].first; ++k1) v += col[k1].second;
-                if (v < -2 || v > 5) {
+                if (v < -3 || v > 12) {
                     msg = "net stoichiometry " + std::to_string(v) + " of species " + std::to_string(col[k].first) +
-                          " in channel " + std::to_string(j) + " outside [-2, 5]";
+                          " in channel " + std::to_string(j) + " outside [-3, 12]";
                     return TL_ERR_STOICH_RANGE;
                 }
                 if (v != 0) {
@@ -188,9 +193,9 @@ tl_status build_model(tl_model &m, std::string &msg) {
     m.csc.assign(m.lnnz, 0);
     for (int32_t j = 0; j < m.M; ++j)
         for (int32_t e = lcol[j]; e < lcol[j + 1]; ++e) {
-            const int32_t v3 = (lv[e] + 2) & 7;
-            m.csr[fill[lrow[e]]++] = (j << 3) | v3;
-            m.csc[e] = (lrow[e] << 3) | v3;
+            const int32_t v4 = lv[e] + 3;
+            m.csr[fill[lrow[e]]++] = (j << 4) | v4;
+            m.csc[e] = (lrow[e] << 4) | v4;
         }
     m.g_order.assign(m.N, 0);
     for (int32_t i = 0; i < m.N && i < m.M; ++i) m.g_order[i] = m.order[i];
@@ -218,7 +223,7 @@ tl_status build_model(tl_model &m, std::string &msg) {
     for (int32_t j = 0; j < m.M; ++j) max_col = std::max(max_col, lcol[j + 1] - lcol[j]);
     m.colq.clear();
     if (max_col <= 4) {
-        m.colq.assign((size_t)m.M * 4, 2);
+        m.colq.assign((size_t)m.M * 4, 3);  // padding: species 0, v = 0
         for (int32_t j = 0; j < m.M; ++j)
             for (int32_t e = lcol[j]; e < lcol[j + 1]; ++e) m.colq[(size_t)j * 4 + (e - lcol[j])] = m.csc[e];
     }
@@ -259,7 +264,7 @@ tl_status build_model(tl_model &m, std::string &msg) {
             int32_t e = m.row_ptr[i];
             for (int32_t t = 0; t <= nt; ++t) {
                 const int32_t bound = (t == nt) ? INT32_MAX : t * kTile;
-                while (e < m.row_ptr[i + 1] && (m.csr[e] >> 3) < bound) ++e;
+                while (e < m.row_ptr[i + 1] && (m.csr[e] >> 4) < bound) ++e;
                 m.tile_ptr[(size_t)t * m.N + i] = e;
             }
         }

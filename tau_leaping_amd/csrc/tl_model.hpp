@@ -45,12 +45,12 @@ struct tl_model {
     int32_t lnnz = 0;               // entries of csr / csc
     std::vector<int8_t> order;
     std::vector<int32_t> s0, s1;
-    std::vector<int32_t> s2;  // third reactant of order-4 channels; empty unless an order 0, 4 or 5 exists
-    std::vector<int32_t> row_ptr, csr;  // species-major, (channel << 3) | ((v + 2) & 7)
-    std::vector<int32_t> csc;           // channel-major, (species << 3) | ((v + 2) & 7)
+    std::vector<int32_t> s2;  // third reactant of order-4 channels; empty unless an order 0, 4, 5 or 6 exists
+    std::vector<int32_t> row_ptr, csr;  // species-major, (channel << 4) | (v + 3)
+    std::vector<int32_t> csc;           // channel-major, (species << 4) | (v + 3)
     std::vector<int8_t> g_order;        // order[i] for i < M else 0
     std::vector<int32_t> dep_ptr, dep;  // species i -> channels with i as a reactant
-    // [M][4] column entries padded with (0 << 3) | 2 (v = 0); empty unless
+    // [M][4] column entries padded with (0 << 4) | 3 (v = 0); empty unless
     // every column has at most 4 entries
     std::vector<int32_t> colq;
     // channel j -> channels with a reactant among j's column species (the

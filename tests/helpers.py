@@ -43,7 +43,7 @@ def ks_ok(a, b, alpha=0.01):
 
 
 def trimolecular_net():
-    """Channels with three reactant molecules (Sa+Sb+Sc, 2Sa+Sb) among
+    """Channels with three reactant molecules (Sa+Sb+Sc, 2Sa+Sb, 3S) among
     sources, decays and their reverse reactions."""
     from tau_leaping_amd.models import Network, Reaction
     rx = [Reaction({0: 1, 1: 1, 2: 1}, {3: 1}, 2e-6), Reaction({3: 1}, {0: 1, 1: 1, 2: 1}, 0.2),
@@ -51,7 +51,7 @@ def trimolecular_net():
           Reaction({5: 1, 6: 1, 7: 1}, {5: 2}, 1e-5), Reaction({5: 1}, {6: 1, 7: 1}, 0.05),
           Reaction({}, {0: 1}, 20.0), Reaction({0: 1}, {}, 0.02), Reaction({}, {1: 1}, 15.0),
           Reaction({1: 1}, {}, 0.02), Reaction({}, {2: 1}, 10.0), Reaction({2: 1}, {}, 0.02),
-          Reaction({6: 2, 7: 1}, {}, 1e-7)]
+          Reaction({6: 2, 7: 1}, {}, 1e-7), Reaction({6: 3}, {7: 1}, 1e-6)]
     return Network(8, rx, np.array([600, 500, 400, 10, 5, 50, 300, 200], np.int32))
 
 

@@ -80,8 +80,8 @@ def test_net_stoichiometry_layout_matches_oracle():
 
 
 def test_net_stoichiometry_range_error():
-    # two product entries of one species: 5 + 5 leaves the 3-bit range once merged
-    m = T.Model.from_csc(2, [0, 3], [0, 1, 1], [-1, 5, 5], [1.0], [10, 0])
+    # two product entries of one species: 7 + 7 leaves the 4-bit range [-3, 12] once merged
+    m = T.Model.from_csc(2, [0, 3], [0, 1, 1], [-1, 7, 7], [1.0], [10, 0])
     with pytest.raises(T.TLError) as e:
         m.net_stoichiometry = True
     assert e.value.status == 4
@@ -103,9 +103,13 @@ def test_validation_status_codes():
     assert e.value.status == 3
     order, s0, s1, _, _ = mk([Reaction({}, {0: 1}, 1.0), Reaction({0: 1}, {}, 1.0)]).structure()
     assert list(order) == [0, 1]  # a source is order 0
+    assert list(mk([Reaction({0: 3}, {1: 1}, 1.0)]).structure()[0]) == [6]  # 3S
     with pytest.raises(T.TLError) as e:
-        mk([Reaction({0: 3}, {1: 1}, 1.0)])
+        mk([Reaction({0: 1}, {1: 13}, 1.0)])  # stoichiometry 13 does not pack in 4 bits
     assert e.value.status == 4
+    with pytest.raises(T.TLError) as e:
+        mk([Reaction({0: 4}, {1: 1}, 1.0)])  # four molecules of one species
+    assert e.value.status in (3, 4)
     with pytest.raises(T.TLError) as e:
         mk([Reaction({0: 1}, {1: 1}, -1.0)])
     assert e.value.status == 9

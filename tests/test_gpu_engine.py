@@ -324,11 +324,11 @@ def test_exact_sampler_law(cuda):
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("mode,bs", [("persistent", 64), ("persistent", 256), ("persistent", 1024), ("plain", 0)])
 def test_three_reactant_channels_match_oracle(cuda, prec, mode, bs):
-    # Sa+Sb+Sc and 2Sa+Sb channels (the extended propensity formulas and
-    # their third reactant) in every form
+    # Sa+Sb+Sc, 2Sa+Sb and 3S channels (the extended propensity formulas
+    # and their third reactant) in every form
     from helpers import trimolecular_net
     m, om = pair(trimolecular_net())
-    assert list(m.structure()[0][:6]) == [4, 1, 5, 1, 4, 1]
+    assert list(m.structure()[0][:6]) == [4, 1, 5, 1, 4, 1] and m.structure()[0][-1] == 6
     R, tf = 1024, 2.0
     res = m.simulate(tf, R, seed=808, precision=prec, mode=mode, block_size=bs)
     ora = om.simulate(R, tf, seed=808, precision=prec)

@@ -189,7 +189,7 @@ def cyclic1024_sensitivity_check(R=16384, R_fd=262144, n_fd=8):
     fd_z = np.abs(lrv - fd) / np.hypot(lrse, fd_se)
     return dict(realizations=R, closed_form_rel_err=float(cf_rel), closed_form_entries=int(big.sum()),
                 closed_form_max_z=float(cf_z.max()), fd_realizations=R_fd, fd_rel_err=float(fd_rel),
-                fd_max_z=float(fd_z.max()), fd_entries=int(fd.size),
+                fd_max_z=float(fd_z.max()), fd_mean_z2=float(np.mean(fd_z ** 2)), fd_entries=int(fd.size),
                 fd_rel_noise=float(np.linalg.norm(np.hypot(lrse, fd_se)) / np.linalg.norm(fd)),
                 fd_vs_closed_form_rel=float(np.linalg.norm(fd - ex) / np.linalg.norm(ex)),
                 grad_kernel_ms=res.kernel_ms)
@@ -202,6 +202,10 @@ def test_cyclic1024_sensitivities_full_size(cuda):
     assert r["closed_form_entries"] >= 8
     assert r["closed_form_rel_err"] < 0.05, r
     assert r["closed_form_max_z"] < 4.5, r
-    # against finite differences of the same process, only noise remains
-    assert r["fd_rel_err"] < 0.05, r
+    # against finite differences of the same process, only noise remains:
+    # the z-scores of the 32 entries are those of noise (mean z^2 near 1),
+    # and the relative error stays within the 5 % bar or, where the
+    # estimates' own noise is near 5 %, within 1.5 times that noise
     assert r["fd_max_z"] < 4.5, r
+    assert r["fd_mean_z2"] < 2.0, r
+    assert r["fd_rel_err"] < max(0.05, 1.5 * r["fd_rel_noise"]), r

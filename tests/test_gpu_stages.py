@@ -5,7 +5,7 @@ Deterministic stages use the same operation order as the oracle, so the
 bar is: propensities, criticality, tau1 / argmin, leap selection and the
 integer update exact; a0 within 1e-12 relative (FP64) / 1e-6 (FP32 values
 summed in FP64); firing counts identical except where a last-ulp difference
-of the device exp/log flips a Knuth comparison (at most 1e-4 of entries).
+of the device exp/log flips an inversion comparison (at most 1e-4 of entries).
 """
 import numpy as np
 import pytest
@@ -119,8 +119,8 @@ def test_direct_events(cuda, name, prec):
 
 
 def test_poisson_sampler_stream(cuda):
-    """Fire on a single-channel model exercises the Knuth (< 80) and normal
-    (>= 80) branches of one stream against the oracle's draws."""
+    """Fire on a single-channel model exercises the inversion (< 80) and
+    normal (>= 80) branches of one stream against the oracle's draws."""
     from tau_leaping_amd.models import Network, Reaction
 
     net = Network(1, [Reaction({0: 1}, {}, 1.0)], np.array([100], np.int32))

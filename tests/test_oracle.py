@@ -5,7 +5,7 @@ The reference ships no tests or golden outputs, so the oracle is pinned by:
     independent implementation), the dimer-decay stage values K1/K2;
   * exact rational arithmetic on random tiny models built from the reaction
     lists (not from the oracle's packed layouts);
-  * distributions: the Knuth branch against scipy's Poisson pmf, the normal
+  * distributions: the inversion branch against scipy's Poisson pmf, the normal
     branch against the exact law of floor|N(m, m)|;
   * closed forms: direct-method trajectories of the linear cyclic chain
     against exp(T L), and likelihood-ratio sensitivities against the
@@ -104,7 +104,7 @@ def _random_tiny(seed, with_catalyst=True, with_sources=True):
     M = int(rng.integers(4, 9))
     rx = []
     for _ in range(M):
-        t = int(rng.integers(0, 6)) if with_sources else int(rng.integers(1, 4))
+        t = int(rng.integers(0, 7)) if with_sources else int(rng.integers(1, 4))
         sp = rng.permutation(N)
         if t == 0:
             reac = {}
@@ -112,6 +112,8 @@ def _random_tiny(seed, with_catalyst=True, with_sources=True):
             reac = {int(sp[0]): 1, int(sp[1]): 1, int(sp[2]): 1}
         elif t == 5:  # 2Sa + Sb
             reac = {int(sp[0]): 2, int(sp[1]): 1}
+        elif t == 6:  # 3S
+            reac = {int(sp[0]): 3}
         elif t == 1:
             reac = {int(sp[0]): 1}
         elif t == 2:
@@ -144,15 +146,18 @@ def _exact_prop(rx, x):
         (i, _), (j, _) = items
         return c * x[i] * x[j], x[i] < 5 or x[j] < 5
     (i, v), = items
+    if v == 3:
+        return c * x[i] * (x[i] - 1) * (x[i] - 2) / 6, 3 * x[i] < 5
     if v == 2:
         return c * x[i] * (x[i] - 1) / 2, 2 * x[i] < 5
     return c * x[i], x[i] < 5
 
 
+@pytest.mark.parametrize("net_stoich", [False, True], ids=["as_read", "net"])
 @pytest.mark.parametrize("seed", range(12))
-def test_stages_exact_rational(seed):
+def test_stages_exact_rational(seed, net_stoich):
     net = _random_tiny(seed)
-    om = oracle_of(net)
+    om = O.OracleModel(net.n_species, net.n_channels, *net.to_csc(), net_stoichiometry=net_stoich)
     rng = np.random.default_rng(100 + seed)
     R = 16
     x = rng.integers(0, 12, size=(net.n_species, R)).astype(np.int32)
@@ -166,7 +171,8 @@ def test_stages_exact_rational(seed):
             assert crit[j, r] == int(cj)
         assert a0[r] == pytest.approx(float(sum(e[0] for e in ex)), rel=1e-15)
         # mean / variance bound per species from the reaction lists; a species
-        # on both sides contributes a reactant and a product entry
+        # on both sides contributes a reactant and a product entry, or with
+        # net stoichiometry one entry of their difference
         best = None
         for i in range(net.n_species):
             mu = Fraction(0)
@@ -180,6 +186,8 @@ def test_stages_exact_rational(seed):
                     vs.append(-rx.reactants[i])
                 if i in rx.products:
                     vs.append(rx.products[i])
+                if net_stoich:
+                    vs = [sum(vs)]
                 for v in vs:
                     mu += v * aj
                     sig += v * v * aj
@@ -194,13 +202,15 @@ def test_stages_exact_rational(seed):
             elif len(kinds) == 2:
                 order = 5 if 2 in kinds else 2
             else:
-                order = 3 if kinds[0] == 2 else 1
-            g = {0: Fraction(1), 1: Fraction(1), 2: Fraction(2), 3: None, 4: Fraction(3), 5: None}[order]
+                order = {1: 1, 2: 3, 3: 6}[kinds[0]]
+            g = {0: Fraction(1), 1: Fraction(1), 2: Fraction(2), 3: None, 4: Fraction(3), 5: None, 6: None}[order]
             xi = xr[i]
             if order == 3:
                 g = None if xi == 1 else Fraction(2) + Fraction(1, xi - 1)
             if order == 5:
                 g = None if xi == 1 else Fraction(3, 2) * (Fraction(2) + Fraction(1, xi - 1))
+            if order == 6:  # Cao-Gillespie-Petzold's g for 3S
+                g = None if xi in (1, 2) else Fraction(3) + Fraction(1, xi - 1) + Fraction(2, xi - 2)
             q = Fraction(0) if g is None else eps * xi / g
             tmp = max(q, Fraction(1))
             t1 = None if mu == 0 else tmp / abs(mu)
@@ -258,7 +268,7 @@ def test_select_is_inverse_cdf():
 # ----------------------------------------------------------- sampler laws --
 @pytest.mark.parametrize("mean", [0.3, 2.5, 11.0, 37.0, 79.5])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_knuth_branch_is_poisson(mean, prec):
+def test_inversion_branch_is_poisson(mean, prec):
     om = oracle_of(models.dimer_decay())
     n = 20000
     k = np.array([om.poisson(mean, 11, rid=e % 97, epoch=e, channel=e % 5, precision=prec) for e in range(n)])
@@ -338,28 +348,35 @@ def _death_chain_law(rates, t):
     return linalg.expm(t * Q)[0]
 
 
-@pytest.mark.parametrize("kind", ["abc", "2ab"])
+@pytest.mark.parametrize("kind", ["abc", "2ab", "3a"])
 def test_three_reactant_law(kind):
     """Channels with three reactant molecules in the direct-method regime,
     against the chemical master equation solved exactly: A + B + C -> 0 at
-    rate k a b c from (6, 6, 6), and 2A + B -> 0 at rate k a(a-1)/2 b from
-    (8, 5); the number of events by time t has the law exp(tQ) of a death
-    chain.  Chi-square over that law (an error in either propensity formula
-    changes the rates and fails it)."""
+    rate k a b c from (6, 6, 6), 2A + B -> 0 at rate k a(a-1)/2 b from
+    (8, 5), and 3A -> 0 at rate k a(a-1)(a-2)/6 from 12; the number of
+    events by time t has the law exp(tQ) of a death chain.  Chi-square over
+    that law (an error in a propensity formula changes the rates and fails
+    it)."""
     if kind == "abc":
         k, t = 0.004, 1.0
         net = Network(3, [Reaction({0: 1, 1: 1, 2: 1}, {}, k)], np.array([6, 6, 6], np.int32))
         rates = [k * (6 - m) ** 3 for m in range(6)]
-    else:
+    elif kind == "2ab":
         k, t = 0.05, 1.0
         net = Network(2, [Reaction({0: 2, 1: 1}, {}, k)], np.array([8, 5], np.int32))
         rates = [k * (8 - 2 * m) * (7 - 2 * m) / 2 * (5 - m) for m in range(4)]
+    else:
+        k, t = 0.02, 1.0
+        net = Network(1, [Reaction({0: 3}, {}, k)], np.array([12], np.int32))
+        rates = [k * (12 - 3 * m) * (11 - 3 * m) * (10 - 3 * m) / 6 for m in range(4)]
     om = oracle_of(net)
     res = om.simulate(20000, t, seed=17)
     assert (res["status"] == 0).all()
     events = net.x0[0] - res["x_final"][:, 0]
     if kind == "2ab":
         events //= 2
+    if kind == "3a":
+        events //= 3
     law = _death_chain_law(rates, t)
     obs = np.bincount(events, minlength=law.size).astype(float)
     keep = law * events.size >= 5
@@ -530,8 +547,9 @@ def test_model_validation():
         build([Reaction({0: 2, 1: 2}, {}, 1.0)])  # 2S + 2S'
     with pytest.raises(ValueError, match="status 3"):
         build([Reaction({0: 1, 1: 1, 2: 1}, {}, 1.0)] + [Reaction({0: 2, 2: 2}, {}, 1.0)])
+    assert list(build([Reaction({0: 3}, {}, 1.0)]).order) == [6]  # 3S
     with pytest.raises(ValueError, match="status 4"):
-        build([Reaction({0: 3}, {}, 1.0)])  # stoichiometry -3 does not pack in 3 bits
+        build([Reaction({0: 1}, {1: 13}, 1.0)])  # stoichiometry 13 does not pack in 4 bits
     cat = Network(2, [Reaction({0: 1}, {0: 1, 1: 1}, 1.0)], np.array([5, 0], np.int32))
     col_ptr, row_ind, stoich, _, _ = cat.to_csc()
     assert row_ind.tolist() == [0, 0, 1] and stoich.tolist() == [-1, 1, 1]
@@ -558,8 +576,8 @@ def test_ring_jacobian_quadrature_matches_frechet():
 
 def test_fast_sampler_is_ptrs_from_10():
     # the 'fast' option draws means in [10, 80) by PTRS on the channel's
-    # stream (PTRS is pinned by test_ptrs_is_poisson), and keeps the Knuth
-    # product below 10 and the normal approximation from 80
+    # stream (PTRS is pinned by test_ptrs_is_poisson), and keeps the
+    # inversion below 10 and the normal approximation from 80
     om = oracle_of(models.dimer_decay(100))
     for mean in (0.7, 4.0, 9.99):
         for rid in range(50):

@@ -34,6 +34,11 @@ restatement of the oracle's code:
     out of the bound and S2 adds a2.  tau1, its species and its branch are
     fixed in closed form for 256 states; planted mistakes in the merge (no
     merge, summed magnitudes, last entry only) each break the net pin.
+  * the firing count below mean 80 (sequential-search inversion of the
+    channel's first uniform): every draw equals the point whose interval of
+    scipy's cumulative Poisson pmf holds the uniform; planted mistakes (a
+    wrong recursion, a cumulative sum without P(0), a wrong P(0)) each break
+    it.
 """
 import math
 import os
@@ -351,13 +356,13 @@ def test_catalyst_net_layout_idempotent():
     assert np.array_equal(np.ctypeslib.as_array(m.csr_packed, (int(om.row_ptr[-1]),)), om.csr_packed)
     # species 0 has no entry; species 2 one entry of -1 in channel 2
     assert list(om.row_ptr) == [0, 0, 2, 3]
-    assert om.csr_packed[-1] == (2 << 3) | (-1 + 2)
+    assert om.csr_packed[-1] == (2 << 4) | (-1 + 3)
 
 
 PLANTED_NET = {
     "no_merge": (r"(int tlo_model_set_net_stoichiometry\(tlo_model \*m\) \{)", r"\1\n    return TLO_OK;"),
-    "magnitudes": (r"v \+= \(m->csr_packed\[e\] & 7\) - 2;", r"v += abs((m->csr_packed[e] & 7) - 2);"),
-    "last_entry_only": (r"v \+= \(m->csr_packed\[e\] & 7\) - 2;", r"v = (m->csr_packed[e] & 7) - 2;"),
+    "magnitudes": (r"v \+= \(m->csr_packed\[e\] & 15\) - 3;", r"v += abs((m->csr_packed[e] & 15) - 3);"),
+    "last_entry_only": (r"v \+= \(m->csr_packed\[e\] & 15\) - 3;", r"v = (m->csr_packed[e] & 15) - 3;"),
 }
 
 
@@ -366,4 +371,55 @@ def test_catalyst_pin_catches_planted_mistakes(mistake, tmp_path):
     pat, rep = PLANTED_NET[mistake]
     r = run_planted(tmp_path, "tl_oracle.c", pat, rep, "test_catalyst_tau_closed_form[net-fp64]")
     assert r.returncode != 0, f"the closed form did not catch {mistake}:\n{r.stdout[-2000:]}"
+    assert "1 failed" in r.stdout, r.stdout[-2000:]
+
+
+# ------------------------------------------------- inversion below mean 80 --
+def head_uniform(seed, rid, epoch, channel, model_id=0, prec="fp64"):
+    """Channel `channel`'s first uniform at `epoch`: output channel & 3 of the
+    block at counter (channel >> 2, 0, epoch, model_id << 2 | 3)."""
+    o = philox([channel >> 2, 0, epoch, (model_id << 2) | 3], [seed & 0xFFFFFFFF, ((seed >> 32) + rid) & 0xFFFFFFFF])
+    b = o[channel & 3]
+    return (b + 0.5) * 2.0 ** -32 if prec == "fp64" else ((b >> 9) + 0.5) * 2.0 ** -23
+
+
+def inversion_by_pmf(mean, cap=300):
+    """(support points 0..cap, their cumulative probabilities) from scipy's
+    Poisson pmf."""
+    ks = np.arange(cap + 1)
+    return ks, np.cumsum(stats.poisson.pmf(ks, mean))
+
+
+@pytest.mark.parametrize("mean", [0.3, 0.97, 2.5, 7.3, 23.7, 61.2, 79.6])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_inversion_matches_pmf_search(mean, prec):
+    # every draw is the support point whose cumulative interval holds the
+    # channel's first uniform (draws within 1e-9 (FP64) / 1e-5 (FP32) of an
+    # interval end are not compared)
+    om = oracle_of(Network(1, [Reaction({0: 1}, {}, 1.0)], np.array([1], np.int32)))
+    order, F = inversion_by_pmf(mean)
+    tol = 1e-9 if prec == "fp64" else 1e-5
+    n_cmp = 0
+    for rid in range(1500):
+        u = head_uniform(77, rid, 5, 2, prec=prec)
+        i = int(np.searchsorted(F, u, side="right"))
+        if min(abs(u - F[max(i - 1, 0)]), abs(F[min(i, len(F) - 1)] - u)) < tol:
+            continue
+        assert om.poisson(mean, 77, rid, 5, 2, precision=prec) == order[i], (rid, u)
+        n_cmp += 1
+    assert n_cmp > 1400
+
+
+PLANTED_INV = {
+    "divide_by_k_plus_1": (r"p = \(p \* mean\) / \(REAL\)k;", r"p = (p * mean) / (REAL)(k + 1);"),
+    "cumulative_without_p0": (r"REAL F = p;\n    int32_t k = 0;", r"REAL F = 0;\n    int32_t k = 0;"),
+    "exp_of_half_mean": (r"REAL p = R_EXP\(-mean\);", r"REAL p = R_EXP(-mean / 2);"),
+}
+
+
+@pytest.mark.parametrize("mistake", sorted(PLANTED_INV))
+def test_inversion_pin_catches_planted_mistakes(mistake, tmp_path):
+    pat, rep = PLANTED_INV[mistake]
+    r = run_planted(tmp_path, "tl_oracle_real.h", pat, rep, "test_inversion_matches_pmf_search[fp64-0.97]")
+    assert r.returncode != 0, f"the pin did not catch {mistake}:\n{r.stdout[-2000:]}"
     assert "1 failed" in r.stdout, r.stdout[-2000:]

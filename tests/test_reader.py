@@ -232,7 +232,7 @@ def test_large_generated_model(tmp_path):
 
 def test_extended_forms_round_trip(tmp_path):
     """Sources (an empty Reactants list) and channels with three reactant
-    molecules read back as orders 0, 4 and 5."""
+    molecules read back as orders 0, 4, 5 and 6."""
     from helpers import trimolecular_net
 
     net = trimolecular_net()
@@ -240,4 +240,4 @@ def test_extended_forms_round_trip(tmp_path):
     p.write_text(net.to_stochkit_xml())
     m = assert_same(p)
     order = list(m.structure()[0])
-    assert order[:7] == [4, 1, 5, 1, 4, 1, 0] and order[-1] == 5
+    assert order[:7] == [4, 1, 5, 1, 4, 1, 0] and order[-2:] == [5, 6]
