@@ -25,7 +25,7 @@ cost ratio); the 31,250-channel colloid model (persistent FP64 / FP32 and a
 plain sample); the heterogeneous 64-model batch in one persistent launch
 against the plain per-model loop; large-model ingestion (generator and
 reader at 1e6 channels, the HBM-state persistent kernel on 256 realizations,
-and the default mode on 4096, where a timed trial picks the form).
+and the default mode on 4096, which runs it in the plain form).
 `--workload colloid` and `--workload large` time the 31,250-channel colloid
 and that 1e6-channel model instead of the toy network.
 """
@@ -40,7 +40,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "leaps/s (realizations×steps) batched vs plain at fixed CI; grad/fwd cost ratio"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_persistent_toy128x256_fp64_traffic.json")
+TRAFFIC_FILES = [os.path.join(ROOT, "profiles", n) for n in
+                 ("k_persistent_toy128x256_fp64_traffic.json", "k_persistent_hbm_toy200000x1000000_fp64_traffic.json")]
 ISSUE_FILE = os.path.join(ROOT, "profiles", "k_persistent_toy128x256_fp64_issue.json")
 
 
@@ -54,16 +55,18 @@ def bytes_per_leap(N, M, fp64=True, sens=False):
 
 
 def measured_traffic(workload):
-    """HBM bytes per launch from the committed counter summary, if it was
-    taken on this workload."""
-    try:
-        with open(TRAFFIC_FILE) as f:
-            t = json.load(f)
-    except (OSError, ValueError):
-        return None
-    if t.get("workload") != workload:
-        return None
-    return float(t["fetch_kib"]) * 1024 * float(t["fetch_scale"]) + float(t["write_kib"]) * 1024 * float(t["write_scale"])
+    """HBM bytes per launch from the committed counter summaries (one JSON
+    object per workload), if one was taken on this workload."""
+    for path in TRAFFIC_FILES:
+        try:
+            with open(path) as f:
+                t = json.load(f)
+        except (OSError, ValueError):
+            continue
+        if t.get("workload") == workload:
+            return (float(t["fetch_kib"]) * 1024 * float(t["fetch_scale"]) +
+                    float(t["write_kib"]) * 1024 * float(t["write_scale"]))
+    return None
 
 
 def measured_issue(kernel_name):
@@ -138,9 +141,15 @@ def hetero_run(T, models, np, tf=0.1):
     sb = T.batch_simulate(sub_ms, sub, mode="persistent", want_final=False)
     dsb = time.perf_counter() - t0
     log("hetero matched subset: plain loop")
-    t0 = time.perf_counter()
-    sp = T.batch_simulate(sub_ms, sub, mode="plain", want_final=False)
-    dsp = time.perf_counter() - t0
+    # the plain form runs the models one after another; one call per model
+    # (the same work) so that each model's time is logged as it ends
+    sp, dsp = [], 0.0
+    for i, mm, kk in zip(pick, sub_ms, sub):
+        t0 = time.perf_counter()
+        sp += T.batch_simulate([mm], [kk], mode="plain", want_final=False)
+        dt = time.perf_counter() - t0
+        dsp += dt
+        log(f"  plain model {i}: {dt:.1f} s, {sp[-1].total_steps} leaps")
     lb, lp = sum(r.total_steps for r in sb), sum(r.total_steps for r in sp)
     out["matched_subset"] = {
         "models": pick, "realizations": int(sum(k["n_realizations"] for k in sub)), "sensitivities": True,
@@ -263,7 +272,7 @@ def extras_run(T, models, np, m_toy, kw_toy):
         t_read = time.perf_counter() - t0
     rb = big.simulate(1e-6, 256, seed=1, track=np.arange(4, dtype=np.int32), max_iter=3000, mode="persistent")
     lb = bytes_per_leap(big.n_species, big.n_channels)
-    # the default mode on more realizations: a timed trial picks the form
+    # the default mode on more realizations: a model this large runs in the plain form
     t0 = time.perf_counter()
     ra = big.simulate(1e-6, 4096, seed=1, track=np.arange(4, dtype=np.int32), max_iter=3000)
     t_auto = time.perf_counter() - t0
@@ -272,7 +281,7 @@ def extras_run(T, models, np, m_toy, kw_toy):
         "generate_seconds": t_gen, "read_seconds": t_read, "realizations": 256, "t_final": 1e-6,
         "leaps_per_s": rate(rb), "kernel_ms": rb.kernel_ms, "block_size": rb.block_size, "failed": rb.n_failed,
         "bytes_per_leap": lb, "hbm_frac": lb * rb.total_steps / (rb.kernel_ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
-        "auto_4096": {"form": "plain" if ra.block_size == 256 else "persistent", "seconds_with_trial": t_auto,
+        "auto_4096": {"form": ra.form, "seconds": t_auto,
                       "leaps_per_s": rate(ra), "kernel_ms": ra.kernel_ms, "failed": ra.n_failed,
                       "hbm_frac": lb * ra.total_steps / (ra.kernel_ms * 1e-3) / (HBM_PEAK_GBS * 1e9)}}
     return ex
@@ -290,6 +299,9 @@ def main():
                          "colloid: 250-species colloid (31,250 channels), 4,096 realizations, t_final 0.05; "
                          "large: toy_system_gen --toy network of 200,000 species x 1,000,000 channels, 4,096 "
                          "realizations, t_final 1e-6, default mode (a timed choice of form)")
+    ap.add_argument("--mode", default="auto", choices=["auto", "persistent", "plain"],
+                    help="engine form (auto: the library's choice; persistent with --workload large runs the "
+                         "kernel that keeps each realization's state in HBM)")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=4000)
@@ -335,7 +347,7 @@ def main():
     if net is not None:
         m = T.Model.from_network(net)
     fp64 = args.precision == "fp64"
-    kw = dict(seed=1234, precision=args.precision, track=np.arange(4, dtype=np.int32))
+    kw = dict(seed=1234, precision=args.precision, track=np.arange(4, dtype=np.int32), mode=args.mode)
 
     log(f"warmup x{args.warmup}")
     for _ in range(args.warmup):
@@ -354,17 +366,18 @@ def main():
     B = bytes_per_leap(m.n_species, m.n_channels, fp64)
     avg_ms = float(np.mean(kernel_ms))
     achieved = B * (steps_total / len(res)) / (avg_ms * 1e-3) / 1e9
-    plain_form = res[-1].block_size == 256 and args.workload == "large"
+    plain_form = res[-1].form == "plain"
+    form = "plain form" if plain_form else "persistent engine"
+    mode = "default mode" if args.mode == "auto" else f"mode {args.mode}"
     if args.workload == "colloid":
         workload = (f"colloid 250 species x {net.n_channels} channels, {R} realizations, t_final 0.05, eps 0.03, "
-                    f"{args.precision}, persistent engine")
+                    f"{args.precision}, {form}")
     elif args.workload == "large":
         workload = (f"toy network 200000 species x 1000000 channels (toy_system_gen --toy --seed 3), {R} "
-                    f"realizations, t_final 1e-06, eps 0.03, {args.precision}, default mode "
-                    f"({'plain form' if plain_form else 'persistent engine'} chosen)")
+                    f"realizations, t_final 1e-06, eps 0.03, {args.precision}, {mode} ({form})")
     else:
         workload = (f"toy network 128 species x 256 channels, {R} realizations, t_final 0.1, eps 0.03, "
-                    f"{args.precision}, persistent engine")
+                    f"{args.precision}, {form}")
     line = {
         "metric": METRIC,
         "value": value,
@@ -389,7 +402,8 @@ def main():
                      if (args.workload == "toy" and fp64 and res[-1].block_size == 64) else None},
     }
     if not args.no_extras and args.workload == "toy":
-        line["extras"] = extras_run(T, models, np, m, dict(n_realizations=R, kw=kw))
+        line["extras"] = extras_run(T, models, np, m,
+                                    dict(n_realizations=R, kw={k: v for k, v in kw.items() if k != "mode"}))
     if not args.no_cpu and args.workload == "toy":
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(net, args.cpu_sample, tf, 1234)

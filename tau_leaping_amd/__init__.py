@@ -30,6 +30,7 @@ __all__ = ["Model", "SimResult", "simulate", "batch_simulate", "mean_final_state
 FP64, FP32 = 0, 1
 MODE_AUTO, MODE_PLAIN, MODE_PERSISTENT = 0, 1, 2
 POISSON_NORMAL, POISSON_EXACT, POISSON_FAST, POISSON_FAST_EXACT = 0, 1, 2, 3
+BLOCK_TUNE = -1  # block_size: a timed choice among the launch shapes, cached per size class
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtau_leaping_amd.so")
