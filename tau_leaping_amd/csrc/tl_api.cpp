@@ -449,6 +449,11 @@ tl_status run_plain(const tl_model *m, const tl_sim_params *p, tl_result *res) {
     TL_TRY(B.alloc(st.clamped, R));
     TL_TRY(B.alloc(st.neg, R));
     TL_TRY(B.alloc(st.n_done, 1));
+    const bool group_direct = m->M >= kPlainDirectGroupM;
+    if (group_direct) {
+        TL_TRY(B.alloc(st.dlist, R));
+        TL_TRY(B.alloc(st.dcount, 1));
+    }
     TL_HIP(hipMemcpyAsync(d_track, trk.data(), T * 4, hipMemcpyHostToDevice, s));
     // sensitivities: scores and dt/dc [M][R], the per-chunk sums of a leap's
     // terms, and the tau stage's arg-min species and bound
@@ -498,8 +503,11 @@ tl_status run_plain(const tl_model *m, const tl_sim_params *p, tl_result *res) {
         launch_prop(is64, dm, R, nchunk, x, a, crit, a0, a0c, part, partc, st.active, s);
         launch_tau(is64, dm, R, p->eps, x, a, crit, tau_i, which_i, tau1, argmin, which, st.active, s, nrc, pbest,
                    pidx, pwhich);
+        if (group_direct) launch_fill_u64(st.dcount, 0ull, 1, s);
         launch_plain_select(is64, dm, R, seed, p->model_id, p->realization_offset, p->t_final, p->max_iter, x, a, crit,
                             a0, a0c, tau1, st, nchunk, partc, s);
+        if (group_direct)
+            launch_plain_direct(is64, dm, R, seed, p->model_id, p->realization_offset, p->t_final, x, a, st, s);
         launch_fire(is64, dm, R, seed, p->model_id, p->realization_offset, st.epoch, a, crit, st.tau, st.jc, k, st.mode,
                     pflags, s);
         launch_update(dm, R, x, k, xn, st.neg, st.mode, s);

@@ -105,7 +105,18 @@ struct PlainState {
     int32_t *next_g = nullptr, *snap = nullptr;
     const int32_t *track = nullptr;
     int32_t n_track = 0, n_intervals = 0;
+    // models with many channels (dlist non-null): the select stage only lists
+    // the realizations that take a direct-method block (dlist [R], dcount
+    // cleared before each select), and launch_plain_direct runs each block
+    // with one workgroup
+    int64_t *dlist = nullptr;
+    unsigned long long *dcount = nullptr;
 };
+// Channel count from which the plain form runs direct-method blocks one
+// workgroup per realization (below it, one thread per realization).
+constexpr int32_t kPlainDirectGroupM = 2048;
+void launch_plain_direct(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase,
+                         double t_final, int32_t *x, void *a, PlainState st, hipStream_t s);
 void launch_plain_init(const DevModel &m, int64_t R, int32_t *x, PlainState st, hipStream_t s);
 void launch_plain_select(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id,
                          int64_t rbase, double t_final, int64_t max_iter, int32_t *x, void *a, const uint8_t *crit,

@@ -787,6 +787,11 @@ __global__ __launch_bounds__(kT) void k_plain_select(DevModel m, int64_t Rn, uin
         const double t_before = t;
         if (st.grid)
             for (int32_t q = 0; q < st.n_track; ++q) st.snap[(int64_t)q * Rn + r] = xtrk(q);
+        if (st.dlist) {  // k_plain_direct runs the block (mode 3: no leap stage touches r)
+            st.mode[r] = 3;
+            st.dlist[atomicAdd(st.dcount, 1ull)] = r;
+            return;
+        }
         uint32_t ep = st.epoch[r];
         double a0s = a0r;
         int64_t n = 0;
@@ -818,6 +823,163 @@ __global__ __launch_bounds__(kT) void k_plain_select(DevModel m, int64_t Rn, uin
     st.jc[r] = jc;
     st.clamped[r] = cl;
     st.mode[r] = 1;
+}
+
+// Direct-method blocks of the realizations listed by k_plain_select, one
+// workgroup per realization (workgroups stride the list): the 100 events of
+// DirectMethodStep (model.cu:427-451, with the fixes of ssa_event_soa), with
+// the channel work of each event spread over the group.  Lane l owns the
+// contiguous channels [l*C, (l+1)*C): a0 is the group scan of the lane sums
+// (a fixed order, so runs repeat bit for bit), the pick walks the one chunk
+// whose prefix passes a0*r1, and after an event only the channels that read
+// a species of the fired column are recomputed.  Every lane computes the
+// same time, epoch and draws; the fired channel comes through LDS.
+constexpr int kDT = 256;
+
+template <class R>
+__global__ __launch_bounds__(kDT) void k_plain_direct(DevModel m, int64_t Rn, uint64_t seed, uint32_t model_id,
+                                                      int64_t rbase, double tf, int32_t *__restrict__ x,
+                                                      R *__restrict__ a, PlainState st) {
+    __shared__ double wtot[kDT / 64];
+    __shared__ int32_t slot[2];
+    const int tid = threadIdx.x, wl = tid & 63, wid = tid >> 6;
+    const unsigned long long nd = *st.dcount;
+    const R *rc = Real<R>::rates(m);
+    const int32_t C = (m.M + kDT - 1) / kDT;
+    const int32_t j0 = min(m.M, tid * C), j1 = min(m.M, j0 + C);
+    const uint32_t c3 = model_id << 2;
+    for (unsigned long long li = blockIdx.x; li < nd; li += gridDim.x) {
+        const int64_t r = st.dlist[li];
+        const uint2 key = rng_key(seed, rbase + r);
+        double t = st.t[r];
+        const double t_before = t;
+        uint32_t ep = st.epoch[r];
+        int64_t n = 0;
+        int32_t jf = -1;
+        for (int s = 0; s < kSsaSteps; ++s) {
+            if (s > 0) {  // propensities after jf fired
+                if (m.aff_ptr) {
+                    const int32_t d1 = m.aff_ptr[jf + 1];
+                    for (int32_t d = m.aff_ptr[jf] + tid; d < d1; d += kDT) {
+                        const int32_t k = m.aff[d];
+                        bool cr;
+                        a[(int64_t)k * Rn + r] = soa_prop<R>(m, k, rc[k], x, Rn, r, cr);
+                    }
+                } else {
+                    for (int32_t j = tid; j < m.M; j += kDT) {
+                        bool cr;
+                        a[(int64_t)j * Rn + r] = soa_prop<R>(m, j, rc[j], x, Rn, r, cr);
+                    }
+                }
+                __syncthreads();
+            }
+            // lane sum of the chunk (loads batched), then the group scan
+            double ps = 0.0;
+            for (int32_t jb = j0; jb < j1; jb += kWalk) {
+                R av[kWalk];
+#pragma unroll
+                for (int u = 0; u < kWalk; ++u) av[u] = jb + u < j1 ? a[(int64_t)(jb + u) * Rn + r] : (R)0;
+#pragma unroll
+                for (int u = 0; u < kWalk; ++u) ps += (double)av[u];
+            }
+            double incl = ps;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const double y = __shfl_up(incl, o, 64);
+                if (wl >= o) incl += y;
+            }
+            double prev = __shfl_up(incl, 1, 64);
+            if (wl == 63) wtot[wid] = incl;
+            if (tid == 0) {
+                slot[0] = 0x7fffffff;
+                slot[1] = -1;
+            }
+            __syncthreads();
+            double base = 0.0, a0s = 0.0;
+#pragma unroll
+            for (int w = 0; w < kDT / 64; ++w) {
+                if (w == wid) base = a0s;
+                a0s += wtot[w];
+            }
+            const double excl = base + (wl == 0 ? 0.0 : prev);
+            incl += base;
+            if (!(a0s > 0.0)) {
+                t = tf;
+                break;
+            }
+            double r1, r2;
+            step_uniforms(key, ep, c3 | kKindSsa, r1, r2);
+            ep++;
+            const double dt = -log(r2) / a0s;
+            if (dt >= tf - t) {
+                if (st.S) {  // survival over rem = tf - t: d/dc_j of -a0*rem
+                    const double rem = tf - t;
+                    for (int32_t j = tid; j < m.M; j += kDT) {
+                        const int64_t o = (int64_t)j * Rn + r;
+                        if (rc[j] != (R)0) st.S[o] += a0s * st.D[o] - (double)a[o] * rem / (double)rc[j];
+                    }
+                }
+                t = tf;
+                break;
+            }
+            const double target = a0s * r1;
+            if (incl > target) atomicMin(&slot[0], tid);
+            __syncthreads();
+            const int L = slot[0];
+            if (L == 0x7fffffff) {  // rounding at the end: the last channel with a_j > 0
+                int32_t lp = -1;
+                for (int32_t j = j0; j < j1; ++j)
+                    if (a[(int64_t)j * Rn + r] > (R)0) lp = j;
+                if (lp >= 0) atomicMax(&slot[1], lp);
+            } else if (tid == L) {
+                double cum = excl;
+                int32_t hitj = -1, lp = -1;
+                for (int32_t j = j0; j < j1 && hitj < 0; ++j) {
+                    const R aj = a[(int64_t)j * Rn + r];
+                    if (aj > (R)0) lp = j;
+                    cum += (double)aj;
+                    if (cum > target) hitj = j;
+                }
+                slot[1] = hitj >= 0 ? hitj : lp;  // rounding at the chunk end
+            }
+            __syncthreads();
+            jf = slot[1];
+            if (st.S) {
+                for (int32_t j = tid; j < m.M; j += kDT) {
+                    const int64_t o = (int64_t)j * Rn + r;
+                    if (rc[j] != (R)0) {
+                        st.S[o] -= (double)a[o] * dt / (double)rc[j];
+                        if (j == jf) st.S[o] += 1.0 / (double)rc[j];
+                    }
+                }
+            }
+            if (tid == 0)  // entries of one column may name a species twice: one lane adds them
+                for (int32_t e = m.col_ptr[jf]; e < m.col_ptr[jf + 1]; ++e) {
+                    const int32_t p = m.csc[e];
+                    x[(int64_t)entry_index(p) * Rn + r] += entry_value(p);
+                }
+            t += dt;
+            ++n;
+            __syncthreads();
+        }
+        if (tid == 0) {
+            auto xtrk = [&](int32_t q) { return x[(int64_t)st.track[q] * Rn + r]; };
+            if (st.grid)
+                grid_step(st, r, tf, t_before, t, [&](int32_t q) { return st.snap[(int64_t)q * Rn + r]; }, xtrk);
+            st.t[r] = t;
+            st.epoch[r] = ep;
+            st.ssa[r] += n;
+            st.steps[r] += 1;
+            st.retries[r] = 0;
+            st.override_tau1[r] = -1.0;
+            st.mode[r] = 2;
+            if (t >= tf) {
+                st.active[r] = 0;
+                atomicAdd(st.n_done, 1ull);
+            }
+        }
+        __syncthreads();
+    }
 }
 
 // Accept or reject the leap of each realization (rows i = blockIdx.y...).
@@ -1137,6 +1299,16 @@ void launch_plain_select(bool is64, const DevModel &m, int64_t R, uint64_t seed,
     else
         k_plain_select<float><<<blocks_for(R), kT, 0, s>>>(m, R, seed, model_id, rbase, t_final, max_iter, x,
                                                            (float *)a, crit, a0, a0c, tau1, st, nchunk, partc);
+}
+
+void launch_plain_direct(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase,
+                         double t_final, int32_t *x, void *a, PlainState st, hipStream_t s) {
+    // workgroups stride the list (its length is read on the device)
+    const unsigned g = (unsigned)(R < 2048 ? R : 2048);
+    if (is64)
+        k_plain_direct<double><<<g, kDT, 0, s>>>(m, R, seed, model_id, rbase, t_final, x, (double *)a, st);
+    else
+        k_plain_direct<float><<<g, kDT, 0, s>>>(m, R, seed, model_id, rbase, t_final, x, (float *)a, st);
 }
 
 void launch_plain_commit(const DevModel &m, int64_t R, int32_t *x, const int32_t *x_new, PlainState st,

@@ -136,3 +136,37 @@ def test_net_stoichiometry_matches_oracle(cuda, prec, mode, bs):
     assert np.quantile((np.abs(S - So) / scale).max(axis=1), 0.99) < 1e-8
     # the double-counted catalysts shrink tau: more leaps per realization
     assert lit.steps.mean() > 1.3 * res.steps.mean(), (lit.steps.mean(), res.steps.mean())
+
+
+def hub_net(K=1500):
+    """S0 -> S_k and S_k -> S0 for k = 1..K (2K channels, rates 1), S0 = 100,
+    S_k = 5: S0 carries most of the flux, so tau1 < 10/a0 and the first steps
+    are direct-method blocks; as S0 drains, leaps take over."""
+    from tau_leaping_amd.models import Network, Reaction
+    rx = ([Reaction({0: 1}, {k: 1}, 1.0) for k in range(1, K + 1)] +
+          [Reaction({k: 1}, {0: 1}, 1.0) for k in range(1, K + 1)])
+    return Network(K + 1, rx, np.array([100] + [5] * K, np.int32))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_plain_direct_blocks_by_workgroup(cuda, prec):
+    # 3000 channels: the plain form runs each direct block with one workgroup
+    # (lane chunk sums, group scan, one chunk walked per pick); paths, event
+    # counts, scores and grid points equal the oracle's on identical paths
+    net = hub_net()
+    assert net.n_channels >= 2048
+    m, om = pair(net)
+    R, tf, n = 512, 0.05, 5
+    trk = np.array([0, 5, 9], np.int32)
+    res = m.simulate(tf, R, seed=77, precision=prec, mode="plain", sensitivities=True, return_scores=True,
+                     track=trk, n_intervals=n, return_grid=True)
+    ora = om.simulate(R, tf, seed=77, precision=prec, sensitivities=True, track=trk, n_intervals=n)
+    assert res.form == "plain" and res.n_failed == 0 and (ora["status"] == 0).all()
+    assert ora["ssa_events"].min() >= 100
+    assert abs(res.total_ssa_events - int(ora["ssa_events"].sum())) <= 0.01 * ora["ssa_events"].sum()
+    same = np.all(res.final_x == ora["x_final"][:, trk], axis=1) & (res.steps == ora["steps"])
+    assert same.mean() >= 0.97, same.mean()
+    S, So = res.score[same], ora["score"][same]
+    scale = np.abs(So).max(axis=1, keepdims=True) + 1e-300
+    assert np.quantile((np.abs(S - So) / scale).max(axis=1), 0.99) < 1e-8
+    np.testing.assert_allclose(res.grid[same], ora["grid"][same], rtol=1e-9, atol=1e-9)
