@@ -661,9 +661,10 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
     return out;
 }
 
-// Default shape without timing: one wavefront per realization up to 1024
-// channels while the LDS (or registers) still hold at least 8 such groups per
-// CU, then 256-lane blocks, and 1024 lanes from 8192 channels; propensities
+// Default shape without timing: one wavefront per realization up to the
+// one-wavefront channel limit below while the LDS (or registers) still hold
+// at least 8 such groups per CU, then 256-lane blocks, and 1024 lanes from
+// 8192 channels; propensities
 // in LDS when they fit.  Measured: a 128 x 256 toy network runs 5x faster on
 // one wavefront than on 256-lane blocks, a 300 x 900 one 1.4x; the
 // 1024-species cyclic chain (37 KB of LDS per realization: 4 one-wavefront
@@ -672,15 +673,24 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
 // per CU, but those lose on batches of such models with many direct-method
 // events); the 250-species colloid (31,250 channels) 1.8x faster on 1024-lane
 // blocks than on 512.
-PersistentConfig heuristic(const std::vector<PersistentConfig> &cands, int32_t m_max) {
+//
+// The channel limit of one-wavefront groups depends on the arithmetic: on the
+// heterogeneous batch's models (FP32 with scores, 2048 realizations each)
+// one wavefront is 1.5x / 1.55x / 1.4x faster than 256-lane blocks at 1109 /
+// 1278 / 1411 channels and 1.26x / 1.38x slower at 1674 / 1800; in FP64
+// without scores it is 1.63x faster at 1278 channels and 1.21x slower at
+// 1411.
+int32_t one_wave_channel_limit(bool is64, bool sens) { return (is64 && !sens) ? 1344 : 1536; }
+
+PersistentConfig heuristic(const std::vector<PersistentConfig> &cands, int32_t m_max, int32_t one_wave_limit) {
     auto pick = [&](int b, int g, int min_waves) -> const PersistentConfig * {
         for (auto &c : cands)  // preferred variants come first
             if (c.block == b && c.group == g && c.occ * (b / 64) >= min_waves) return &c;
         return nullptr;
     };
     const PersistentConfig *c = nullptr;
-    if (m_max <= 1024) c = pick(64, 64, 8);
-    if (!c && m_max <= 1024) c = pick(256, 64, 8);
+    if (m_max <= one_wave_limit) c = pick(64, 64, 8);
+    if (!c && m_max <= one_wave_limit) c = pick(256, 64, 8);
     if (!c && m_max <= 8192) c = pick(256, 256, 0);
     if (!c) c = pick(1024, 1024, 0);
     if (!c) c = pick(512, 512, 0);
@@ -855,7 +865,7 @@ tl_status autotune(DeviceContext &dc, bool is64, bool sens, bool ext, bool timed
     }
     auto all = candidates(dc, is64, sens, n_max, m_max, ext);
     if (all.empty()) return set_error(TL_ERR_TOO_LARGE, "model too large for any persistent launch shape");
-    best = heuristic(all, m_max);
+    best = heuristic(all, m_max, one_wave_channel_limit(is64, sens));
     if (best.hbm || !timed) {  // global-memory state shapes are not timed (no trial launches)
         std::lock_guard<std::mutex> g(g_tune_mu);
         g_tuned[key] = best;
@@ -968,8 +978,11 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
     // Size classes: models whose one-wavefront LDS footprint is within a
     // factor of two of each other share a launch, so a small model's groups
     // are not laid out (and their occupancy set) for the largest model of the
-    // batch.  An explicit block size puts every model in one class.
+    // batch; models past the one-wavefront channel limit form classes of
+    // their own (their default shape is a block group).  An explicit block
+    // size puts every model in one class.
     const int bs_req = params[0].block_size;
+    const int32_t ow_limit = one_wave_channel_limit(is64, sens);
     std::map<int, std::vector<int32_t>> classes;
     for (int32_t i = 0; i < n_models; ++i) {
         int cls = 0;
@@ -980,6 +993,7 @@ tl_status run_persistent(const tl_model *const *models, int32_t n_models, const 
                 lim *= 2;
                 ++cls;
             }
+            if (E[i].m->M > ow_limit) cls += 16;
         }
         classes[cls].push_back(i);
     }
@@ -1138,7 +1152,7 @@ tl_status default_shape(DeviceContext &dc, bool is64, bool sens, bool ext, int32
                         PersistentConfig &cfg) {
     const auto all = candidates(dc, is64, sens, n_max, m_max, ext);
     if (all.empty()) return set_error(TL_ERR_TOO_LARGE, "model too large for any persistent launch shape");
-    cfg = heuristic(all, m_max);
+    cfg = heuristic(all, m_max, one_wave_channel_limit(is64, sens));
     return TL_OK;
 }
 
