@@ -268,6 +268,27 @@ __device__ __forceinline__ R invert_below(bool fast) {
 // kInvertCap (the mass beyond is below 1e-60 for means below 80).  It returns
 // 0 without evaluating exp when u < 1 - mean - margin, a bound below the
 // rounded exp(-mean) (the result is the one the full search returns).
+//
+// In FP64 the step P(i) = (P(i-1) * mean) / i, a division by a small integer,
+// takes the reciprocal from a table: with y = RN(1/i), q = RN(x * y) and the
+// exactly computed remainder x - i * q (one FMA), q + (x - i * q) * y rounded
+// once (a second FMA) is the correctly rounded quotient x / i (P. Markstein,
+// IBM J. Res. Dev. 34 (1990) 111-119: y within half an ulp of 1/i and q
+// within an ulp of x/i), so the search keeps the IEEE division's values at
+// about half its instructions (toy network, FP64: 5 % less kernel time).  The
+// loop counter is the same in every active lane, so the table entry is a
+// scalar load, issued one step ahead.  FP32 keeps the division (its table
+// load, not hidden by the cheaper FP32 step, cost as much as it saved).
+struct RecipTable {
+    double d[kInvertCap + 2];
+};
+constexpr RecipTable make_recip_table() {
+    RecipTable t{};
+    for (int i = 1; i <= kInvertCap + 1; ++i) t.d[i] = 1.0 / (double)i;  // IEEE division at compile time
+    return t;
+}
+static __constant__ RecipTable kRecip = make_recip_table();
+
 template <class R>
 __device__ __forceinline__ int32_t poisson_invert(R mean, uint32_t head) {
     const R u = Real<R>::unif(head);
@@ -276,10 +297,23 @@ __device__ __forceinline__ int32_t poisson_invert(R mean, uint32_t head) {
     R p = Real<R>::exp_(-mean);
     R F = p;
     int32_t k = 0;
-    while (!(u < F) && k < kInvertCap) {
-        ++k;
-        p = (p * mean) / (R)k;
-        F += p;
+    if constexpr (sizeof(R) == 8) {
+        double y = kRecip.d[1];  // RN(1/(k + 1))
+        while (!(u < F) && k < kInvertCap) {
+            ++k;
+            const double yk = y;
+            y = kRecip.d[k + 1];
+            const double x = p * mean;
+            const double q = x * yk;
+            p = fma(fma(-(double)k, q, x), yk, q);  // = x / k, correctly rounded
+            F += p;
+        }
+    } else {
+        while (!(u < F) && k < kInvertCap) {
+            ++k;
+            p = (p * mean) / (R)k;
+            F += p;
+        }
     }
     return k;
 }

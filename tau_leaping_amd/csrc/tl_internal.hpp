@@ -114,7 +114,7 @@ struct PlainState {
 };
 // Channel count from which the plain form runs direct-method blocks one
 // workgroup per realization (below it, one thread per realization).
-constexpr int32_t kPlainDirectGroupM = 2048;
+constexpr int32_t kPlainDirectGroupM = 128;
 void launch_plain_direct(bool is64, const DevModel &m, int64_t R, uint64_t seed, uint32_t model_id, int64_t rbase,
                          double t_final, int32_t *x, void *a, PlainState st, hipStream_t s);
 void launch_plain_init(const DevModel &m, int64_t R, int32_t *x, PlainState st, hipStream_t s);

@@ -39,6 +39,11 @@ restatement of the oracle's code:
     scipy's cumulative Poisson pmf holds the uniform; planted mistakes (a
     wrong recursion, a cumulative sum without P(0), a wrong P(0)) each break
     it.
+  * the FP64 inversion step on the GPU divides by the count k through a
+    table of RN(1/k) and two FMAs (Markstein's correction); that this is
+    the IEEE quotient x / k for every k the search reaches is checked here
+    on random and boundary operands with the C library's FMA, and a dropped
+    correction fails it.
 """
 import math
 import os
@@ -423,3 +428,37 @@ def test_inversion_pin_catches_planted_mistakes(mistake, tmp_path):
     r = run_planted(tmp_path, "tl_oracle_real.h", pat, rep, "test_inversion_matches_pmf_search[fp64-0.97]")
     assert r.returncode != 0, f"the pin did not catch {mistake}:\n{r.stdout[-2000:]}"
     assert "1 failed" in r.stdout, r.stdout[-2000:]
+
+
+def _markstein_quotients(x, k, correct=True):
+    """(x * y) corrected by one FMA remainder step, y = RN(1/k), through the C
+    library's correctly rounded fma()."""
+    import ctypes
+    libm = ctypes.CDLL("libm.so.6")
+    libm.fma.restype = ctypes.c_double
+    libm.fma.argtypes = [ctypes.c_double] * 3
+    y = 1.0 / k
+    out = np.empty(x.size)
+    for n, xv in enumerate(x.tolist()):
+        q = xv * y
+        out[n] = libm.fma(libm.fma(-float(k), q, xv), y, q) if correct else q
+    return out
+
+
+def test_markstein_division_by_small_integers():
+    # every k the inversion search reaches (1..300), operands over the range
+    # of P(k-1) * mean (tiny to ~80) plus mantissas near 1 and 2
+    rng = np.random.default_rng(5)
+    for k in range(1, 302):
+        x = np.ldexp(1.0 + rng.random(120), rng.integers(-1000, 8, 120))
+        x = np.concatenate([x, np.nextafter(np.ldexp(1.0, np.arange(-60, 7)), 0.0), np.ldexp(1.0, np.arange(-60, 7)),
+                            np.array([float(k), 3.0 * k, 80.0, 79.99999999999999])])
+        assert np.array_equal(_markstein_quotients(x, k), x / k), k
+
+
+def test_markstein_division_needs_the_correction():
+    # without the remainder step the product x * RN(1/k) is off by an ulp for
+    # some operands: the check above would catch a dropped correction
+    rng = np.random.default_rng(6)
+    x = np.ldexp(1.0 + rng.random(2000), rng.integers(-50, 7, 2000))
+    assert any(not np.array_equal(_markstein_quotients(x, k, correct=False), x / k) for k in (3, 7, 11, 49, 300))
