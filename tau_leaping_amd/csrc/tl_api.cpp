@@ -606,10 +606,16 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
     for (auto &sh : shapes)
         for (int ai = 0; ai < 2; ++ai) {
             const int al = (sh[1] >= 1024) ? ai : 1 - ai;
-            // 512-lane groups: also compiled for two blocks per CU
-            const int wpe512[] = {4, 1};
-            const int *wl = sh[1] == 64 ? (sens ? wpe64s : wpe64) : (sh[1] == 512 ? wpe512 : nullptr);
-            const int nw = sh[1] == 64 ? 3 : (sh[1] == 512 ? 2 : 1);
+            // 512-lane groups: also compiled for two blocks per CU; 256-lane
+            // groups for four per CU (128 VGPRs; the unrestricted build takes
+            // about 160 and fits three): measured on the heterogeneous batch's
+            // 1800-2924-channel models (FP32 with scores) 15-17 % faster, on
+            // the 1024-species cyclic chain 13 % (16 % with scores); five per
+            // CU spilled and gained less
+            const int wpe512[] = {4, 1}, wpe256[] = {4, 1};
+            const int *wl = sh[1] == 64 ? (sens ? wpe64s : wpe64)
+                                        : (sh[1] == 512 ? wpe512 : (sh[1] == 256 ? wpe256 : nullptr));
+            const int nw = sh[1] == 64 ? 3 : (sh[1] == 512 || sh[1] == 256 ? 2 : 1);
             for (int w = 0; w < nw; ++w) {
                 PersistentConfig c{sh[0], sh[1], al != 0, wl ? wl[w] : 1};
                 c.ext = ext;
@@ -691,7 +697,23 @@ PersistentConfig heuristic(const std::vector<PersistentConfig> &cands, int32_t m
     const PersistentConfig *c = nullptr;
     if (m_max <= one_wave_limit) c = pick(64, 64, 8);
     if (!c && m_max <= one_wave_limit) c = pick(256, 64, 8);
-    if (!c && m_max <= 8192) c = pick(256, 256, 0);
+    if (!c && m_max <= 8192) {
+        // 256-lane groups in the preferred layout (a[M] in LDS first): the
+        // four-per-CU register budget where LDS holds three or more such
+        // groups per CU, else the unrestricted build (its extra registers are
+        // free there)
+        const PersistentConfig *first = pick(256, 256, 0);
+        auto variant = [&](int wpe, int min_occ) -> const PersistentConfig * {
+            for (auto &cc : cands)
+                if (first && cc.block == 256 && cc.group == 256 && cc.a_lds == first->a_lds &&
+                    cc.tile_grains == first->tile_grains && cc.wpe == wpe && cc.occ >= min_occ)
+                    return &cc;
+            return nullptr;
+        };
+        c = variant(4, 3);
+        if (!c) c = variant(1, 0);
+        if (!c) c = first;
+    }
     if (!c) c = pick(1024, 1024, 0);
     if (!c) c = pick(512, 512, 0);
     if (!c && !cands.empty()) c = &cands.front();

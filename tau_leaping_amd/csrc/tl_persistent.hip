@@ -1390,6 +1390,7 @@ const void *kernel_ptr(int bs, int gs, bool alds, int wpe, bool hbm) {
     TL_K(256, 64, 5)
     TL_K(64, 64, 6)
     TL_K(256, 64, 6)
+    TL_K(256, 256, 4)
     TL_K(256, 256, 1)
     TL_K(512, 512, 1)
     TL_K(512, 512, 4)

@@ -137,6 +137,27 @@ def test_poisson_sampler_stream(cuda):
     assert np.array_equal(k, k_o)
 
 
+def test_inversion_draws_many_means(cuda):
+    """16,384 FP64 inversion searches (means 0.5 to 80, one draw each, fresh
+    streams): the device's step divides through a reciprocal table with an
+    FMA correction, the oracle divides; the counts agree draw for draw."""
+    from tau_leaping_amd.models import Network, Reaction
+
+    net = Network(1, [Reaction({0: 1}, {}, 1.0)], np.array([100], np.int32))
+    m, om = pair(net)
+    R = 16384
+    means = np.random.default_rng(11).uniform(0.5, 79.999, R)
+    a = np.ascontiguousarray(means.reshape(1, R))
+    crit = np.zeros((1, R), np.uint8)
+    tau = np.ones(R)
+    jc = -np.ones(R, np.int32)
+    epoch = np.arange(R, dtype=np.uint32) * 7 + 3
+    k = _n(T.stage_fire(m, 9, _t(epoch, cuda), _t(a, cuda), _t(crit, cuda), _t(tau, cuda), _t(jc, cuda)))
+    k_o = om.fire(9, epoch, a, crit, tau, jc)
+    assert np.array_equal(k, k_o)
+    assert k.max() > 80  # searches well past the mean
+
+
 def test_stage_waits_for_callers_stream(cuda):
     # the input is produced by torch kernels queued just before the stage
     # call (a long matrix product first, so they are still running when the
