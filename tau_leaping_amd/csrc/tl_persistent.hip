@@ -211,6 +211,20 @@ template <int CTRL, int ROWS = 0xf>
 __device__ __forceinline__ int dpp_i(int v, int ident) {
     return __builtin_amdgcn_update_dpp(ident, v, CTRL, ROWS, 0xf, false);
 }
+// Lane move for sums: lanes whose source is out of range read zero
+// (bound_ctrl) and every row is written, so no identity has to be placed in
+// the destination first (10 fewer VALU instructions per FP64 wave scan).
+template <int CTRL>
+__device__ __forceinline__ int dpp_zero_i(int v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d_zero(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_zero_i<CTRL>((int)(b & 0xffffffffll));
+    const int hi = dpp_zero_i<CTRL>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 template <int CTRL, int ROWS = 0xf>
 __device__ __forceinline__ double dpp_d(double v) {
     const long long b = __double_as_longlong(v);
@@ -231,25 +245,25 @@ __device__ __forceinline__ double dpp_d_ident(double v, double ident) {
 }
 
 __device__ __forceinline__ double wave_incl_scan(double v, int /*wl*/) {
-    v += dpp_d<kDppRowShr | 1>(v);
-    v += dpp_d<kDppRowShr | 2>(v);
-    v += dpp_d<kDppRowShr | 4>(v);
-    v += dpp_d<kDppRowShr | 8>(v);
+    v += dpp_d_zero<kDppRowShr | 1>(v);
+    v += dpp_d_zero<kDppRowShr | 2>(v);
+    v += dpp_d_zero<kDppRowShr | 4>(v);
+    v += dpp_d_zero<kDppRowShr | 8>(v);
     v += dpp_d<kDppBcast15, 0xa>(v);
     v += dpp_d<kDppBcast31, 0xc>(v);
     return v;
 }
 // Value of the previous lane (0.0 in lane 0).
-__device__ __forceinline__ double wave_shift_up(double v) { return dpp_d<kDppWaveShr1>(v); }
+__device__ __forceinline__ double wave_shift_up(double v) { return dpp_d_zero<kDppWaveShr1>(v); }
 
 // Wave total, the same in every lane (scalar register).
 __device__ __forceinline__ double wave_sum(double v) { return readlane(wave_incl_scan(v, 0), 63); }
 
 __device__ __forceinline__ float wave_incl_scan_f(float v) {
-    v += dpp_f<kDppRowShr | 1>(v, 0.f);
-    v += dpp_f<kDppRowShr | 2>(v, 0.f);
-    v += dpp_f<kDppRowShr | 4>(v, 0.f);
-    v += dpp_f<kDppRowShr | 8>(v, 0.f);
+    v += __int_as_float(dpp_zero_i<kDppRowShr | 1>(__float_as_int(v)));
+    v += __int_as_float(dpp_zero_i<kDppRowShr | 2>(__float_as_int(v)));
+    v += __int_as_float(dpp_zero_i<kDppRowShr | 4>(__float_as_int(v)));
+    v += __int_as_float(dpp_zero_i<kDppRowShr | 8>(__float_as_int(v)));
     v += dpp_f<kDppBcast15, 0xa>(v, 0.f);
     v += dpp_f<kDppBcast31, 0xc>(v, 0.f);
     return v;
