@@ -597,9 +597,10 @@ std::vector<PersistentConfig> candidates(const DeviceContext &dc, bool is64, boo
     // one-wavefront groups come in register budgets of 4, 5 and 6 waves per
     // SIMD; the first listed is the default.  Measured on the 128 x 256 toy
     // network without scores: 6 waves 3783 / 3324 ms per launch (FP64 /
-    // FP32), 5 waves 3907 / 3554 ms; with scores (more live values) 5 stays
-    // first.
-    const int wpe64[] = {6, 5, 4}, wpe64s[] = {5, 6, 4};
+    // FP32), 5 waves 3907 / 3554 ms; with scores (more live values) on the
+    // heterogeneous batch, 4 waves 70.5 s per batch against 71.7 s with 5
+    // (same box), and 6 waves 8-27 % slower per model.
+    const int wpe64[] = {6, 5, 4}, wpe64s[] = {4, 5, 6};
     // order within a shape = preference: a[M] in LDS first, except for
     // 1024-lane groups, where recomputed propensities with a large tile win
     // (the 31,250-channel colloid)
