@@ -189,9 +189,10 @@ tl_status tl_simulate(const tl_model *m, const tl_sim_params *p, tl_result *r);
 
 /* Many models, uneven sizes and realization counts.  params[i] / results[i]
  * belong to models[i]; precision and sensitivities must agree across the
- * batch.  PERSISTENT runs every (model, realization) in one work-stealing
- * launch replayed from a hipGraph; PLAIN runs tl_simulate's plain form model
- * after model. */
+ * batch.  PERSISTENT runs every (model, realization) as work items of
+ * persistent work-stealing launches, one per size class of models, the
+ * classes concurrent branches of one cached hipGraph; PLAIN runs
+ * tl_simulate's plain form model after model. */
 tl_status tl_batch_simulate(const tl_model *const *models, int32_t n_models,
                             const tl_sim_params *params, tl_result *results, int32_t mode);
 
