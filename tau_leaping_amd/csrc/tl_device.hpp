@@ -299,11 +299,18 @@ constexpr RecipTable make_recip_table() {
 }
 static __constant__ RecipTable kRecip = make_recip_table();
 
+// True when the count is 0 by that bound alone (no exp, no search).
 template <class R>
-__device__ __forceinline__ int32_t poisson_invert(R mean, uint32_t head) {
+__device__ __forceinline__ bool poisson_zero_by_bound(R mean, uint32_t head) {
     const R u = Real<R>::unif(head);
     const R margin = sizeof(R) == 8 ? (R)9.094947017729282e-13 : (R)9.5367431640625e-07;  // 2^-40 / 2^-20
-    if (mean < (R)1 && u < (R)1 - mean - margin) return 0;
+    return mean < (R)1 && u < (R)1 - mean - margin;
+}
+
+template <class R>
+__device__ __forceinline__ int32_t poisson_invert(R mean, uint32_t head) {
+    if (poisson_zero_by_bound<R>(mean, head)) return 0;
+    const R u = Real<R>::unif(head);
     R p = Real<R>::exp_(-mean);
     R F = p;
     int32_t k = 0;

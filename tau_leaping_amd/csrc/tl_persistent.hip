@@ -923,8 +923,20 @@ __device__ __forceinline__ void prop_update(const GModel &m, gptr<const R> c, co
 // channels, so the Philox block holding their first draws (channels 4b..4b+3
 // share block b) is its own: no staging, no barrier inside the pass.  The
 // caller synchronizes before reading what f wrote.
-// CRIT_A: f needs a_j of critical channels too (else they get 0 and a
-// recomputed propensity is not evaluated for them).
+// CRIT_A (the score kernels): f sees every channel, in channel order, with
+// a_j of critical channels too (else they get 0 and a recomputed propensity
+// is not evaluated for them).  Without it f acts only on k_j != 0 (integer
+// adds, in any order): a lane first settles the channels whose count needs
+// no search (critical ones; those whose head draw lies below the bound of a
+// zero count), then draws the rest, its r-th remaining channel in round r,
+// so a wave runs as many searches as its busiest lane holds instead of one
+// per channel position whenever any lane has one there.  The counts are the
+// same either way.  The one-wavefront FP64 kernels of the default build keep
+// the draw per position: there the compaction's extra live registers cost
+// more than the searches it saves (toy network 128 x 256, 20,000
+// realizations: 706 -> 736 ms; with it the 1024-lane colloid kernel takes
+// 933 -> 847 ms, FP32 one-wavefront 688 -> 675 ms, the PTRS builds 286 ->
+// 251 ms).
 // pflags: kPoissonExact / kPoissonFast, honoured in the EXT build only (the
 // default build carries none of PTRS's registers).
 template <class R, int GS, bool CRIT_A, bool EXT, class P, class F>
@@ -938,13 +950,50 @@ __device__ __forceinline__ void for_each_firing(const GModel &m, const Grp<GS> &
         const int32_t jb = c0 + 4 * G.lane;
         if (jb >= m.M) continue;
         const uint4 h = head_block(key, (uint32_t)(jb >> 2), epoch, c3);
+        const uint32_t cw = (uint32_t)(critq[jb >> 6] >> (jb & 63)) & 15u;  // jb is a multiple of 4
         // recomputed propensities: the four first (their loads overlap)
         R a4[4] = {(R)0, (R)0, (R)0, (R)0};
         if constexpr (!P::kStored) {
-            const uint32_t cw = (uint32_t)(critq[jb >> 6] >> (jb & 63)) & 15u;  // jb is a multiple of 4
 #pragma unroll
             for (int i = 0; i < 4; ++i)
                 a4[i] = (jb + i < m.M && (CRIT_A || !((cw >> i) & 1u))) ? pa.get(jb + i) : (R)0;
+        }
+        if constexpr (!CRIT_A && !(GS == 64 && sizeof(R) == 8 && !EXT)) {
+            // todo: channels that need a draw, and the critical channel that
+            // fires once (its bit also in once)
+            uint32_t todo = 0, once = 0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int32_t j = jb + i;
+                if (j < m.M) {
+                    if ((cw >> i) & 1u) {
+                        if (j == jfire) once = 1u << i;
+                    } else {
+                        R aj;
+                        if constexpr (P::kStored)
+                            aj = pa.get(j);
+                        else
+                            aj = a4[i];
+                        if (!poisson_zero_by_bound<R>(aj * tr, quad_pick(h, i))) todo |= 1u << i;
+                    }
+                }
+            }
+            todo |= once;
+            while (todo) {
+                const int i = __builtin_ctz(todo);
+                todo &= todo - 1;
+                const int32_t j = jb + i;
+                R aj;
+                if constexpr (P::kStored)
+                    aj = pa.get(j);
+                else
+                    aj = i == 0 ? a4[0] : (i == 1 ? a4[1] : (i == 2 ? a4[2] : a4[3]));
+                const bool cr = (once >> i) & 1u;
+                const int32_t kj =
+                    cr ? 1 : poisson_draw<R>(aj * tr, quad_pick(h, i), key, (uint32_t)j, epoch, ck, exact, fast);
+                if (kj) f(j, cr, aj, kj);
+            }
+            continue;
         }
 #pragma unroll 1
         for (int i = 0; i < 4; ++i) {
