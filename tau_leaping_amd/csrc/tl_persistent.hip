@@ -1245,7 +1245,11 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
             step_uniforms(key, epoch, c3 | kKindLeap, r1, r2);
             int32_t jc = m.M;
             double tau2 = (double)kFltMax;
-            if (a0c > 0.0) {
+            // The critical channel is searched for only when its waiting time
+            // t2 is below tau1: otherwise tau = tau1 and nothing fires whatever
+            // the search returns (tau1 <= t2 <= tau2).
+            const double t2 = a0c > 0.0 ? -log(r2) / a0c : (double)kFltMax;
+            if (a0c > 0.0 && t2 < tau1) {
                 int32_t f;
                 if constexpr (GS == 64)
                     f = G.search(pa, critq, true, m.M, a0c * r1);
@@ -1253,7 +1257,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                     f = G.search_critical(pa, critq, m.M, wsum, a0c * r1);
                 if (f >= 0) {
                     jc = f;
-                    tau2 = -log(r2) / a0c;
+                    tau2 = t2;
                 }
             }
             double tau = tau1 < tau2 ? tau1 : tau2;

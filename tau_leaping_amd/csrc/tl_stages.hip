@@ -404,7 +404,10 @@ __device__ void select_leap(const DevModel &m, uint2 key, uint32_t c3, uint32_t 
     step_uniforms(key, epoch, c3 | kKindLeap, r1, r2);
     int32_t jc = m.M;
     double tau2 = (double)kFltMax;
-    if (a0c > 0.0) {
+    // the critical channel is searched for only when its waiting time t2 is
+    // below tau1: otherwise tau = tau1 and nothing fires whatever it is
+    const double t2 = a0c > 0.0 ? -log(r2) / a0c : (double)kFltMax;
+    if (a0c > 0.0 && t2 < tau1) {
         const double target = a0c * r1;
         double cum = 0.0;
         int32_t jstart = 0;
@@ -449,7 +452,7 @@ __device__ void select_leap(const DevModel &m, uint2 key, uint32_t c3, uint32_t 
                 }
             }
         }
-        if (jc != m.M) tau2 = -log(r2) / a0c;
+        if (jc != m.M) tau2 = t2;
     }
     tau = tau1 < tau2 ? tau1 : tau2;
     bool fire = (jc != m.M) && (tau2 < tau1);
