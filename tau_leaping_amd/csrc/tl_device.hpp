@@ -299,7 +299,8 @@ constexpr RecipTable make_recip_table() {
 }
 static __constant__ RecipTable kRecip = make_recip_table();
 
-// True when the count is 0 by that bound alone (no exp, no search).
+// The early exit of poisson_invert: mean < 1 and u < 1 - mean - margin, so
+// the count is 0 without exp or search.
 template <class R>
 __device__ __forceinline__ bool poisson_zero_by_bound(R mean, uint32_t head) {
     const R u = Real<R>::unif(head);
