@@ -939,10 +939,14 @@ __device__ __forceinline__ void prop_update(const GModel &m, gptr<const R> c, co
 // 251 ms).
 // pflags: kPoissonExact / kPoissonFast, honoured in the EXT build only (the
 // default build carries none of PTRS's registers).
+// tile, tile_lo: block groups with recomputed propensities hold a_j of the
+// non-critical channels j >= tile_lo (the last tile of the propensity pass,
+// the same values) in the tile; without CRIT_A they are read there instead
+// of recomputed.  tile_lo = M: no tile.
 template <class R, int GS, bool CRIT_A, bool EXT, class P, class F>
 __device__ __forceinline__ void for_each_firing(const GModel &m, const Grp<GS> &G, const P &pa, const uint64_t *critq,
                                                 uint2 key, uint32_t epoch, uint32_t c3, R tr, int32_t jfire,
-                                                int pflags, F &&f) {
+                                                int pflags, const R *tile, int32_t tile_lo, F &&f) {
     const bool exact = EXT && (pflags & kPoissonExact) != 0;
     const bool fast = EXT && (pflags & kPoissonFast) != 0;
     const uint32_t ck = c3 | kKindPoisson;
@@ -955,8 +959,11 @@ __device__ __forceinline__ void for_each_firing(const GModel &m, const Grp<GS> &
         R a4[4] = {(R)0, (R)0, (R)0, (R)0};
         if constexpr (!P::kStored) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                a4[i] = (jb + i < m.M && (CRIT_A || !((cw >> i) & 1u))) ? pa.get(jb + i) : (R)0;
+            for (int i = 0; i < 4; ++i) {
+                const int32_t j = jb + i;
+                a4[i] = (R)0;
+                if (j < m.M && (CRIT_A || !((cw >> i) & 1u))) a4[i] = (!CRIT_A && j >= tile_lo) ? tile[j - tile_lo] : pa.get(j);
+            }
         }
         if constexpr (!CRIT_A && !(GS == 64 && sizeof(R) == 8 && !EXT)) {
             // todo: channels that need a draw, and the critical channel that
@@ -1075,6 +1082,13 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
     // exposure is flushed when its propensity changes and at the end of the
     // block (O(affected channels) per event instead of O(M)).
     const bool lazy = SENS && GS == 64 && P::kStored && m.aff_ptr != nullptr;
+    // first channel of the propensity pass's last tile (block groups with
+    // recomputed propensities; M: none)
+    int32_t tile_lo = m.M;
+    if constexpr (GS > 64 && !P::kStored) {
+        const int32_t TILE = tile_grains * kTileGrain;
+        if (m.M > 0) tile_lo = ((m.M - 1) / TILE) * TILE;
+    }
 
     for (int32_t i = lane; i < m.N; i += GS) {
         xs[i] = m.x0[i];
@@ -1290,7 +1304,8 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                                            if (c[j] != (R)0) S[j] += term / (double)c[j];
                                        }
                                    };
-            for_each_firing<R, GS, SENS, EXT>(m, G, pa, critq, key, epoch, c3, tr, jfire, pflags, fire_one);
+            for_each_firing<R, GS, SENS, EXT>(m, G, pa, critq, key, epoch, c3, tr, jfire, pflags, tile, tile_lo,
+                                              fire_one);
             sync_();  // dx complete
             bool neg = false;
             for (int32_t i = lane; i < m.N; i += GS) neg |= (xs[i] + dx[i]) < 0;
@@ -1301,7 +1316,8 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
                     auto undo = [&](int32_t j, bool cr, R aj, int32_t kj) {
                         if (c[j] != (R)0) S[j] -= score_term<R>(cr, aj, tr, tau, kj, exact) / (double)c[j];
                     };
-                    for_each_firing<R, GS, SENS, EXT>(m, G, pa, critq, key, epoch, c3, tr, jfire, pflags, undo);
+                    for_each_firing<R, GS, SENS, EXT>(m, G, pa, critq, key, epoch, c3, tr, jfire, pflags, tile,
+                                                      tile_lo, undo);
                 }
                 epoch++;
                 nrej++;
