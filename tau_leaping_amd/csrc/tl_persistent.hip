@@ -259,6 +259,25 @@ __device__ __forceinline__ double wave_shift_up(double v) { return dpp_d_zero<kD
 // Wave total, the same in every lane (scalar register).
 __device__ __forceinline__ double wave_sum(double v) { return readlane(wave_incl_scan(v, 0), 63); }
 
+// Wave totals of x and of y together: the upper 32 lanes of x trade places
+// with the lower 32 of y (v_permlane32_swap), so lane l < 32 holds
+// x_l + x_(l+32) and lane l >= 32 holds y_(l-32) + y_l; a reduction over
+// each half-wave then leaves the x total in lane 31, the y total in lane 63.
+__device__ __forceinline__ double wave_sum_pair_lanes(double x, double y) {
+    const long long bx = __double_as_longlong(x), by = __double_as_longlong(y);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)bx, (unsigned)by, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(bx >> 32), (unsigned)(by >> 32), false, false);
+    const double x2 = __longlong_as_double(((long long)hi[0] << 32) | (unsigned)lo[0]);
+    const double y2 = __longlong_as_double(((long long)hi[1] << 32) | (unsigned)lo[1]);
+    double v = x2 + y2;
+    v += dpp_d_zero<kDppRowShr | 1>(v);
+    v += dpp_d_zero<kDppRowShr | 2>(v);
+    v += dpp_d_zero<kDppRowShr | 4>(v);
+    v += dpp_d_zero<kDppRowShr | 8>(v);
+    v += dpp_d<kDppBcast15, 0xa>(v);
+    return v;
+}
+
 __device__ __forceinline__ float wave_incl_scan_f(float v) {
     v += __int_as_float(dpp_zero_i<kDppRowShr | 1>(__float_as_int(v)));
     v += __int_as_float(dpp_zero_i<kDppRowShr | 2>(__float_as_int(v)));
@@ -828,9 +847,14 @@ __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c,
             const unsigned long long bala = __ballot(cra), balb = __ballot(crb);
             const int32_t ta = j0 + (lane & ~63), tb = ta + GS;
             // each wavefront holds whole 64-channel words: their critical
-            // propensity sums (for the critical search) by wave reduction
-            const double wa = (bala != 0ull) ? wave_sum(cra ? (double)aa : 0.0) : 0.0;
-            const double wb = (balb != 0ull) ? wave_sum(crb ? (double)ab : 0.0) : 0.0;
+            // propensity sums (for the critical search), both words in one
+            // reduction
+            double wa = 0.0, wb = 0.0;
+            if ((bala | balb) != 0ull) {
+                const double v = wave_sum_pair_lanes(cra ? (double)aa : 0.0, crb ? (double)ab : 0.0);
+                wa = readlane(v, 31);
+                wb = readlane(v, 63);
+            }
             if ((lane & 63) == 0) {
                 if (ta < t1) {
                     critq[ta >> 6] = bala;
