@@ -1228,13 +1228,14 @@ void tl_sim_params_default(tl_sim_params *p) {
 
 // AUTO mode: whether a model runs in the plain form.  Always when no
 // persistent launch shape fits; and for a model whose state does not fit LDS
-// when the persistent kernel would keep a[M] in HBM (more than 16 propensity
-// tiles).  Measured on 1e6 channels (62 tiles), all leaps: the plain form
-// (realizations across the lanes, every gather coalesced) 4x faster than the
-// HBM-state persistent kernel; on 1e5 channels (7 tiles) within 15 %, where
-// the persistent kernel stays (a direct-method event costs the plain form an
-// O(M) pass per realization thread).  A pure function of the sizes, the
-// precision and the kernel build: the same call always runs the same form.
+// when the HBM-state persistent kernel would need more than one propensity
+// tile (or keep a[M] in HBM).  Measured on toy networks, R = 2048 / 8192,
+// HBM-state persistent against plain: 12,000 channels (one tile) persistent
+// 1.28x / 1.0x faster; 24,000 (two tiles) plain 1.03x / 1.16x faster,
+// 40,000 1.09x / 1.20x, 60,000 1.14x / 1.30x, 1e5 1.8x, 2e5 2.0x, 1e6 (all
+// leaps) about 7x: realizations across the lanes coalesce every gather.  A
+// pure function of the sizes, the precision and the kernel build: the same
+// call always runs the same form.
 static tl_status auto_plain(const tl_model *m, const tl_sim_params *p, bool &plain) {
     plain = false;
     DeviceContext *dc;
@@ -1245,7 +1246,8 @@ static tl_status auto_plain(const tl_model *m, const tl_sim_params *p, bool &pla
         plain = true;
         return TL_OK;
     }
-    plain = cands.front().hbm && cands.front().a_lds;
+    const PersistentConfig &f = cands.front();
+    plain = f.hbm && (f.a_lds || (int64_t)m->M > (int64_t)f.tile_grains * kTileGrain);
     return TL_OK;
 }
 
