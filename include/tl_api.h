@@ -76,9 +76,9 @@ enum { TL_POISSON_NORMAL = 0, TL_POISSON_EXACT = 1, TL_POISSON_FAST = 2, TL_POIS
  * of HBM per block for models too large for LDS (all models of a batch in one
  * launch); AUTO = PERSISTENT, except for a model no persistent launch shape
  * can run (PERSISTENT returns TL_ERR_TOO_LARGE) and for a model too large for
- * LDS whose persistent shape would keep a[M] in HBM (more than 16 propensity
- * tiles: about 260,000 channels and up), where PLAIN's coalesced
- * realization-major accesses were measured 4x faster: those run PLAIN.  The
+ * LDS whose persistent shape would need more than one propensity tile (16,384
+ * channels for FP64 1024-lane groups) or a[M] in HBM, where PLAIN's coalesced
+ * realization-major accesses were measured 1.03-7x faster: those run PLAIN.  The
  * choice is a pure function of the model sizes and the precision, and
  * tl_result.form reports it.  PLAIN holds about
  * 8 N + (5 + p) M bytes of device memory per realization (p = 8 for FP64, 4
