@@ -986,7 +986,8 @@ __device__ __forceinline__ void for_each_firing(const GModel &m, const Grp<GS> &
             for (int i = 0; i < 4; ++i) {
                 const int32_t j = jb + i;
                 a4[i] = (R)0;
-                if (j < m.M && (CRIT_A || !((cw >> i) & 1u))) a4[i] = (!CRIT_A && j >= tile_lo) ? tile[j - tile_lo] : pa.get(j);
+                if (j < m.M && (CRIT_A || !((cw >> i) & 1u)))
+                    a4[i] = (!CRIT_A && j >= tile_lo) ? tile[j - tile_lo] : pa.get(j);
             }
         }
         if constexpr (!CRIT_A && !(GS == 64 && sizeof(R) == 8 && !EXT)) {
