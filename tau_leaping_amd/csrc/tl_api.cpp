@@ -115,7 +115,7 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     }
     const double t0 = now_s();
     // one allocation: c64 | c32 | s0 | s1 | row_ptr | csr | col_ptr | csc | x0 | order | g_order | rs |
-    // dep_ptr | dep | colq | aff_ptr | aff | tile_ptr | aff_rs | aff_c64 | aff_c32 | s2
+    // dep_ptr | dep | colq | aff_ptr | aff | tile_ptr | aff_rs | aff_c64 | aff_c32 | s2 | row_order
     const size_t M = (size_t)m.M, N = (size_t)m.N, Z = (size_t)std::max(m.lnnz, 1);
     std::vector<size_t> off;
     size_t total = 0;
@@ -147,7 +147,9 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     add(NA * 8);  // aff_c64
     add(NA * 4);  // aff_c32
     add(std::max<size_t>(m.s2.size(), 1) * 4);  // s2 (three-reactant channels)
+    add(N * 4);  // row_order
     std::vector<unsigned char> host(total, 0);
+    bool rows_uneven = false;
     std::vector<float> c32(M);
     for (size_t j = 0; j < M; ++j) c32[j] = (float)m.rates[j];
     memcpy(&host[off[0]], m.rates.data(), M * 8);
@@ -167,6 +169,18 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
             rs[2 * j] = m.s0[j] | ((int32_t)m.order[j] << 29);
             rs[2 * j + 1] = m.s1[j];
         }
+    }
+    {
+        // species rows longest first: the block groups' row gather gives each
+        // pass's lanes rows of similar length, so a wavefront waits less on
+        // its longest row (each row is still summed by the same lanes in the
+        // same order)
+        int32_t *ro = reinterpret_cast<int32_t *>(&host[off[22]]);
+        for (size_t i = 0; i < N; ++i) ro[i] = (int32_t)i;
+        std::stable_sort(ro, ro + N, [&](int32_t a, int32_t b) {
+            return m.row_ptr[a + 1] - m.row_ptr[a] > m.row_ptr[b + 1] - m.row_ptr[b];
+        });
+        rows_uneven = N > 0 && m.row_ptr[ro[0] + 1] - m.row_ptr[ro[0]] != m.row_ptr[ro[N - 1] + 1] - m.row_ptr[ro[N - 1]];
     }
     memcpy(&host[off[12]], m.dep_ptr.data(), (N + 1) * 4);
     if (!m.dep.empty()) memcpy(&host[off[13]], m.dep.data(), m.dep.size() * 4);
@@ -231,6 +245,7 @@ tl_status device_model(const tl_model &m, int device, DevModel &out, double *upl
     d.aff_c64 = (const double *)(b + off[19]);
     d.aff_c32 = (const float *)(b + off[20]);
     d.s2 = m.s2.empty() ? nullptr : (const int32_t *)(b + off[21]);
+    d.row_order = rows_uneven ? (const int32_t *)(b + off[22]) : nullptr;
     c->version = m.version;
     out = d;
     if (upload_seconds) *upload_seconds = now_s() - t0;

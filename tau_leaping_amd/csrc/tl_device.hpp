@@ -62,6 +62,8 @@ struct DevModel {
     const double *aff_c64;   // [aff_ptr[M]] its rate
     const float *aff_c32;
     const int32_t *tile_ptr; // [n_grains + 1][N] CSR offset of each species row at channel g * kTileGrain
+    const int32_t *row_order; // [N] species by CSR row length, longest first (the row gather's schedule), or
+                              // null when every row has the same length
 };
 
 // ---- Philox4x32-10 ----

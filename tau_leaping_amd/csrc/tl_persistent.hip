@@ -72,6 +72,7 @@ struct GModel {
     gptr<const double> aff_c64;
     gptr<const float> aff_c32;
     gptr<const int32_t> s2;  // null unless some channel has three reactant molecules
+    gptr<const int32_t> row_order;  // species by CSR row length, longest first
     // column entries of a wave-uniform channel (scalar load)
     __device__ __forceinline__ int4 colq_uniform(int32_t j) const {
         const v4i v = ((cptr<v4i>)colq)[j];
@@ -118,6 +119,7 @@ __device__ __forceinline__ GModel global_model(const DevModel &d) {
     g.aff_c64 = as_global(d.aff_c64);
     g.s2 = as_global(d.s2);
     g.aff_c32 = as_global(d.aff_c32);
+    g.row_order = as_global(d.row_order);
     return g;
 }
 
@@ -801,6 +803,7 @@ __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c,
     const int32_t ng = (m.M + kTileGrain - 1) / kTileGrain;  // grains (tile_ptr has ng + 1 rows)
     const int32_t nt = (m.M + TILE - 1) / TILE;
     const int32_t rows_per_pass = GS / L;
+    const bool sorted_rows = m.row_order != nullptr && m.N > rows_per_pass;
     const int k = lane & (L - 1);
     for (int32_t t = 0; t < nt; ++t) {
         const int32_t t0 = t * TILE;
@@ -868,7 +871,10 @@ __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c,
         }
         G.sync();
         for (int32_t i0 = 0; i0 < m.N; i0 += rows_per_pass) {
-            const int32_t i = i0 + lane / L;
+            // with more than one pass, rows in order of length, longest first:
+            // the rows of one pass are of similar length
+            const int32_t pos = i0 + lane / L;
+            const int32_t i = pos >= m.N ? m.N : (sorted_rows ? m.row_order[pos] : pos);
             double pm = 0.0, ps = 0.0;
             if (i < m.N) {
                 const int32_t e0 = m.tile_ptr[(size_t)g0 * m.N + i], e1 = m.tile_ptr[(size_t)g1 * m.N + i];
