@@ -342,7 +342,7 @@ def main():
                          "hetero: the 64-model heterogeneous batch (161,120 realizations, FP32, scores on) in "
                          "one batched call; "
                          "large: toy_system_gen --toy network of 200,000 species x 1,000,000 channels, 4,096 "
-                         "realizations, t_final 1e-6, default mode (a timed choice of form)")
+                         "realizations, t_final 1e-6, default mode (AUTO: the plain form for this model)")
     ap.add_argument("--mode", default="auto", choices=["auto", "persistent", "plain"],
                     help="engine form (auto: the library's choice; persistent with --workload large runs the "
                          "kernel that keeps each realization's state in HBM)")

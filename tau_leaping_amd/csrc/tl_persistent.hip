@@ -790,7 +790,7 @@ __device__ __forceinline__ void prop_pass(const GModel &m, gptr<const R> c, cons
 // consecutive lanes (L a power of two, about GS / N) that stride its entries
 // in the tile and combine by butterfly; the row's first lane adds the tile's
 // part to mu[i], sig[i].  No atomics, and a fixed order of additions.
-template <class R, int GS, class P>
+template <class R, int GS, bool HBM, class P>
 __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c, const int32_t *xs, const P &pa,
                                                R *tile, int tile_grains, uint64_t *critq, double *mu, double *sig,
                                                double *wsum, const Grp<GS> &G, double &s, double &sc) {
@@ -803,7 +803,9 @@ __device__ __forceinline__ void prop_tau_tiled(const GModel &m, gptr<const R> c,
     const int32_t ng = (m.M + kTileGrain - 1) / kTileGrain;  // grains (tile_ptr has ng + 1 rows)
     const int32_t nt = (m.M + TILE - 1) / TILE;
     const int32_t rows_per_pass = GS / L;
-    const bool sorted_rows = m.row_order != nullptr && m.N > rows_per_pass;
+    // (not with the state in HBM: there the permuted rows would scatter the
+    // species-sum and row-offset accesses that consecutive lanes now coalesce)
+    const bool sorted_rows = !HBM && m.row_order != nullptr && m.N > rows_per_pass;
     const int k = lane & (L - 1);
     for (int32_t t = 0; t < nt; ++t) {
         const int32_t t0 = t * TILE;
@@ -1074,7 +1076,7 @@ __device__ __forceinline__ void grid_record(const ModelItem &mi, int64_t r, int 
     }
 }
 
-template <class R, int GS, bool SENS, bool EXT, class P>
+template <class R, int GS, bool SENS, bool EXT, bool HBM, class P>
 __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, int32_t *xs, int32_t *dx,
                               int32_t *snap, uint64_t *critq, double *mu, double *sig, double *wsum, R *tile, int tile_grains,
                               const P &pa) {
@@ -1161,7 +1163,7 @@ __device__ void simulate_item(const ModelItem &mi, int64_t r, const Grp<GS> &G, 
         if constexpr (GS == 64)
             prop_pass<R, GS>(m, c, xs, pa, critq, mu, sig, lane, a0, a0c);
         else
-            prop_tau_tiled<R, GS>(m, c, xs, pa, tile, tile_grains, critq, mu, sig, wsum, G, a0, a0c);
+            prop_tau_tiled<R, GS, HBM>(m, c, xs, pa, tile, tile_grains, critq, mu, sig, wsum, G, a0, a0c);
         G.sum2(a0, a0c);
         sync_();
         if (!(a0 > 0.0)) {  // absorbing state
@@ -1473,13 +1475,13 @@ __global__ __launch_bounds__(BS, WPE) void k_persistent(PersistentLaunch L) {
         const ModelItem mi = ld_uniform(&L.items[uni(lo)]);
         if constexpr (ALDS) {
             const PropStored<R> pa{reinterpret_cast<R *>(base + lay.off_a)};
-            simulate_item<R, GS, SENS, EXT>(mi, item - mi.item_base, G, xs, dx, snap, critq, mu, sig, wsum, tile,
-                                            L.tile_grains, pa);
+            simulate_item<R, GS, SENS, EXT, HBM>(mi, item - mi.item_base, G, xs, dx, snap, critq, mu, sig, wsum, tile,
+                                                 L.tile_grains, pa);
         } else {
             const GModel gm = global_model(mi.dm);
             const PropRecomputed<R> pa{gm.rs, gm.template rates<R>(), xs, EXT ? gm.s2 : nullptr};
-            simulate_item<R, GS, SENS, EXT>(mi, item - mi.item_base, G, xs, dx, snap, critq, mu, sig, wsum, tile,
-                                            L.tile_grains, pa);
+            simulate_item<R, GS, SENS, EXT, HBM>(mi, item - mi.item_base, G, xs, dx, snap, critq, mu, sig, wsum, tile,
+                                                 L.tile_grains, pa);
         }
     }
 }
