@@ -220,11 +220,11 @@ def test_large_model_with_extended_forms(cuda):
     assert np.array_equal(res.steps[same], ora["steps"][same])
 
 
-@pytest.mark.parametrize("M,form", [(12000, "persistent"), (100000, "plain"), (300000, "plain")])
+@pytest.mark.parametrize("M,form", [(12000, "persistent"), (24000, "plain"), (100000, "plain"), (300000, "plain")])
 def test_auto_form_for_large_models(cuda, M, form):
     # models too large for LDS: AUTO keeps the HBM-state persistent kernel
     # while one propensity tile holds the channels (12,000) and runs the plain
-    # form beyond (1e5 channels: 7 tiles; 3e5: a[M] in HBM), a pure function
+    # form beyond (24,000: two tiles; 1e5: 7; 3e5: a[M] in HBM), a pure function
     # of the sizes; the result is that form's, bit for bit, and its first
     # realizations are the oracle's
     import tl_oracle as O
